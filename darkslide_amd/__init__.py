@@ -1,0 +1,283 @@
+"""Python binding of the MI355X raw detail path (libdarkslide_gpu.so).
+
+The functions mirror the reference engine's stage functions
+(Syha-01/darkslide crates/core/src): ``demosaic`` (demosaic.rs:20),
+``estimate_noise`` / ``nlm`` / ``denoise`` / ``raw_details``
+(enhance.rs:10, :36, :75, :96).  Arrays follow the reference layouts:
+a Plane is an ``(h, w)`` float32 array, an Image3 an ``(h, w, 3)`` float32
+array, a CFA an ``(h, w)`` float32 (normalised) or uint16 (raw DN) array.
+
+Everything runs on the GPU through the C ABI in include/darkslide_gpu.h.
+There is no CPU fallback: without the built library or a GPU the calls raise.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+__all__ = [
+    "DarkslideError", "Context", "lib", "default_context", "RGGB", "ALGO_RCD", "ALGO_MHC",
+    "demosaic", "demosaic_u16", "estimate_noise", "nlm", "denoise", "raw_details", "synth_bayer",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdarkslide_gpu.so")
+
+ALGO_RCD = 0
+ALGO_MHC = 1
+RGGB = (0, 1, 1, 2)
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+class DarkslideError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"darkslide error {code}: {msg}")
+        self.code = code
+
+
+class ProfEntry(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("total_ms", ctypes.c_double), ("launches", ctypes.c_long)]
+
+
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_F = ctypes.c_float
+_I = ctypes.c_int
+_L = ctypes.c_long
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGS = {
+    "ds_ctx_create": [_I, ctypes.POINTER(_P)],
+    "ds_ctx_destroy": [_P],
+    "ds_last_error": [_P],
+    "ds_version": [],
+    "ds_demosaic_f32": [_P, _P, _SZ, _SZ, _P, _P, _I, _P, _P],
+    "ds_demosaic_u16": [_P, _P, _SZ, _SZ, _SZ, _P, _F, _P, _P, _I, _P, _P],
+    "ds_estimate_noise": [_P, _P, _SZ, _SZ, _P],
+    "ds_nlm": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _P],
+    "ds_denoise": [_P, _P, _SZ, _SZ, _F, _P],
+    "ds_denoise_ex": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _P],
+    "ds_raw_details": [_P, _P, _SZ, _SZ, _F, _SZ, _P],
+    "ds_dev_alloc": [_P, _SZ, ctypes.POINTER(_P)],
+    "ds_dev_free": [_P, _P],
+    "ds_memcpy_h2d": [_P, _P, _P, _SZ],
+    "ds_memcpy_d2h": [_P, _P, _P, _SZ],
+    "ds_sync": [_P],
+    "ds_demosaic_u16_dev": [_P, _P, _SZ, _SZ, _SZ, _P, _F, _P, _P, _I, _P, _P],
+    "ds_demosaic_f32_dev": [_P, _P, _SZ, _SZ, _P, _P, _I, _P, _P],
+    "ds_estimate_noise_dev": [_P, _P, _SZ, _SZ, _P],
+    "ds_nlm_dev": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _P],
+    "ds_denoise_ex_dev": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _P],
+    "ds_raw_details_dev": [_P, _P, _SZ, _SZ, _F, _SZ, _P],
+    "ds_prof_enable": [_P, _I],
+    "ds_prof_reset": [_P],
+    "ds_prof_get": [_P, _P, _I],
+    "ds_synth_bayer_host": [ctypes.c_uint64, _SZ, _SZ, _I, _F, _P, _P],
+    "ds_synth_bayer_dev": [_P, ctypes.c_uint64, _SZ, _SZ, _I, _F, _P, _P],
+}
+_RESTYPES = {"ds_ctx_destroy": None, "ds_last_error": ctypes.c_char_p, "ds_version": ctypes.c_char_p}
+
+
+def lib():
+    """Loads libdarkslide_gpu.so (built by darkslide_amd/build.py)."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise DarkslideError(-2, f"{LIB_PATH} is not built; run darkslide_amd/build.py")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, args in _SIGS.items():
+                f = getattr(L, name)
+                f.argtypes = args
+                f.restype = _RESTYPES.get(name, _I)
+            _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _pat(pattern):
+    return (ctypes.c_uint8 * 4)(*[int(v) for v in pattern])
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+class DeviceBuffer:
+    """Device allocation owned by a Context (freed with it or by .free())."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        ctx._check(lib().ds_dev_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("array larger than buffer")
+        self.ctx._check(lib().ds_memcpy_h2d(self.ctx.handle, self.ptr, _ptr(arr), arr.nbytes))
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("requested more than buffer holds")
+        self.ctx._check(lib().ds_memcpy_d2h(self.ctx.handle, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None:
+            self.ctx._check(lib().ds_dev_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+
+class Context:
+    """One GPU context (device + HIP stream) of the C ABI."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.ds_ctx_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise DarkslideError(rc, L.ds_last_error(None).decode())
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().ds_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DarkslideError(rc, lib().ds_last_error(self.handle).decode())
+        return rc
+
+    # ---- device memory helpers
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def sync(self):
+        self._check(lib().ds_sync(self.handle))
+
+    # ---- profiling
+    def prof_enable(self, on=True):
+        self._check(lib().ds_prof_enable(self.handle, 1 if on else 0))
+
+    def prof_reset(self):
+        self._check(lib().ds_prof_reset(self.handle))
+
+    def prof(self):
+        n = lib().ds_prof_get(self.handle, None, 0)
+        if n < 0:
+            self._check(n)
+        arr = (ProfEntry * max(n, 1))()
+        lib().ds_prof_get(self.handle, ctypes.cast(arr, ctypes.c_void_p), n)
+        return {arr[i].name.decode(): (arr[i].total_ms, arr[i].launches) for i in range(n)}
+
+    # ---- stage functions (host arrays in, host arrays out)
+    def demosaic(self, cfa, pattern=RGGB, mul=(1.0, 1.0, 1.0), algo=ALGO_RCD, clip=True):
+        cfa = np.ascontiguousarray(cfa, np.float32)
+        h, w = cfa.shape
+        rgb = np.empty((h, w, 3), np.float32)
+        cl = np.empty((h, w), np.float32) if clip else None
+        self._check(lib().ds_demosaic_f32(self.handle, _ptr(cfa), w, h, _pat(pattern), _f3(mul), algo, _ptr(rgb), _ptr(cl)))
+        return (rgb, cl) if clip else rgb
+
+    def demosaic_u16(self, raw, pattern=RGGB, black=512.0, white=(16383.0,) * 3, mul=(1.0, 1.0, 1.0), algo=ALGO_RCD, clip=True):
+        raw = np.ascontiguousarray(raw, np.uint16)
+        h, w = raw.shape
+        rgb = np.empty((h, w, 3), np.float32)
+        cl = np.empty((h, w), np.float32) if clip else None
+        self._check(lib().ds_demosaic_u16(self.handle, _ptr(raw), w, h, w, _pat(pattern), float(black), _f3(white), _f3(mul), algo,
+                                          _ptr(rgb), _ptr(cl)))
+        return (rgb, cl) if clip else rgb
+
+    def estimate_noise(self, plane):
+        plane = np.ascontiguousarray(plane, np.float32)
+        h, w = plane.shape
+        s = np.zeros(1, np.float32)
+        self._check(lib().ds_estimate_noise(self.handle, _ptr(plane), w, h, _ptr(s)))
+        return float(s[0])
+
+    def nlm(self, plane, h_param, search, patch):
+        plane = np.ascontiguousarray(plane, np.float32)
+        h, w = plane.shape
+        out = np.empty_like(plane)
+        self._check(lib().ds_nlm(self.handle, _ptr(plane), w, h, float(h_param), int(search), int(patch), _ptr(out)))
+        return out
+
+    def denoise(self, rgb, amount, search=3, patch=1):
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        h, w, _ = rgb.shape
+        out = np.empty_like(rgb)
+        self._check(lib().ds_denoise_ex(self.handle, _ptr(rgb), w, h, float(amount), int(search), int(patch), _ptr(out)))
+        return out
+
+    def raw_details(self, rgb, sigma, iterations):
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        h, w, _ = rgb.shape
+        out = np.empty_like(rgb)
+        self._check(lib().ds_raw_details(self.handle, _ptr(rgb), w, h, float(sigma), int(iterations), _ptr(out)))
+        return out
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def demosaic(cfa, pattern=RGGB, mul=(1.0, 1.0, 1.0), algo=ALGO_RCD):
+    """demosaic.rs:20 demosaic(cfa, mul, Quality::Full) -> (rgb, clip)."""
+    return default_context().demosaic(cfa, pattern, mul, algo, clip=True)
+
+
+def demosaic_u16(raw, pattern=RGGB, black=512.0, white=(16383.0,) * 3, mul=(1.0, 1.0, 1.0), algo=ALGO_RCD):
+    """CFA normalisation (decode.rs:291-306) fused with demosaic -> (rgb, clip)."""
+    return default_context().demosaic_u16(raw, pattern, black, white, mul, algo, clip=True)
+
+
+def estimate_noise(plane):
+    """enhance.rs:10 estimate_noise."""
+    return default_context().estimate_noise(plane)
+
+
+def nlm(plane, h_param, search, patch):
+    """enhance.rs:36 nlm."""
+    return default_context().nlm(plane, h_param, search, patch)
+
+
+def denoise(rgb, amount, search=3, patch=1):
+    """enhance.rs:75 denoise (search / patch default to the reference's 3, 1)."""
+    return default_context().denoise(rgb, amount, search, patch)
+
+
+def raw_details(rgb, sigma, iterations):
+    """enhance.rs:96 raw_details."""
+    return default_context().raw_details(rgb, sigma, iterations)
+
+
+def synth_bayer(seed, w, h, bits=14, iso=1.0, pattern=RGGB):
+    """Seeded synthetic Bayer frame, generated on the host (uint16 (h, w))."""
+    out = np.empty((h, w), np.uint16)
+    rc = lib().ds_synth_bayer_host(int(seed), int(w), int(h), int(bits), float(iso), _pat(pattern), _ptr(out))
+    if rc != 0:
+        raise DarkslideError(rc, "invalid synthetic frame arguments")
+    return out

@@ -1,0 +1,662 @@
+// C ABI of the raw detail path (include/darkslide_gpu.h): contexts,
+// device workspace, host<->device staging, stage timing and the entry
+// points that stand in for the reference's demosaic / enhance functions.
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "darkslide_gpu.h"
+#include "demosaic.h"
+#include "denoise.h"
+#include "nlm.h"
+#include "rl.h"
+#include "synth.h"
+
+#pragma clang fp contract(off)
+
+namespace ds {
+
+// ---- Oklab matrices exactly as color.rs builds them (f32 entries; mul_mm
+// and invert accumulate in f64).
+namespace {
+typedef float M3[3][3];
+void mul_mm(const M3 a, const M3 b, M3 o) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0.0;
+            for (int k = 0; k < 3; k++) s += (double)a[i][k] * (double)b[k][j];
+            o[i][j] = (float)s;
+        }
+}
+void invert(const M3 mi, M3 o) {
+    double m[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) m[i][j] = mi[i][j];
+    double det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                 m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    if (std::fabs(det) < 1e-12) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) o[i][j] = i == j ? 1.0f : 0.0f;
+        return;
+    }
+    double inv = 1.0 / det;
+    double r[3][3] = {
+        {(m[1][1] * m[2][2] - m[2][1] * m[1][2]) * inv, (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * inv, (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv},
+        {(m[1][2] * m[2][0] - m[1][0] * m[2][2]) * inv, (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * inv, (m[1][0] * m[0][2] - m[0][0] * m[1][2]) * inv},
+        {(m[1][0] * m[2][1] - m[2][0] * m[1][1]) * inv, (m[2][0] * m[0][1] - m[0][0] * m[2][1]) * inv, (m[0][0] * m[1][1] - m[1][0] * m[0][1]) * inv},
+    };
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) o[i][j] = (float)r[i][j];
+}
+void xy_to_xyz(const float xy[2], float o[3]) {
+    float x = xy[0], y = std::fmax(xy[1], 1e-6f);
+    o[0] = x / y;
+    o[1] = 1.0f;
+    o[2] = (1.0f - x - y) / y;
+}
+}  // namespace
+
+OklabM build_oklab() {
+    const float d50[2] = {0.34567f, 0.35850f}, d65[2] = {0.31271f, 0.32902f};            // color.rs:62-63
+    const M3 brad = {{0.8951f, 0.2664f, -0.1614f}, {-0.7502f, 1.7135f, 0.0367f}, {0.0389f, -0.0685f, 1.0296f}};  // color.rs:79
+    const float pr[2] = {0.734699f, 0.265301f}, pg[2] = {0.159597f, 0.840403f}, pb[2] = {0.036598f, 0.000105f};  // color.rs:129
+    const M3 m1 = {{0.818933f, 0.36186674f, -0.12885971f}, {0.032984544f, 0.9293119f, 0.03614564f}, {0.0482003f, 0.26436627f, 0.6338517f}};
+    const M3 m2 = {{0.21045426f, 0.7936178f, -0.004072047f}, {1.9779985f, -2.4285922f, 0.4505937f}, {0.025904037f, 0.78277177f, -0.80867577f}};
+    // bradford(D50, D65) (color.rs:82-87)
+    float xs[3], xd[3], s[3], d[3];
+    xy_to_xyz(d50, xs);
+    xy_to_xyz(d65, xd);
+    mul_mv(brad, xs, s);
+    mul_mv(brad, xd, d);
+    M3 scale = {{d[0] / s[0], 0, 0}, {0, d[1] / s[1], 0}, {0, 0, d[2] / s[2]}};
+    M3 binv, sb, bradm;
+    invert(brad, binv);
+    mul_mm(scale, brad, sb);
+    mul_mm(binv, sb, bradm);
+    // ProPhoto -> XYZ D50 (color.rs:90-98, white = D50)
+    float vr[3], vg[3], vb[3], wx[3], sc[3];
+    xy_to_xyz(pr, vr);
+    xy_to_xyz(pg, vg);
+    xy_to_xyz(pb, vb);
+    M3 m = {{vr[0], vg[0], vb[0]}, {vr[1], vg[1], vb[1]}, {vr[2], vg[2], vb[2]}};
+    M3 minv;
+    invert(m, minv);
+    xy_to_xyz(d50, wx);
+    mul_mv(minv, wx, sc);
+    M3 dg = {{sc[0], 0, 0}, {0, sc[1], 0}, {0, 0, sc[2]}};
+    M3 xyz, bx;
+    mul_mm(m, dg, xyz);
+    mul_mm(bradm, xyz, bx);
+    OklabM ok;
+    mul_mm(m1, bx, ok.to_lms);  // color.rs:382-385
+    invert(ok.to_lms, ok.from_lms);
+    std::memcpy(ok.m2, m2, sizeof(M3));
+    invert(m2, ok.m2_inv);
+    return ok;
+}
+
+namespace {
+
+__global__ void k_synth(const SynthScene* __restrict__ sp, uint16_t* __restrict__ out) {
+    const SynthScene& s = *sp;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)s.w * (size_t)s.h;
+    if (i >= n) return;
+    out[i] = synth_dn(s, (int)(i % (size_t)s.w), (int)(i / (size_t)s.w));
+}
+
+// plain fill; kept as a kernel so the enqueued work stays graph-capturable
+__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+}  // namespace ds
+
+using namespace ds;
+
+// ---------------------------------------------------------------------------
+// context
+
+namespace {
+
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+enum Slot { S_IN = 0, S_OUT, S_CLIP, S_WS, S_L, S_A, S_B, S_LF, S_SIGMA, S_NOISE, S_SCENE, S_COUNT };
+
+struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct ds_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    std::string err;
+    Buf slots[S_COUNT];
+    std::vector<void*> user_allocs;
+    OklabM ok;
+    bool prof = false;
+    std::vector<Pending> pending;
+    std::map<std::string, std::pair<double, long>> totals;
+};
+
+namespace {
+
+int fail(ds_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c)
+        c->err = buf;
+    else
+        g_create_error = buf;
+    return code;
+}
+
+int hip_fail(ds_ctx* c, hipError_t e, const char* what) {
+    return fail(c, e == hipErrorOutOfMemory ? DS_ERR_ALLOC : DS_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, expr, what)                          \
+    do {                                                 \
+        hipError_t e_ = (expr);                          \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
+    } while (0)
+
+void* slot(ds_ctx* c, Slot s, size_t bytes, hipError_t* err) {
+    *err = hipSuccess;
+    Buf& b = c->slots[s];
+    if (b.bytes >= bytes && b.p) return b.p;
+    if (b.p) {
+        hipStreamSynchronize(c->stream);
+        hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    *err = hipMalloc(&b.p, want);
+    if (*err != hipSuccess) {
+        b.p = nullptr;
+        return nullptr;
+    }
+    b.bytes = want;
+    return b.p;
+}
+
+#define SLOT(ctx, var, type, s, bytes)                                   \
+    type var;                                                            \
+    {                                                                    \
+        hipError_t e_;                                                   \
+        var = (type)slot(ctx, s, bytes, &e_);                            \
+        if (!var) return hip_fail(ctx, e_, "device workspace allocation"); \
+    }
+
+// Launch wrapper: with profiling on, brackets the launch with events.
+template <class F>
+int run(ds_ctx* c, const char* name, F&& f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c->prof) {
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, c->stream);
+    }
+    hipError_t e = f();
+    if (c->prof) {
+        hipEventRecord(b, c->stream);
+        c->pending.push_back({name, a, b});
+    }
+    if (e != hipSuccess) return fail(c, DS_ERR_LAUNCH, "%s: %s", name, hipGetErrorString(e));
+    return DS_OK;
+}
+
+void collect(ds_ctx* c) {
+    if (c->pending.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        auto& t = c->totals[p.name];
+        t.first += ms;
+        t.second += 1;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    c->pending.clear();
+}
+
+bool valid_bayer(const uint8_t* p) {
+    if (!p) return false;
+    for (int i = 0; i < 4; i++)
+        if (p[i] > 2) return false;
+    bool diag03 = p[0] == 1 && p[3] == 1 && ((p[1] == 0 && p[2] == 2) || (p[1] == 2 && p[2] == 0));
+    bool diag12 = p[1] == 1 && p[2] == 1 && ((p[0] == 0 && p[3] == 2) || (p[0] == 2 && p[3] == 0));
+    return diag03 || diag12;
+}
+
+int check_dims(ds_ctx* c, size_t w, size_t h) {
+    if (!c) return DS_ERR_INVALID;
+    if (w > (size_t)INT32_MAX / 4 || h > (size_t)INT32_MAX / 4 || (w && h && w * h > ((size_t)1 << 34)))
+        return fail(c, DS_ERR_INVALID, "image %zux%zu too large", w, h);
+    return DS_OK;
+}
+
+int set_device(ds_ctx* c) {
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    return DS_OK;
+}
+
+// ---- enqueue-only implementations (device pointers)
+
+int demosaic_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t* pat, float black,
+                 const float* white, const float* mul, int algo, float* rgb, float* clip) {
+    if (!valid_bayer(pat)) return fail(c, DS_ERR_INVALID, "pattern must be a Bayer 2x2 with a green checkerboard");
+    if (algo != DS_ALGO_RCD && algo != DS_ALGO_MHC) return fail(c, DS_ERR_INVALID, "unknown demosaic algorithm %d", algo);
+    if (!mul || !rgb || (!cfa && !raw)) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (w == 0 || h == 0) return DS_OK;
+    if (stride < w) return fail(c, DS_ERR_INVALID, "stride %zu < width %zu", stride, w);
+    DemosaicArgs a{};
+    a.cfa_f32 = cfa;
+    a.raw_u16 = raw;
+    a.stride = stride;
+    a.w = w;
+    a.h = h;
+    std::memcpy(a.pattern, pat, 4);
+    a.black = black;
+    for (int k = 0; k < 3; k++) {
+        a.white[k] = white ? white[k] : 1.0f;
+        a.mul[k] = mul[k];
+    }
+    a.algo = algo;
+    a.rgb = rgb;
+    a.clip = clip;
+    return run(c, algo == DS_ALGO_RCD && w >= 16 && h >= 16 ? "demosaic_rcd" : "demosaic_mhc", [&] { return launch_demosaic(a, c->stream); });
+}
+
+int noise_dev(ds_ctx* c, const float* plane, size_t w, size_t h, float* sigma_dev) {
+    size_t cnt = noise_sample_count(w, h);
+    SLOT(c, scratch, uint32_t*, S_NOISE, (cnt ? cnt : 1) * sizeof(uint32_t));
+    return run(c, "estimate_noise", [&] { return launch_estimate_noise(plane, w, h, scratch, sigma_dev, c->stream); });
+}
+
+int zero_fill(ds_ctx* c, float* out, size_t n) {
+    return run(c, "fill_zero", [&] {
+        hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, out, n, 0.0f);
+        return hipGetLastError();
+    });
+}
+
+int nlm_dev(ds_ctx* c, const float* in, size_t w, size_t h, NlmH hp, long search, size_t patch, float* out) {
+    if (w == 0 || h == 0) return DS_OK;
+    // (-S..=S) is empty for S < 0: no weights at all, so acc / max(0, 1e-12) = 0
+    if (search < 0) return zero_fill(c, out, w * h);
+    if (patch > 64) return fail(c, DS_ERR_INVALID, "patch radius %zu too large", patch);
+    if (search > 1024) return fail(c, DS_ERR_INVALID, "search radius %ld too large", search);
+    return run(c, "nlm", [&] { return launch_nlm(in, out, w, h, hp, search, patch, c->stream); });
+}
+
+int denoise_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* out) {
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    SLOT(c, L, float*, S_L, n * 4);
+    SLOT(c, A, float*, S_A, n * 4);
+    SLOT(c, B, float*, S_B, n * 4);
+    SLOT(c, LF, float*, S_LF, n * 4);
+    SLOT(c, sig, float*, S_SIGMA, 64);
+    SLOT(c, ws, float*, S_WS, 10 * n * 4);
+    int rc = run(c, "oklab_fwd", [&] { return launch_oklab_fwd(rgb, n, c->ok, L, A, B, c->stream); });
+    if (rc) return rc;
+    if ((rc = noise_dev(c, L, w, h, sig))) return rc;
+    // enhance.rs:82-88
+    const float k = amount / 100.0f;
+    NlmH hp{0.0f, sig, 0.6f + 1.6f * k};
+    if ((rc = nlm_dev(c, L, w, h, hp, search, patch, LF))) return rc;
+    const float m = 0.5f + 0.5f * k;
+    float rf = std::fmax(((float)(w > h ? w : h) / 600.0f) * (1.0f + 2.0f * k), 2.0f);
+    size_t r = (size_t)rf;
+    return run(c, "guided_filter", [&] { return launch_guided_pair(L, LF, A, B, m, r, sig, c->ok, ws, out, w, h, c->stream); });
+}
+
+int raw_details_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float sigma, size_t iters, float* out) {
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    if (!(sigma >= 0.0f) || sigma > 1000.0f) return fail(c, DS_ERR_INVALID, "sigma %g out of range", (double)sigma);
+    SLOT(c, ws, float*, S_WS, rl_workspace_floats(w, h) * 4);
+    return run(c, "raw_details", [&] { return launch_raw_details(rgb, w, h, sigma, iters, ws, out, c->stream); });
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// exported API
+
+extern "C" {
+
+const char* ds_version(void) { return "darkslide-mi355x 0.1 (gfx950)"; }
+
+int ds_ctx_create(int device, ds_ctx** out) {
+    if (!out) return fail(nullptr, DS_ERR_INVALID, "out is null");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(nullptr, DS_ERR_DEVICE, "no HIP device: %s", hipGetErrorString(e));
+    if (device < 0 || device >= n) return fail(nullptr, DS_ERR_INVALID, "device %d out of range (%d devices)", device, n);
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, DS_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+    ds_ctx* c = new ds_ctx();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(nullptr, DS_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    c->ok = build_oklab();
+    *out = c;
+    return DS_OK;
+}
+
+void ds_ctx_destroy(ds_ctx* c) {
+    if (!c) return;
+    {
+        std::lock_guard<std::recursive_mutex> g(c->mu);
+        hipSetDevice(c->device);
+        hipStreamSynchronize(c->stream);
+        for (auto& p : c->pending) {
+            hipEventDestroy(p.a);
+            hipEventDestroy(p.b);
+        }
+        for (auto& b : c->slots)
+            if (b.p) hipFree(b.p);
+        for (void* p : c->user_allocs) hipFree(p);
+        hipStreamDestroy(c->stream);
+    }
+    delete c;
+}
+
+const char* ds_last_error(const ds_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+#define LOCK(c)                                       \
+    if (!c) return DS_ERR_INVALID;                    \
+    std::lock_guard<std::recursive_mutex> guard_(c->mu); \
+    c->err.clear();                                   \
+    {                                                 \
+        int rc_ = set_device(c);                      \
+        if (rc_) return rc_;                          \
+    }
+
+int ds_dev_alloc(ds_ctx* c, size_t bytes, void** dptr) {
+    LOCK(c);
+    if (!dptr) return fail(c, DS_ERR_INVALID, "dptr is null");
+    HIPCHK(c, hipMalloc(dptr, bytes ? bytes : 1), "hipMalloc");
+    c->user_allocs.push_back(*dptr);
+    return DS_OK;
+}
+
+int ds_dev_free(ds_ctx* c, void* p) {
+    LOCK(c);
+    for (size_t i = 0; i < c->user_allocs.size(); i++)
+        if (c->user_allocs[i] == p) {
+            HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            HIPCHK(c, hipFree(p), "hipFree");
+            c->user_allocs.erase(c->user_allocs.begin() + (long)i);
+            return DS_OK;
+        }
+    return fail(c, DS_ERR_INVALID, "pointer was not allocated by ds_dev_alloc on this context");
+}
+
+int ds_memcpy_h2d(ds_ctx* c, void* dst, const void* src, size_t bytes) {
+    LOCK(c);
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream), "hipMemcpy H2D");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_memcpy_d2h(ds_ctx* c, void* dst, const void* src, size_t bytes) {
+    LOCK(c);
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "hipMemcpy D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_sync(ds_ctx* c) {
+    LOCK(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_demosaic_u16_dev(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
+                        const float white[3], const float mul[3], int algo, float* rgb_out, float* clip_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!white) return fail(c, DS_ERR_INVALID, "white is null");
+    return demosaic_dev(c, nullptr, raw, w, h, stride, pattern, black, white, mul, algo, rgb_out, clip_out);
+}
+
+int ds_demosaic_f32_dev(ds_ctx* c, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], int algo,
+                        float* rgb_out, float* clip_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    return demosaic_dev(c, cfa, nullptr, w, h, w, pattern, 0.0f, nullptr, mul, algo, rgb_out, clip_out);
+}
+
+int ds_estimate_noise_dev(ds_ctx* c, const float* plane, size_t w, size_t h, float* sigma_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!sigma_out || (!plane && w * h)) return fail(c, DS_ERR_INVALID, "null pointer");
+    return noise_dev(c, plane, w, h, sigma_out);
+}
+
+int ds_nlm_dev(ds_ctx* c, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if ((!in || !out) && w * h) return fail(c, DS_ERR_INVALID, "null pointer");
+    return nlm_dev(c, in, w, h, NlmH{h_param, nullptr, 1.0f}, search, patch, out);
+}
+
+int ds_denoise_ex_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* rgb_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if ((!rgb || !rgb_out) && w * h) return fail(c, DS_ERR_INVALID, "null pointer");
+    return denoise_dev(c, rgb, w, h, amount, search, patch, rgb_out);
+}
+
+int ds_raw_details_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float sigma, size_t iterations, float* rgb_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if ((!rgb || !rgb_out) && w * h) return fail(c, DS_ERR_INVALID, "null pointer");
+    return raw_details_dev(c, rgb, w, h, sigma, iterations, rgb_out);
+}
+
+// ---- synchronous host-pointer entry points
+
+int ds_demosaic_f32(ds_ctx* c, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], int algo,
+                    float* rgb_out, float* clip_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!cfa || !rgb_out || !mul) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    SLOT(c, d_in, float*, S_IN, n * 4);
+    SLOT(c, d_rgb, float*, S_OUT, n * 12);
+    float* d_clip = nullptr;
+    if (clip_out) {
+        SLOT(c, dc, float*, S_CLIP, n * 4);
+        d_clip = dc;
+    }
+    HIPCHK(c, hipMemcpyAsync(d_in, cfa, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = demosaic_dev(c, d_in, nullptr, w, h, w, pattern, 0.0f, nullptr, mul, algo, d_rgb, d_clip)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb_out, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (clip_out) HIPCHK(c, hipMemcpyAsync(clip_out, d_clip, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_demosaic_u16(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
+                    const float white[3], const float mul[3], int algo, float* rgb_out, float* clip_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!raw || !rgb_out || !mul || !white) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (stride < w) return fail(c, DS_ERR_INVALID, "stride %zu < width %zu", stride, w);
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    SLOT(c, d_raw, uint16_t*, S_IN, h * stride * 2);
+    SLOT(c, d_rgb, float*, S_OUT, n * 12);
+    float* d_clip = nullptr;
+    if (clip_out) {
+        SLOT(c, dc, float*, S_CLIP, n * 4);
+        d_clip = dc;
+    }
+    HIPCHK(c, hipMemcpyAsync(d_raw, raw, ((h - 1) * stride + w) * 2, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = demosaic_dev(c, nullptr, d_raw, w, h, stride, pattern, black, white, mul, algo, d_rgb, d_clip)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb_out, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (clip_out) HIPCHK(c, hipMemcpyAsync(clip_out, d_clip, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_estimate_noise(ds_ctx* c, const float* plane, size_t w, size_t h, float* sigma_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!sigma_out || (!plane && w * h)) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t n = w * h;
+    SLOT(c, d_in, float*, S_IN, (n ? n : 1) * 4);
+    SLOT(c, d_sig, float*, S_SIGMA, 64);
+    if (n) HIPCHK(c, hipMemcpyAsync(d_in, plane, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = noise_dev(c, d_in, w, h, d_sig)) return rc;
+    HIPCHK(c, hipMemcpyAsync(sigma_out, d_sig, 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_nlm(ds_ctx* c, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    if (!in || !out) return fail(c, DS_ERR_INVALID, "null pointer");
+    SLOT(c, d_in, float*, S_IN, n * 4);
+    SLOT(c, d_out, float*, S_OUT, n * 4);
+    HIPCHK(c, hipMemcpyAsync(d_in, in, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = nlm_dev(c, d_in, w, h, NlmH{h_param, nullptr, 1.0f}, search, patch, d_out)) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, d_out, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_denoise_ex(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* rgb_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    if (!rgb || !rgb_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    SLOT(c, d_in, float*, S_IN, n * 12);
+    SLOT(c, d_out, float*, S_OUT, n * 12);
+    HIPCHK(c, hipMemcpyAsync(d_in, rgb, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = denoise_dev(c, d_in, w, h, amount, search, patch, d_out)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb_out, d_out, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_denoise(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, float* rgb_out) {
+    return ds_denoise_ex(c, rgb, w, h, amount, 3, 1, rgb_out);  // enhance.rs:84
+}
+
+int ds_raw_details(ds_ctx* c, const float* rgb, size_t w, size_t h, float sigma, size_t iterations, float* rgb_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    if (!rgb || !rgb_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    SLOT(c, d_in, float*, S_IN, n * 12);
+    SLOT(c, d_out, float*, S_OUT, n * 12);
+    HIPCHK(c, hipMemcpyAsync(d_in, rgb, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = raw_details_dev(c, d_in, w, h, sigma, iterations, d_out)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb_out, d_out, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+// ---- profiling
+
+int ds_prof_enable(ds_ctx* c, int on) {
+    LOCK(c);
+    collect(c);
+    c->prof = on != 0;
+    return DS_OK;
+}
+
+int ds_prof_reset(ds_ctx* c) {
+    LOCK(c);
+    collect(c);
+    c->totals.clear();
+    return DS_OK;
+}
+
+int ds_prof_get(ds_ctx* c, ds_prof_entry* out, int cap) {
+    LOCK(c);
+    collect(c);
+    int i = 0;
+    for (auto& kv : c->totals) {
+        if (out && i < cap) {
+            std::memset(&out[i], 0, sizeof(ds_prof_entry));
+            std::strncpy(out[i].name, kv.first.c_str(), sizeof(out[i].name) - 1);
+            out[i].total_ms = kv.second.first;
+            out[i].launches = kv.second.second;
+        }
+        i++;
+    }
+    return i;
+}
+
+// ---- synthetic frames
+
+int ds_synth_bayer_host(uint64_t seed, size_t w, size_t h, int bits, float iso, const uint8_t pattern[4], uint16_t* raw_out) {
+    if (!raw_out || !valid_bayer(pattern) || (bits != 12 && bits != 14) || !(iso >= 1.0f)) return DS_ERR_INVALID;
+    if (w > (1u << 20) || h > (1u << 20)) return DS_ERR_INVALID;
+    SynthScene* s = new SynthScene;
+    synth_scene_init(s, seed, (int)w, (int)h, bits, iso, pattern);
+    for (size_t y = 0; y < h; y++)
+        for (size_t x = 0; x < w; x++) raw_out[y * w + x] = synth_dn(*s, (int)x, (int)y);
+    delete s;
+    return DS_OK;
+}
+
+int ds_synth_bayer_dev(ds_ctx* c, uint64_t seed, size_t w, size_t h, int bits, float iso, const uint8_t pattern[4], uint16_t* raw_out) {
+    LOCK(c);
+    if (!raw_out || !valid_bayer(pattern) || (bits != 12 && bits != 14) || !(iso >= 1.0f) || w > (1u << 20) || h > (1u << 20))
+        return fail(c, DS_ERR_INVALID, "invalid synthetic frame arguments");
+    SynthScene s;
+    synth_scene_init(&s, seed, (int)w, (int)h, bits, iso, pattern);
+    size_t n = w * h;
+    if (n == 0) return DS_OK;
+    SLOT(c, d_scene, SynthScene*, S_SCENE, sizeof(SynthScene));
+    HIPCHK(c, hipMemcpyAsync(d_scene, &s, sizeof(SynthScene), hipMemcpyHostToDevice, c->stream), "H2D");
+    int rc = run(c, "synth_bayer", [&] {
+        hipLaunchKernelGGL(k_synth, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d_scene, raw_out);
+        return hipGetLastError();
+    });
+    // the scene lives on this stack frame until the copy has been consumed
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return rc;
+}
+
+}  // extern "C"

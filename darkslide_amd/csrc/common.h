@@ -1,0 +1,46 @@
+// Shared helpers for the gfx950 kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace ds {
+
+// Bayer colour at (x, y) for a 2x2 pattern stored as pat[(y&1)*2 + (x&1)]
+// (0 = R, 1 = G, 2 = B), as CfaPattern::color_at does for a 2x2 pattern
+// (decode.rs:26-28).
+struct Bayer {
+    uint8_t c[4];
+    __host__ __device__ __forceinline__ int at(long x, long y) const { return c[((y & 1) << 1) | (x & 1)]; }
+    // parity (x + y) & 1 of the non-green sites; Bayer greens form a checkerboard
+    __host__ __device__ __forceinline__ int rb_parity() const { return c[0] == 1 ? 1 : 0; }
+};
+
+__host__ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// f32::max / f32::clamp semantics (fmaxf returns the non-NaN operand)
+__device__ __forceinline__ float fmaxr(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ float clampr(float v, float lo, float hi) {
+    if (v < lo) v = lo;
+    if (v > hi) v = hi;
+    return v;
+}
+
+// Oklab matrices (from / to linear ProPhoto), built on the host exactly as
+// color.rs:382-418 builds them, and passed to kernels by value.
+struct OklabM {
+    float to_lms[3][3];
+    float from_lms[3][3];
+    float m2[3][3];
+    float m2_inv[3][3];
+};
+
+__host__ __device__ __forceinline__ void mul_mv(const float m[3][3], const float v[3], float o[3]) {
+    o[0] = m[0][0] * v[0] + m[0][1] * v[1] + m[0][2] * v[2];
+    o[1] = m[1][0] * v[0] + m[1][1] * v[1] + m[1][2] * v[2];
+    o[2] = m[2][0] * v[0] + m[2][1] * v[1] + m[2][2] * v[2];
+}
+
+}  // namespace ds

@@ -1,0 +1,27 @@
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "darkslide_gpu.h"
+
+namespace ds {
+
+struct DemosaicArgs {
+    const float* cfa_f32;     // normalised CFA (device), or null
+    const uint16_t* raw_u16;  // raw DN (device), or null
+    size_t stride;            // elements per input row
+    size_t w, h;
+    uint8_t pattern[4];
+    float black;
+    float white[3];
+    float mul[3];
+    int algo;     // DS_ALGO_RCD / DS_ALGO_MHC
+    float* rgb;   // interleaved out (device)
+    float* clip;  // clip plane out (device) or null
+};
+
+hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream);
+
+}  // namespace ds

@@ -1,0 +1,427 @@
+// Bayer demosaic for gfx950: RCD (demosaic.rs:145-344) with MHC frame
+// borders (demosaic.rs:89-124), the clip plane (demosaic.rs:29-42) and the
+// CFA normalisation of decode.rs:291-306 fused into one tile kernel.
+//
+// One workgroup owns a TX x TY output tile.  It loads cfa*mul for the tile
+// plus a 10-pixel halo (the dependency radius of RCD's last step on its
+// input) into LDS with clamped coordinates, then runs every RCD stage on
+// LDS, each over exactly the halo the later stages read.  Every stage keeps
+// the reference's per-stage frame border, so a pixel gets the same
+// computation (RCD or MHC) it gets in the reference.  Arithmetic keeps the
+// reference's operation order and is compiled without FMA contraction.
+//
+// Tile-local coordinates (tx, ty) run from -halo to T+halo; the global
+// pixel is (ox + tx, oy + ty).  LDS slot (tx, ty) of the input holds the
+// value at the clamped global coordinate, which is what MHC's clamp-to-edge
+// addressing reads.
+
+#include "common.h"
+#include "demosaic.h"
+
+#pragma clang fp contract(off)
+
+namespace ds {
+
+namespace {
+
+constexpr float kClip = 0.995f;  // demosaic.rs:17
+constexpr float kEps = 1e-5f;    // demosaic.rs:155
+constexpr int kBorder = 4;       // demosaic.rs:158
+
+template <int TX, int TY, int H>
+struct Tile {
+    static constexpr int W = TX + 2 * H;
+    static constexpr int R = TY + 2 * H;
+    static constexpr int N = W * R;
+    float* p;
+    __device__ __forceinline__ float& operator()(int x, int y) const { return p[(y + H) * W + (x + H)]; }
+};
+
+struct CfaSrc {
+    // exactly one of f32 / u16 is set
+    const float* f32;
+    const uint16_t* u16;
+    size_t stride;  // elements per row
+    float black;
+    float white[3];
+    float mul[3];
+    Bayer pat;
+    int w, h;
+
+    // normalised CFA value at an in-image pixel: decode.rs:302-303
+    __device__ __forceinline__ float cfa(int x, int y, int c) const {
+        if (f32) return f32[(size_t)y * stride + x];
+        float raw = (float)u16[(size_t)y * stride + x];
+        return (raw - black) / fmaxr(white[c] - black, 1.0f);
+    }
+};
+
+// Malvar-He-Cutler at tile position (x, y) for output channel ch, reading
+// cm = cfa*mul (not floored) with clamp-to-edge via the clamped LDS tile.
+// demosaic.rs:99-123, same operand order.
+template <class T>
+__device__ __forceinline__ float mhc_at(const T& cm, int x, int y, int col, int row_c, int ch) {
+    auto p = [&](int dx, int dy) { return cm(x + dx, y + dy); };
+    float v = cm(x, y);
+    float out;
+    if (col == ch) {
+        out = v;
+    } else if (col == 1) {
+        // R/B at G: horizontal neighbours carry row_c
+        bool want_row = (ch == row_c);
+        if (want_row) {
+            out = (5.0f * v + 4.0f * (p(-1, 0) + p(1, 0)) - (p(-2, 0) + p(2, 0) + p(-1, -1) + p(1, -1) + p(-1, 1) + p(1, 1)) +
+                   0.5f * (p(0, -2) + p(0, 2))) /
+                  8.0f;
+        } else {
+            out = (5.0f * v + 4.0f * (p(0, -1) + p(0, 1)) - (p(0, -2) + p(0, 2) + p(-1, -1) + p(1, -1) + p(-1, 1) + p(1, 1)) +
+                   0.5f * (p(-2, 0) + p(2, 0))) /
+                  8.0f;
+        }
+    } else if (ch == 1) {
+        out = (4.0f * v + 2.0f * (p(0, -1) + p(-1, 0) + p(1, 0) + p(0, 1)) - (p(0, -2) + p(-2, 0) + p(2, 0) + p(0, 2))) / 8.0f;
+    } else {
+        out = (6.0f * v + 2.0f * (p(-1, -1) + p(1, -1) + p(-1, 1) + p(1, 1)) - 1.5f * (p(0, -2) + p(-2, 0) + p(2, 0) + p(0, 2))) / 8.0f;
+    }
+    return fmaxr(out, 0.0f);
+}
+
+__device__ __forceinline__ float sq(float v) { return v * v; }
+
+template <class T>
+__device__ __forceinline__ float disc(const T& d, int x, int y) {
+    float central = d(x, y);
+    float neigh = 0.25f * (d(x - 1, y - 1) + d(x + 1, y - 1) + d(x - 1, y + 1) + d(x + 1, y + 1));
+    return fabsf(0.5f - central) < fabsf(0.5f - neigh) ? neigh : central;
+}
+
+// Halo of each stage, in pixels around the output tile.
+constexpr int H_CM = 10;   // cfa*mul
+constexpr int H_BVH = 7;   // bv, bh   (later bp, bq at 5)
+constexpr int H_VH = 6;    // vh
+constexpr int H_LPF = 7;   // lpf
+constexpr int H_G = 5;     // green
+constexpr int H_PQ = 4;    // pq
+constexpr int H_RB = 3;    // r, b after step 4.3
+constexpr int H_FLAG = 1;  // clip flags
+
+template <int TX, int TY>
+struct RcdLds {
+    float cm[Tile<TX, TY, H_CM>::N];
+    // bv/bh, then bp/bq, then r43/b43 share this space
+    float scratch_a[2 * Tile<TX, TY, H_BVH>::N];
+    float vh[Tile<TX, TY, H_VH>::N];
+    float lpf[Tile<TX, TY, H_LPF>::N];
+    float g[Tile<TX, TY, H_G>::N];
+    float pq[Tile<TX, TY, H_PQ>::N];
+    uint8_t flag[Tile<TX, TY, H_FLAG>::N];
+};
+
+template <int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_rcd(CfaSrc src, float* __restrict__ rgb, float* __restrict__ clip) {
+    static_assert((TX % 2) == 0 && (TY % 2) == 0, "tile origin must keep Bayer parity");
+    static_assert(2 * Tile<TX, TY, H_RB>::N <= 2 * Tile<TX, TY, H_BVH>::N, "r43/b43 alias");
+    __shared__ RcdLds<TX, TY> L;
+    const int tid = threadIdx.x;
+    const int ox = blockIdx.x * TX, oy = blockIdx.y * TY;
+    const int w = src.w, h = src.h;
+    const Bayer pat = src.pat;
+
+    Tile<TX, TY, H_CM> cm{L.cm};
+    Tile<TX, TY, H_BVH> bv{L.scratch_a}, bh{L.scratch_a + Tile<TX, TY, H_BVH>::N};
+    Tile<TX, TY, H_G> bp{L.scratch_a}, bq{L.scratch_a + Tile<TX, TY, H_G>::N};
+    Tile<TX, TY, H_RB> r43{L.scratch_a}, b43{L.scratch_a + Tile<TX, TY, H_RB>::N};
+    Tile<TX, TY, H_VH> vh{L.vh};
+    Tile<TX, TY, H_LPF> lpf{L.lpf};
+    Tile<TX, TY, H_G> gg{L.g};
+    Tile<TX, TY, H_PQ> pq{L.pq};
+
+    auto in_img = [&](int gx, int gy) { return gx >= 0 && gx < w && gy >= 0 && gy < h; };
+    auto in_box = [&](int gx, int gy, int b) { return gx >= b && gx < w - b && gy >= b && gy < h - b; };
+    auto cval = [&](int x, int y) { return fmaxr(cm(x, y), 0.0f); };
+
+    // ---- stage 0: load cfa*mul (and clip flags) with clamped coordinates
+    for (int i = tid; i < Tile<TX, TY, H_CM>::N; i += NT) {
+        int lx = i % Tile<TX, TY, H_CM>::W, ly = i / Tile<TX, TY, H_CM>::W;
+        int tx = lx - H_CM, ty = ly - H_CM;
+        int gx = clampi(ox + tx, 0, w - 1), gy = clampi(oy + ty, 0, h - 1);
+        int c = pat.at(gx, gy);
+        float v = src.cfa(gx, gy, c);
+        cm(tx, ty) = v * src.mul[c];
+        if (clip && tx >= -H_FLAG && tx < TX + H_FLAG && ty >= -H_FLAG && ty < TY + H_FLAG)
+            L.flag[(ty + H_FLAG) * Tile<TX, TY, H_FLAG>::W + (tx + H_FLAG)] = v >= kClip ? 1 : 0;
+    }
+    __syncthreads();
+
+    // ---- stage 1: bv / bh (demosaic.rs:160-172) over halo 7
+    for (int i = tid; i < Tile<TX, TY, H_BVH>::N; i += NT) {
+        int tx = i % Tile<TX, TY, H_BVH>::W - H_BVH, ty = i / Tile<TX, TY, H_BVH>::W - H_BVH;
+        int gx = ox + tx, gy = oy + ty;
+        float v = 0.0f, hz = 0.0f;
+        if (in_box(gx, gy, 3)) {
+            v = sq((cval(tx, ty - 3) - cval(tx, ty - 1) - cval(tx, ty + 1) + cval(tx, ty + 3)) - 3.0f * (cval(tx, ty - 2) + cval(tx, ty + 2)) +
+                   6.0f * cval(tx, ty));
+            hz = sq((cval(tx - 3, ty) - cval(tx - 1, ty) - cval(tx + 1, ty) + cval(tx + 3, ty)) - 3.0f * (cval(tx - 2, ty) + cval(tx + 2, ty)) +
+                    6.0f * cval(tx, ty));
+        }
+        bv(tx, ty) = v;
+        bh(tx, ty) = hz;
+    }
+    // lpf at R/B sites (demosaic.rs:188-201) over halo 7; G sites hold 0
+    for (int i = tid; i < Tile<TX, TY, H_LPF>::N; i += NT) {
+        int tx = i % Tile<TX, TY, H_LPF>::W - H_LPF, ty = i / Tile<TX, TY, H_LPF>::W - H_LPF;
+        int gx = ox + tx, gy = oy + ty;
+        float v = 0.0f;
+        if (in_box(gx, gy, 2) && pat.at(gx, gy) != 1) {
+            v = cval(tx, ty) + 0.5f * (cval(tx, ty - 1) + cval(tx, ty + 1) + cval(tx - 1, ty) + cval(tx + 1, ty)) +
+                0.25f * (cval(tx - 1, ty - 1) + cval(tx + 1, ty - 1) + cval(tx - 1, ty + 1) + cval(tx + 1, ty + 1));
+        }
+        lpf(tx, ty) = v;
+    }
+    __syncthreads();
+
+    // ---- stage 2: vh (demosaic.rs:173-184) over halo 6; 0.5 off the border box
+    for (int i = tid; i < Tile<TX, TY, H_VH>::N; i += NT) {
+        int tx = i % Tile<TX, TY, H_VH>::W - H_VH, ty = i / Tile<TX, TY, H_VH>::W - H_VH;
+        int gx = ox + tx, gy = oy + ty;
+        float v = 0.5f;
+        if (in_box(gx, gy, kBorder)) {
+            float vs = fmaxr(bv(tx, ty - 1) + bv(tx, ty) + bv(tx, ty + 1), kEps);
+            float hs = fmaxr(bh(tx - 1, ty) + bh(tx, ty) + bh(tx + 1, ty), kEps);
+            v = vs / (vs + hs);
+        }
+        vh(tx, ty) = v;
+    }
+    __syncthreads();
+
+    // ---- stage 3: green at R/B (demosaic.rs:222-245), MHC green off the
+    // border box; bp / bq (demosaic.rs:248-260) into the freed bv/bh space.
+    for (int i = tid; i < Tile<TX, TY, H_G>::N; i += NT) {
+        int tx = i % Tile<TX, TY, H_G>::W - H_G, ty = i / Tile<TX, TY, H_G>::W - H_G;
+        int gx = ox + tx, gy = oy + ty;
+        float gv = 0.0f;
+        if (in_img(gx, gy)) {
+            int col = pat.at(gx, gy);
+            if (col == 1) {
+                gv = cval(tx, ty);
+            } else if (in_box(gx, gy, kBorder)) {
+                float vd = disc(vh, tx, ty);
+                auto c = [&](int dx, int dy) { return cval(tx + dx, ty + dy); };
+                float n_grad = kEps + fabsf(c(0, -1) - c(0, 1)) + fabsf(c(0, 0) - c(0, -2)) + fabsf(c(0, -1) - c(0, -3)) + fabsf(c(0, -2) - c(0, -4));
+                float s_grad = kEps + fabsf(c(0, -1) - c(0, 1)) + fabsf(c(0, 0) - c(0, 2)) + fabsf(c(0, 1) - c(0, 3)) + fabsf(c(0, 2) - c(0, 4));
+                float w_grad = kEps + fabsf(c(-1, 0) - c(1, 0)) + fabsf(c(0, 0) - c(-2, 0)) + fabsf(c(-1, 0) - c(-3, 0)) + fabsf(c(-2, 0) - c(-4, 0));
+                float e_grad = kEps + fabsf(c(-1, 0) - c(1, 0)) + fabsf(c(0, 0) - c(2, 0)) + fabsf(c(1, 0) - c(3, 0)) + fabsf(c(2, 0) - c(4, 0));
+                float l0 = lpf(tx, ty);
+                float n_est = c(0, -1) * 2.0f * l0 / (kEps + l0 + lpf(tx, ty - 2));
+                float s_est = c(0, 1) * 2.0f * l0 / (kEps + l0 + lpf(tx, ty + 2));
+                float w_est = c(-1, 0) * 2.0f * l0 / (kEps + l0 + lpf(tx - 2, ty));
+                float e_est = c(1, 0) * 2.0f * l0 / (kEps + l0 + lpf(tx + 2, ty));
+                float v_est = (s_grad * n_est + n_grad * s_est) / (n_grad + s_grad);
+                float h_est = (w_grad * e_est + e_grad * w_est) / (e_grad + w_grad);
+                gv = fmaxr(vd * h_est + (1.0f - vd) * v_est, 0.0f);
+            } else {
+                gv = mhc_at(cm, tx, ty, col, pat.at(gx + 1, gy), 1);
+            }
+        }
+        gg(tx, ty) = gv;
+    }
+    __syncthreads();
+    for (int i = tid; i < Tile<TX, TY, H_G>::N; i += NT) {
+        int tx = i % Tile<TX, TY, H_G>::W - H_G, ty = i / Tile<TX, TY, H_G>::W - H_G;
+        int gx = ox + tx, gy = oy + ty;
+        float p = 0.0f, q = 0.0f;
+        if (in_box(gx, gy, 3)) {
+            auto c = [&](int dx, int dy) { return cval(tx + dx, ty + dy); };
+            p = sq((c(-3, -3) - c(-1, -1) - c(1, 1) + c(3, 3)) - 3.0f * (c(-2, -2) + c(2, 2)) + 6.0f * c(0, 0));
+            q = sq((c(3, -3) - c(1, -1) - c(-1, 1) + c(-3, 3)) - 3.0f * (c(2, -2) + c(-2, 2)) + 6.0f * c(0, 0));
+        }
+        bp(tx, ty) = p;
+        bq(tx, ty) = q;
+    }
+    __syncthreads();
+
+    // ---- stage 4: pq (demosaic.rs:261-272) over halo 4
+    for (int i = tid; i < Tile<TX, TY, H_PQ>::N; i += NT) {
+        int tx = i % Tile<TX, TY, H_PQ>::W - H_PQ, ty = i / Tile<TX, TY, H_PQ>::W - H_PQ;
+        int gx = ox + tx, gy = oy + ty;
+        float v = 0.5f;
+        if (in_box(gx, gy, kBorder)) {
+            float ps = fmaxr(bp(tx - 1, ty - 1) + bp(tx, ty) + bp(tx + 1, ty + 1), kEps);
+            float qs = fmaxr(bq(tx + 1, ty - 1) + bq(tx, ty) + bq(tx - 1, ty + 1), kEps);
+            v = ps / (ps + qs);
+        }
+        pq(tx, ty) = v;
+    }
+    __syncthreads();
+
+    // ---- stage 5: r and b at R/B sites after step 4.3 (demosaic.rs:276-308)
+    // over halo 3.  R sites keep c for r; B sites in the border box get the
+    // diagonal estimate, outside it the MHC value; symmetrically for b.  The
+    // pre-pass channel at the diagonal neighbours of a B (R) site is an R (B)
+    // site, i.e. c itself.
+    for (int i = tid; i < Tile<TX, TY, H_RB>::N; i += NT) {
+        int tx = i % Tile<TX, TY, H_RB>::W - H_RB, ty = i / Tile<TX, TY, H_RB>::W - H_RB;
+        int gx = ox + tx, gy = oy + ty;
+        float rv = 0.0f, bvv = 0.0f;
+        if (in_img(gx, gy)) {
+            int col = pat.at(gx, gy);
+            if (col != 1) {
+                float own = cval(tx, ty);
+                float other;
+                if (in_box(gx, gy, kBorder)) {
+                    float pd = disc(pq, tx, ty);
+                    auto cc = [&](int dx, int dy) { return cval(tx + dx, ty + dy); };
+                    auto g = [&](int dx, int dy) { return gg(tx + dx, ty + dy); };
+                    float nw_g = kEps + fabsf(cc(-1, -1) - cc(1, 1)) + fabsf(cc(-1, -1) - cc(-3, -3)) + fabsf(g(0, 0) - g(-2, -2));
+                    float ne_g = kEps + fabsf(cc(1, -1) - cc(-1, 1)) + fabsf(cc(1, -1) - cc(3, -3)) + fabsf(g(0, 0) - g(2, -2));
+                    float sw_g = kEps + fabsf(cc(-1, 1) - cc(1, -1)) + fabsf(cc(-1, 1) - cc(-3, 3)) + fabsf(g(0, 0) - g(-2, 2));
+                    float se_g = kEps + fabsf(cc(1, 1) - cc(-1, -1)) + fabsf(cc(1, 1) - cc(3, 3)) + fabsf(g(0, 0) - g(2, 2));
+                    float nw_e = cc(-1, -1) - g(-1, -1);
+                    float ne_e = cc(1, -1) - g(1, -1);
+                    float sw_e = cc(-1, 1) - g(-1, 1);
+                    float se_e = cc(1, 1) - g(1, 1);
+                    float p_est = (nw_g * se_e + se_g * nw_e) / (nw_g + se_g);
+                    float q_est = (ne_g * sw_e + sw_g * ne_e) / (ne_g + sw_g);
+                    other = fmaxr(g(0, 0) + (1.0f - pd) * p_est + pd * q_est, 0.0f);
+                } else {
+                    other = mhc_at(cm, tx, ty, col, pat.at(gx + 1, gy), 2 - col);
+                }
+                if (col == 0) {
+                    rv = own;
+                    bvv = other;
+                } else {
+                    rv = other;
+                    bvv = own;
+                }
+            }
+        }
+        r43(tx, ty) = rv;
+        b43(tx, ty) = bvv;
+    }
+    __syncthreads();
+
+    // ---- stage 6: outputs; R and B at G sites (demosaic.rs:310-341)
+    for (int i = tid; i < TX * TY; i += NT) {
+        int tx = i % TX, ty = i / TX;
+        int gx = ox + tx, gy = oy + ty;
+        if (gx >= w || gy >= h) continue;
+        int col = pat.at(gx, gy);
+        float o[3];
+        if (col == 1) {
+            float gc = cval(tx, ty);
+            o[1] = gc;
+            if (in_box(gx, gy, kBorder)) {
+                float vd = disc(vh, tx, ty);
+                auto g = [&](int dx, int dy) { return gg(tx + dx, ty + dy); };
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    const Tile<TX, TY, H_RB>& ch = k == 0 ? r43 : b43;
+                    auto cc = [&](int dx, int dy) { return ch(tx + dx, ty + dy); };
+                    float n_g = kEps + fabsf(g(0, 0) - g(0, -2)) + fabsf(cc(0, -1) - cc(0, 1)) + fabsf(cc(0, -1) - cc(0, -3));
+                    float s_g = kEps + fabsf(g(0, 0) - g(0, 2)) + fabsf(cc(0, 1) - cc(0, -1)) + fabsf(cc(0, 1) - cc(0, 3));
+                    float w_g = kEps + fabsf(g(0, 0) - g(-2, 0)) + fabsf(cc(-1, 0) - cc(1, 0)) + fabsf(cc(-1, 0) - cc(-3, 0));
+                    float e_g = kEps + fabsf(g(0, 0) - g(2, 0)) + fabsf(cc(1, 0) - cc(-1, 0)) + fabsf(cc(1, 0) - cc(3, 0));
+                    float n_e = cc(0, -1) - g(0, -1);
+                    float s_e = cc(0, 1) - g(0, 1);
+                    float w_e = cc(-1, 0) - g(-1, 0);
+                    float e_e = cc(1, 0) - g(1, 0);
+                    float v_est = (n_g * s_e + s_g * n_e) / (n_g + s_g);
+                    float h_est = (e_g * w_e + w_g * e_e) / (e_g + w_g);
+                    o[k == 0 ? 0 : 2] = fmaxr(g(0, 0) + (1.0f - vd) * v_est + vd * h_est, 0.0f);
+                }
+            } else {
+                int row_c = pat.at(gx + 1, gy);
+                o[0] = mhc_at(cm, tx, ty, 1, row_c, 0);
+                o[2] = mhc_at(cm, tx, ty, 1, row_c, 2);
+            }
+        } else {
+            o[1] = gg(tx, ty);
+            o[0] = r43(tx, ty);
+            o[2] = b43(tx, ty);
+        }
+        size_t gi = (size_t)gy * (size_t)w + (size_t)gx;
+        rgb[3 * gi + 0] = o[0];
+        rgb[3 * gi + 1] = o[1];
+        rgb[3 * gi + 2] = o[2];
+        if (clip) {
+            uint8_t m = 0;
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) m |= L.flag[(ty + dy + H_FLAG) * Tile<TX, TY, H_FLAG>::W + (tx + dx + H_FLAG)];
+            clip[gi] = m ? 1.0f : 0.0f;
+        }
+    }
+}
+
+// Plain MHC over the whole frame (demosaic.rs:89-124) plus the clip plane:
+// a TX x TY tile with a 2-pixel clamped halo of cfa*mul in LDS.
+template <int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_mhc(CfaSrc src, float* __restrict__ rgb, float* __restrict__ clip) {
+    constexpr int H = 2;
+    __shared__ float s_cm[Tile<TX, TY, H>::N];
+    __shared__ uint8_t s_flag[Tile<TX, TY, 1>::N];
+    Tile<TX, TY, H> cm{s_cm};
+    const int tid = threadIdx.x;
+    const int ox = blockIdx.x * TX, oy = blockIdx.y * TY;
+    const int w = src.w, h = src.h;
+    const Bayer pat = src.pat;
+    for (int i = tid; i < Tile<TX, TY, H>::N; i += NT) {
+        int tx = i % Tile<TX, TY, H>::W - H, ty = i / Tile<TX, TY, H>::W - H;
+        int gx = clampi(ox + tx, 0, w - 1), gy = clampi(oy + ty, 0, h - 1);
+        int c = pat.at(gx, gy);
+        float v = src.cfa(gx, gy, c);
+        cm(tx, ty) = v * src.mul[c];
+        if (clip && tx >= -1 && tx <= TX && ty >= -1 && ty <= TY) s_flag[(ty + 1) * Tile<TX, TY, 1>::W + (tx + 1)] = v >= kClip ? 1 : 0;
+    }
+    __syncthreads();
+    for (int i = tid; i < TX * TY; i += NT) {
+        int tx = i % TX, ty = i / TX;
+        int gx = ox + tx, gy = oy + ty;
+        if (gx >= w || gy >= h) continue;
+        int col = pat.at(gx, gy);
+        int row_c = pat.at(gx + 1, gy);
+        size_t gi = (size_t)gy * (size_t)w + (size_t)gx;
+        rgb[3 * gi + 0] = mhc_at(cm, tx, ty, col, row_c, 0);
+        rgb[3 * gi + 1] = mhc_at(cm, tx, ty, col, row_c, 1);
+        rgb[3 * gi + 2] = mhc_at(cm, tx, ty, col, row_c, 2);
+        if (clip) {
+            uint8_t m = 0;
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) m |= s_flag[(ty + dy + 1) * Tile<TX, TY, 1>::W + (tx + dx + 1)];
+            clip[gi] = m ? 1.0f : 0.0f;
+        }
+    }
+}
+
+constexpr int RCD_TX = 64, RCD_TY = 32, RCD_NT = 256;
+constexpr int MHC_TX = 64, MHC_TY = 16, MHC_NT = 256;
+
+}  // namespace
+
+hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream) {
+    CfaSrc src{};
+    src.f32 = a.cfa_f32;
+    src.u16 = a.raw_u16;
+    src.stride = a.stride;
+    src.black = a.black;
+    for (int c = 0; c < 3; c++) {
+        src.white[c] = a.white[c];
+        src.mul[c] = a.mul[c];
+    }
+    for (int i = 0; i < 4; i++) src.pat.c[i] = a.pattern[i];
+    src.w = (int)a.w;
+    src.h = (int)a.h;
+    // rcd_bayer falls back to MHC below 16 pixels (demosaic.rs:149-151)
+    bool rcd = a.algo == DS_ALGO_RCD && a.w >= 16 && a.h >= 16;
+    if (rcd) {
+        dim3 grid((unsigned)((a.w + RCD_TX - 1) / RCD_TX), (unsigned)((a.h + RCD_TY - 1) / RCD_TY));
+        hipLaunchKernelGGL((k_rcd<RCD_TX, RCD_TY, RCD_NT>), grid, dim3(RCD_NT), 0, stream, src, a.rgb, a.clip);
+    } else {
+        dim3 grid((unsigned)((a.w + MHC_TX - 1) / MHC_TX), (unsigned)((a.h + MHC_TY - 1) / MHC_TY));
+        hipLaunchKernelGGL((k_mhc<MHC_TX, MHC_TY, MHC_NT>), grid, dim3(MHC_NT), 0, stream, src, a.rgb, a.clip);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ds

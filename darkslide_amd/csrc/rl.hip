@@ -1,0 +1,245 @@
+// Richardson-Lucy "raw details" (enhance.rs:94-119) for gfx950.
+//
+// For sigma in [0.3, 2) the PSF blur is gaussian_blur_exact (image.rs:341-361):
+// separable, r = ceil(3 sigma) <= 6 taps per side, clamped edges, taps
+// accumulated in order.  One launch runs one whole iteration
+//     conv = G(est); ratio = y / max(conv, 1e-6); corr = G(ratio);
+//     est = clamp(est * corr, 0, 64)
+// on a TX x TY tile: est with a 2r halo and y with an r halo are staged in
+// LDS, and the four separable passes run LDS to LDS, so an iteration moves
+// est in, y in and est out through HBM once.  The arithmetic keeps the
+// reference's order and is compiled without FMA contraction, so each
+// iteration is bit-identical to the CPU one.
+//
+// Other sigmas fall back to plane-wide passes: sigma < 0.3 (blur is a copy)
+// and sigma >= 2 (three box blurs, image.rs:334-337).
+
+#include "common.h"
+#include "denoise.h"
+#include "rl.h"
+
+#pragma clang fp contract(off)
+
+namespace ds {
+
+namespace {
+
+constexpr int RL_TX = 64, RL_TY = 32, RL_NT = 256;
+
+struct Taps {
+    float k[13];
+};
+
+// luminance (image.rs:174-176, color.rs:196-198), y = max(lum, 1e-6); est = y
+__global__ void k_rl_init(const float* __restrict__ rgb, size_t n, float* __restrict__ y, float* __restrict__ est) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float l = 0.2880402f * rgb[3 * i] + 0.7118741f * rgb[3 * i + 1] + 0.0000857f * rgb[3 * i + 2];
+    float v = fmaxr(l, 1e-6f);
+    y[i] = v;
+    est[i] = v;
+}
+
+// out = p * clamp(est / y, 0.6, 1.6) (enhance.rs:113-115)
+__global__ void k_rl_apply(const float* __restrict__ rgb, const float* __restrict__ est, const float* __restrict__ y, size_t n,
+                           float* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float r = clampr(est[i] / y[i], 0.6f, 1.6f);
+    out[3 * i + 0] = rgb[3 * i + 0] * r;
+    out[3 * i + 1] = rgb[3 * i + 1] * r;
+    out[3 * i + 2] = rgb[3 * i + 2] * r;
+}
+
+template <int R>
+__global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est, const float* __restrict__ yp, float* __restrict__ est_out,
+                                                   int w, int h, Taps taps) {
+    constexpr int TX = RL_TX, TY = RL_TY, NT = RL_NT, K = 2 * R + 1;
+    // est: [-2R, T+2R) both ways; y, ratio: [-R, T+R); tmp: cols [-R, TX+R), rows [-2R, TY+2R)
+    constexpr int EW = TX + 4 * R, EH = TY + 4 * R;
+    constexpr int YW = TX + 2 * R, YH = TY + 2 * R;
+    constexpr int MW = TX + 2 * R, MH = TY + 4 * R;
+    __shared__ float s_est[EH * EW];
+    __shared__ float s_y[YH * YW];
+    __shared__ float s_tmp[MH * MW];
+    __shared__ float s_rat[YH * YW];
+    const int tid = threadIdx.x;
+    const int ox = blockIdx.x * TX, oy = blockIdx.y * TY;
+    float kk[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) kk[i] = taps.k[i];
+
+    for (int i = tid; i < EH * EW; i += NT) {
+        int u = i % EW - 2 * R, v = i / EW - 2 * R;
+        s_est[i] = est[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
+    }
+    for (int i = tid; i < YH * YW; i += NT) {
+        int u = i % YW - R, v = i / YW - R;
+        s_y[i] = yp[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
+    }
+    __syncthreads();
+    auto E = [&](int u, int v) -> float { return s_est[(v + 2 * R) * EW + (u + 2 * R)]; };
+    // Each stage evaluates at the clamped image coordinate of its slot, so a
+    // slot outside the image holds the value of the edge pixel it clamps to.
+    auto cu_of = [&](int u) { return clampi(ox + u, 0, w - 1) - ox; };
+    auto cv_of = [&](int v) { return clampi(oy + v, 0, h - 1) - oy; };
+
+    // horizontal blur of est
+    for (int i = tid; i < MH * MW; i += NT) {
+        int u = i % MW - R, v = i / MW - 2 * R;
+        int cu = cu_of(u);
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < K; j++) acc += kk[j] * E(cu + j - R, v);
+        s_tmp[i] = acc;
+    }
+    __syncthreads();
+    // vertical blur -> conv, ratio = y / max(conv, 1e-6)
+    for (int i = tid; i < YH * YW; i += NT) {
+        int u = i % YW - R, v = i / YW - R;
+        int cv = cv_of(v);
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(cv + j - R + 2 * R) * MW + (u + R)];
+        s_rat[i] = s_y[i] / fmaxr(acc, 1e-6f);
+    }
+    __syncthreads();
+    // horizontal blur of ratio over rows [-R, TY+R), cols [0, TX) (reuses s_tmp)
+    constexpr int QW = TX, QH = TY + 2 * R;
+    for (int i = tid; i < QH * QW; i += NT) {
+        int u = i % QW, v = i / QW - R;
+        int cu = cu_of(u);
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < K; j++) acc += kk[j] * s_rat[(v + R) * YW + (cu + j - R + R)];
+        s_tmp[i] = acc;
+    }
+    __syncthreads();
+    // vertical blur -> corr, est update
+    for (int i = tid; i < TX * TY; i += NT) {
+        int u = i % TX, v = i / TX;
+        int gx = ox + u, gy = oy + v;
+        if (gx >= w || gy >= h) continue;
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(v + j - R + R) * QW + u];
+        est_out[(size_t)gy * w + gx] = clampr(E(u, v) * acc, 0.0f, 64.0f);
+    }
+}
+
+// ---- plane-wide fallback pieces
+__global__ void k_rl_ratio(const float* __restrict__ y, const float* __restrict__ conv, size_t n, float* __restrict__ ratio) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ratio[i] = y[i] / fmaxr(conv[i], 1e-6f);
+}
+__global__ void k_rl_update(float* __restrict__ est, const float* __restrict__ corr, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) est[i] = clampr(est[i] * corr[i], 0.0f, 64.0f);
+}
+
+// Wells' three-box radii (image.rs:363-373)
+void box_radii_for_gauss(float sigma, size_t radii[3]) {
+    const int n = 3;
+    float w_ideal = sqrtf(12.0f * sigma * sigma / (float)n + 1.0f);
+    int wl = (int)floorf(w_ideal);
+    if (wl % 2 == 0) wl -= 1;
+    int wu = wl + 2;
+    float m_ideal = (12.0f * sigma * sigma - (float)(n * wl * wl) - 4.0f * (float)n * (float)wl - 3.0f * (float)n) / (-4.0f * (float)wl - 4.0f);
+    int m = (int)roundf(m_ideal);
+    for (int i = 0; i < n; i++) {
+        int v = ((i < m ? wl : wu) - 1) / 2;
+        radii[i] = (size_t)(v > 0 ? v : 0);
+    }
+}
+
+template <int R>
+hipError_t launch_iter(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, hipStream_t st) {
+    dim3 grid((unsigned)((w + RL_TX - 1) / RL_TX), (unsigned)((h + RL_TY - 1) / RL_TY));
+    hipLaunchKernelGGL(k_rl_iter<R>, grid, dim3(RL_NT), 0, st, est, y, est_out, (int)w, (int)h, t);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Taps of gaussian_blur_exact (image.rs:342-345), computed on the host with
+// the same f32 operations (glibc expf, sequential sum, in-place division).
+int gaussian_taps(float sigma, float* k) {
+    int r = (int)ceilf(sigma * 3.0f);
+    float s = 0.0f;
+    for (int i = -r; i <= r; i++) {
+        k[i + r] = expf(-(float)(i * i) / (2.0f * sigma * sigma));
+    }
+    for (int i = 0; i <= 2 * r; i++) s += k[i];
+    for (int i = 0; i <= 2 * r; i++) k[i] /= s;
+    return r;
+}
+
+size_t rl_workspace_floats(size_t w, size_t h) { return 5 * w * h; }
+
+hipError_t launch_raw_details(const float* rgb, size_t w, size_t h, float sigma, size_t iters, float* ws, float* rgb_out, hipStream_t st) {
+    const size_t n = w * h;
+    if (n == 0) return hipSuccess;
+    float* y = ws;
+    float* est = ws + n;
+    float* est2 = ws + 2 * n;
+    float* t0 = ws + 3 * n;
+    float* t1 = ws + 4 * n;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_rl_init, dim3(nb), dim3(256), 0, st, rgb, n, y, est);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const bool exact = sigma >= 0.3f && sigma < 2.0f;  // image.rs:328-333
+    Taps taps{};
+    int r = 0;
+    if (exact) r = gaussian_taps(sigma, taps.k);
+    for (size_t it = 0; it < iters; it++) {
+        if (exact) {
+            switch (r) {
+                case 1: e = launch_iter<1>(est, y, est2, w, h, taps, st); break;
+                case 2: e = launch_iter<2>(est, y, est2, w, h, taps, st); break;
+                case 3: e = launch_iter<3>(est, y, est2, w, h, taps, st); break;
+                case 4: e = launch_iter<4>(est, y, est2, w, h, taps, st); break;
+                case 5: e = launch_iter<5>(est, y, est2, w, h, taps, st); break;
+                case 6: e = launch_iter<6>(est, y, est2, w, h, taps, st); break;
+                default: return hipErrorInvalidValue;
+            }
+            if (e != hipSuccess) return e;
+            float* t = est;
+            est = est2;
+            est2 = t;
+            continue;
+        }
+        // conv = gaussian_blur(est); ratio; corr = gaussian_blur(ratio); update
+        for (int pass = 0; pass < 2; pass++) {
+            const float* in = pass == 0 ? est : t0;
+            float* blurred = pass == 0 ? est2 : t1;
+            if (sigma < 0.3f) {
+                e = hipMemcpyAsync(blurred, in, n * sizeof(float), hipMemcpyDeviceToDevice, st);
+            } else {
+                size_t radii[3];
+                box_radii_for_gauss(sigma, radii);
+                // box(box(box(in, r0), r1), r2); t0 is free as the
+                // horizontal buffer once `in` has been read by the first box
+                float* scratch = pass == 0 ? t1 : est2;
+                e = launch_box_blur(in, scratch, blurred, w, h, radii[0], st);
+                if (e == hipSuccess) e = hipMemcpyAsync(scratch, blurred, n * sizeof(float), hipMemcpyDeviceToDevice, st);
+                if (e == hipSuccess) e = launch_box_blur(scratch, t0, blurred, w, h, radii[1], st);
+                if (e == hipSuccess) e = hipMemcpyAsync(scratch, blurred, n * sizeof(float), hipMemcpyDeviceToDevice, st);
+                if (e == hipSuccess) e = launch_box_blur(scratch, t0, blurred, w, h, radii[2], st);
+            }
+            if (e != hipSuccess) return e;
+            if (pass == 0) {
+                hipLaunchKernelGGL(k_rl_ratio, dim3(nb), dim3(256), 0, st, y, est2, n, t0);
+                e = hipGetLastError();
+                if (e != hipSuccess) return e;
+            }
+        }
+        hipLaunchKernelGGL(k_rl_update, dim3(nb), dim3(256), 0, st, est, t1, n);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_rl_apply, dim3(nb), dim3(256), 0, st, rgb, est, y, n, rgb_out);
+    return hipGetLastError();
+}
+
+}  // namespace ds

@@ -1,0 +1,137 @@
+/*
+ * darkslide_gpu.h - C ABI of the MI355X (gfx950) raw detail path:
+ * Bayer demosaic (RCD / MHC), NLM luminance denoise and Richardson-Lucy
+ * "raw details" sharpening.
+ *
+ * Each entry point replaces one function of the reference engine
+ * (Syha-01/darkslide, crates/core/src); the replaced function is cited
+ * beside it.  Buffer layouts are the reference's in-memory layouts, so a
+ * Rust caller passes `vec.as_ptr()` straight through:
+ *   - Plane:  w*h f32, row-major, tightly packed (image.rs:15-21)
+ *   - Image3: w*h interleaved [r,g,b] f32 triples (image.rs:7-13)
+ *   - CFA:    w*h f32 normalised samples (decode.rs:49-56), or raw u16 DN
+ *   - Bayer pattern: 4 bytes, colour (0=R,1=G,2=B) of (x%2, y%2) at
+ *     pattern[(y%2)*2 + x%2] (decode.rs:16-28); greens must form a
+ *     checkerboard (RGGB, BGGR, GRBG, GBRG).
+ *
+ * Host-pointer entry points are synchronous: they copy in, run on the
+ * context's HIP stream, copy out and return.  The *_dev entry points take
+ * device pointers, only enqueue on the context's stream and return; call
+ * ds_sync() before reading results.
+ *
+ * Every call returns DS_OK (0) or a negative DS_ERR_* code;
+ * ds_last_error() gives the message of the last failure on that context.
+ * A context serialises its calls with an internal mutex, so it may be shared
+ * between threads; one context per thread gives concurrency.
+ */
+#ifndef DARKSLIDE_GPU_H
+#define DARKSLIDE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DS_OK 0
+#define DS_ERR_INVALID (-1) /* bad argument (size, pattern, null pointer) */
+#define DS_ERR_DEVICE (-2)  /* no usable GPU, or a HIP runtime failure */
+#define DS_ERR_ALLOC (-3)   /* device memory allocation failed */
+#define DS_ERR_LAUNCH (-4)  /* kernel launch failed */
+
+#define DS_ALGO_RCD 0 /* demosaic.rs:147 rcd_bayer (MHC below 16 px) */
+#define DS_ALGO_MHC 1 /* demosaic.rs:90 mhc_bayer */
+
+typedef struct ds_ctx ds_ctx;
+
+/* Context on HIP device `device` with its own stream. */
+int ds_ctx_create(int device, ds_ctx** out);
+void ds_ctx_destroy(ds_ctx* ctx);
+/* Message of the last failure on ctx (ctx may be NULL: last create failure). */
+const char* ds_last_error(const ds_ctx* ctx);
+/* Library version string. */
+const char* ds_version(void);
+
+/* ---- demosaic ---------------------------------------------------------- */
+
+/* demosaic.rs:20 demosaic(cfa, mul, Quality::Full) for a Bayer CFA:
+ * rcd_bayer (demosaic.rs:147) or mhc_bayer (demosaic.rs:90), and
+ * clip_plane (demosaic.rs:31).  cfa: w*h normalised samples; mul: as-shot
+ * WB multipliers (pipeline/mod.rs:166-169).  rgb_out: w*h*3; clip_out:
+ * w*h or NULL. */
+int ds_demosaic_f32(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], int algo,
+                    float* rgb_out, float* clip_out);
+
+/* Same, fused with the CFA normalisation of decode.rs:291-306:
+ * v = (raw - black) / max(white[c] - black, 1).  raw: h rows of `stride`
+ * u16 samples. */
+int ds_demosaic_u16(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
+                    const float white[3], const float mul[3], int algo, float* rgb_out, float* clip_out);
+
+/* ---- enhance ----------------------------------------------------------- */
+
+/* enhance.rs:10 estimate_noise(p) */
+int ds_estimate_noise(ds_ctx* ctx, const float* plane, size_t w, size_t h, float* sigma_out);
+
+/* enhance.rs:36 nlm(p, h_param, search, patch) */
+int ds_nlm(ds_ctx* ctx, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, float* out);
+
+/* enhance.rs:75 denoise(img, amount): NLM with search 3, patch 1. */
+int ds_denoise(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float amount, float* rgb_out);
+
+/* enhance.rs:75 denoise with the NLM search / patch radii as parameters
+ * (the reference hard-codes 3, 1 at enhance.rs:84). */
+int ds_denoise_ex(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* rgb_out);
+
+/* enhance.rs:96 raw_details(img, sigma, iterations) */
+int ds_raw_details(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float sigma, size_t iterations, float* rgb_out);
+
+/* ---- device-resident variants (enqueue only) ---------------------------- */
+
+int ds_dev_alloc(ds_ctx* ctx, size_t bytes, void** dptr);
+int ds_dev_free(ds_ctx* ctx, void* dptr);
+int ds_memcpy_h2d(ds_ctx* ctx, void* dst, const void* src, size_t bytes); /* synchronous */
+int ds_memcpy_d2h(ds_ctx* ctx, void* dst, const void* src, size_t bytes); /* synchronous */
+int ds_sync(ds_ctx* ctx);
+
+int ds_demosaic_u16_dev(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
+                        const float white[3], const float mul[3], int algo, float* rgb_out, float* clip_out);
+int ds_demosaic_f32_dev(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], int algo,
+                        float* rgb_out, float* clip_out);
+/* sigma_out is a device pointer to one float */
+int ds_estimate_noise_dev(ds_ctx* ctx, const float* plane, size_t w, size_t h, float* sigma_out);
+int ds_nlm_dev(ds_ctx* ctx, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, float* out);
+int ds_denoise_ex_dev(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* rgb_out);
+int ds_raw_details_dev(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float sigma, size_t iterations, float* rgb_out);
+
+/* ---- stage timing ------------------------------------------------------ */
+
+/* With profiling on, every kernel launch of the context is bracketed by
+ * HIP events on the context's stream; per-kernel totals accumulate until
+ * ds_prof_reset.  ds_prof_get fills up to `cap` entries and returns the
+ * number of kernels seen (or a negative error). */
+typedef struct ds_prof_entry {
+    char name[48];
+    double total_ms;
+    long launches;
+} ds_prof_entry;
+
+int ds_prof_enable(ds_ctx* ctx, int on);
+int ds_prof_reset(ds_ctx* ctx);
+int ds_prof_get(ds_ctx* ctx, ds_prof_entry* out, int cap);
+
+/* ---- synthetic input --------------------------------------------------- */
+
+/* Seeded synthetic Bayer frame (gradients + polygons + sinusoids, linear in
+ * [0.02, 0.85] of range, Poisson-Gaussian noise with gain 0.5*iso DN/e and
+ * read sigma 4 DN).  bits 14: black 512, white 16383; bits 12: black 128,
+ * white 4095.  raw_out: w*h u16.  The _dev form writes device memory. */
+int ds_synth_bayer_host(uint64_t seed, size_t w, size_t h, int bits, float iso, const uint8_t pattern[4], uint16_t* raw_out);
+int ds_synth_bayer_dev(ds_ctx* ctx, uint64_t seed, size_t w, size_t h, int bits, float iso, const uint8_t pattern[4], uint16_t* raw_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DARKSLIDE_GPU_H */

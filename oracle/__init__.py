@@ -1,0 +1,213 @@
+"""CPU oracle of the raw detail path (TEST INFRASTRUCTURE ONLY).
+
+ctypes binding of oracle/liboracle.so, the C++ restatement of the reference
+Rust (oracle.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline use it, as the checker and as the timed CPU path; the product
+library never loads it.
+
+Every function takes float32 or float64 numpy arrays and dispatches to the
+FP32 (bit-faithful) or FP64 instantiation accordingly.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _sfx(dtype):
+    if dtype == np.float32:
+        return "f32", ctypes.c_float
+    if dtype == np.float64:
+        return "f64", ctypes.c_double
+    raise TypeError(dtype)
+
+
+def _fn(name, dtype, restype=None):
+    sfx, ct = _sfx(dtype)
+    f = getattr(lib(), f"{name}_{sfx}")
+    f.restype = restype
+    return f, ct
+
+
+def _pat(pattern):
+    return (ctypes.c_uint8 * 4)(*[int(v) for v in pattern])
+
+
+def threads():
+    f = lib().or_max_threads
+    f.restype = ctypes.c_int
+    return f()
+
+
+def cfa_normalize_u16(raw, pattern, black, white, dtype=np.float32):
+    raw = np.ascontiguousarray(raw, np.uint16)
+    h, w = raw.shape
+    out = np.empty((h, w), dtype)
+    f, ct = _fn("or_cfa_normalize_u16", dtype)
+    wh = np.asarray(white, dtype)
+    f(_p(raw), ctypes.c_size_t(w), ctypes.c_size_t(h), ctypes.c_size_t(w), _pat(pattern), ct(black), _p(wh), _p(out))
+    return out
+
+
+def _demo(name, cfa, pattern, mul):
+    cfa = np.ascontiguousarray(cfa)
+    h, w = cfa.shape
+    out = np.empty((h, w, 3), cfa.dtype)
+    f, _ = _fn(name, cfa.dtype)
+    m = np.asarray(mul, cfa.dtype)
+    f(_p(cfa), ctypes.c_size_t(w), ctypes.c_size_t(h), _pat(pattern), _p(m), _p(out))
+    return out
+
+
+def mhc_bayer(cfa, pattern=(0, 1, 1, 2), mul=(1, 1, 1)):
+    return _demo("or_mhc_bayer", cfa, pattern, mul)
+
+
+def rcd_bayer(cfa, pattern=(0, 1, 1, 2), mul=(1, 1, 1)):
+    return _demo("or_rcd_bayer", cfa, pattern, mul)
+
+
+def clip_plane(cfa):
+    cfa = np.ascontiguousarray(cfa)
+    h, w = cfa.shape
+    out = np.empty((h, w), cfa.dtype)
+    f, _ = _fn("or_clip_plane", cfa.dtype)
+    f(_p(cfa), ctypes.c_size_t(w), ctypes.c_size_t(h), _p(out))
+    return out
+
+
+def demosaic(cfa, pattern=(0, 1, 1, 2), mul=(1, 1, 1)):
+    return rcd_bayer(cfa, pattern, mul), clip_plane(cfa)
+
+
+def _pix3(name, arr):
+    arr = np.ascontiguousarray(arr)
+    out = np.empty_like(arr)
+    f, _ = _fn(name, arr.dtype)
+    f(_p(arr), ctypes.c_size_t(arr.size // 3), _p(out))
+    return out
+
+
+def oklab_from_prophoto(rgb):
+    return _pix3("or_oklab_from_prophoto", rgb)
+
+
+def oklab_to_prophoto(lab):
+    return _pix3("or_oklab_to_prophoto", lab)
+
+
+def oklab_matrices(dtype=np.float32):
+    mats = [np.empty((3, 3), dtype) for _ in range(4)]
+    f, _ = _fn("or_oklab_matrices", dtype)
+    f(*[_p(m) for m in mats])
+    return dict(zip(["to_lms", "from_lms", "m2", "m2_inv"], mats))
+
+
+def luminance(rgb):
+    rgb = np.ascontiguousarray(rgb)
+    out = np.empty(rgb.shape[:-1], rgb.dtype)
+    f, _ = _fn("or_luminance", rgb.dtype)
+    f(_p(rgb), ctypes.c_size_t(rgb.size // 3), _p(out))
+    return out
+
+
+def estimate_noise(p):
+    p = np.ascontiguousarray(p)
+    h, w = p.shape
+    _, ct = _sfx(p.dtype)
+    f, _ = _fn("or_estimate_noise", p.dtype, ct)
+    return float(f(_p(p), ctypes.c_size_t(w), ctypes.c_size_t(h)))
+
+
+def box_blur(p, r):
+    p = np.ascontiguousarray(p)
+    h, w = p.shape
+    out = np.empty_like(p)
+    f, _ = _fn("or_box_blur", p.dtype)
+    f(_p(p), ctypes.c_size_t(w), ctypes.c_size_t(h), ctypes.c_size_t(r), _p(out))
+    return out
+
+
+def gaussian_blur(p, sigma):
+    p = np.ascontiguousarray(p)
+    h, w = p.shape
+    out = np.empty_like(p)
+    f, ct = _fn("or_gaussian_blur", p.dtype)
+    f(_p(p), ctypes.c_size_t(w), ctypes.c_size_t(h), ct(sigma), _p(out))
+    return out
+
+
+def guided_filter(I, p, r, eps):
+    I = np.ascontiguousarray(I)
+    p = np.ascontiguousarray(p, I.dtype)
+    h, w = I.shape
+    out = np.empty_like(I)
+    f, ct = _fn("or_guided_filter", I.dtype)
+    f(_p(I), _p(p), ctypes.c_size_t(w), ctypes.c_size_t(h), ctypes.c_size_t(r), ct(eps), _p(out))
+    return out
+
+
+def nlm(p, h_param, search, patch, nthreads=None):
+    p = np.ascontiguousarray(p)
+    h, w = p.shape
+    out = np.empty_like(p)
+    f, ct = _fn("or_nlm", p.dtype)
+    f(_p(p), ctypes.c_size_t(w), ctypes.c_size_t(h), ct(h_param), ctypes.c_long(search), ctypes.c_size_t(patch),
+      ctypes.c_int(nthreads or threads()), _p(out))
+    return out
+
+
+def denoise(rgb, amount, search=3, patch=1, nthreads=None):
+    rgb = np.ascontiguousarray(rgb)
+    h, w, _ = rgb.shape
+    out = np.empty_like(rgb)
+    f, ct = _fn("or_denoise", rgb.dtype)
+    f(_p(rgb), ctypes.c_size_t(w), ctypes.c_size_t(h), ct(amount), ctypes.c_long(search), ctypes.c_size_t(patch),
+      ctypes.c_int(nthreads or threads()), _p(out))
+    return out
+
+
+def raw_details(rgb, sigma, iters):
+    rgb = np.ascontiguousarray(rgb)
+    h, w, _ = rgb.shape
+    out = np.empty_like(rgb)
+    f, ct = _fn("or_raw_details", rgb.dtype)
+    f(_p(rgb), ctypes.c_size_t(w), ctypes.c_size_t(h), ct(sigma), ctypes.c_size_t(iters), _p(out))
+    return out
+
+
+def gaussian_taps(sigma):
+    taps = np.zeros(32, np.float32)
+    r = ctypes.c_int()
+    f = lib().or_gaussian_taps_f32
+    f(ctypes.c_float(sigma), _p(taps), ctypes.byref(r))
+    return taps[: 2 * r.value + 1], r.value
+
+
+def box_radii_for_gauss(sigma):
+    radii = (ctypes.c_size_t * 3)()
+    lib().or_box_radii_for_gauss(ctypes.c_float(sigma), radii)
+    return list(radii)

@@ -1,0 +1,269 @@
+"""GPU parity of the C ABI against the CPU oracle.
+
+Gate (enhance / demosaic outputs on [0,1] linear data): max-abs <= 2e-5 against
+the FP32 restatement of the reference.  Small and config-1 sizes compare whole
+frames; the 6000x4000 frame compares whole frames where the oracle is fast
+(demosaic, noise estimate) and crops elsewhere, using the locality of each
+stage: a pixel of NLM depends only on pixels within search + patch, a pixel of
+RL on pixels within 2 r iterations, so the interior of an oracle run on a
+crop with that margin equals the full-frame result.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import PATTERNS, RGGB, maxabs, synth
+
+pytestmark = pytest.mark.gpu
+
+TOL = 2e-5
+WHITE = (16383.0, 16383.0, 16383.0)
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def oracle_demosaic(O, raw, pat, mul, algo, black=512.0, white=WHITE):
+    cfa = O.cfa_normalize_u16(raw, pat, black, list(white))
+    rgb = O.rcd_bayer(cfa, pat, mul) if algo == 0 else O.mhc_bayer(cfa, pat, mul)
+    return rgb, O.clip_plane(cfa), cfa
+
+
+# ------------------------------------------------------------------ demosaic
+
+def test_demosaic_rcd_config1(ctx, oracle_mod):
+    """1024x1024 RGGB uint16 14-bit, black 512, white 16383 (the CPU config)."""
+    O = oracle_mod
+    raw = synth(0x5EED0000, 1024, 1024)
+    rgb, clip = ctx.demosaic_u16(raw, RGGB, 512.0, WHITE, (1, 1, 1), 0)
+    want, wclip, _ = oracle_demosaic(O, raw, RGGB, (1, 1, 1), 0)
+    err = maxabs(rgb, want)
+    say(f"rcd 1024^2 max-abs {err:.3g}, bit-equal {np.mean(rgb == want):.6f}")
+    assert err <= TOL
+    assert np.array_equal(clip, wclip)
+
+
+@pytest.mark.parametrize("pname", list(PATTERNS))
+@pytest.mark.parametrize("algo", [0, 1])
+@pytest.mark.parametrize("w,h", [(64, 64), (97, 61), (16, 16), (15, 40), (130, 67)])
+def test_demosaic_patterns_sizes(ctx, oracle_mod, pname, algo, w, h):
+    O = oracle_mod
+    pat = PATTERNS[pname]
+    raw = synth(1000 + w + h, w, h, 14, 2.0, pat)
+    mul = (2.1, 1.0, 1.6)
+    rgb, clip = ctx.demosaic_u16(raw, pat, 512.0, WHITE, mul, algo)
+    want, wclip, cfa = oracle_demosaic(O, raw, pat, mul, algo)
+    assert maxabs(rgb, want) <= TOL
+    assert np.array_equal(clip, wclip)
+    # the f32 entry point on the normalised CFA gives the same image
+    rgb2, clip2 = ctx.demosaic(cfa, pat, mul, algo)
+    assert maxabs(rgb2, want) <= TOL
+    assert np.array_equal(clip2, wclip)
+
+
+def test_demosaic_clipped_and_negative(ctx, oracle_mod):
+    """Clipped highlights set the clip plane; below-black DNs go negative in
+    the CFA and are floored by RCD/MHC as in the reference."""
+    O = oracle_mod
+    raw = synth(77, 200, 120)
+    raw[30:50, 40:90] = 16383
+    raw[80:90, 10:20] = 300
+    white = (16383.0, 16000.0, 15000.0)
+    rgb, clip = ctx.demosaic_u16(raw, RGGB, 512.0, white, (1.9, 1.0, 1.4), 0)
+    want, wclip, _ = oracle_demosaic(O, raw, RGGB, (1.9, 1.0, 1.4), 0, 512.0, white)
+    assert clip.sum() > 0
+    assert np.array_equal(clip, wclip)
+    assert maxabs(rgb, want) <= TOL
+
+
+def test_demosaic_strided_12bit(ctx, oracle_mod):
+    """12-bit frame (black 128, white 4095) read from a padded row stride."""
+    import darkslide_amd as D
+    O = oracle_mod
+    w, h = 300, 200
+    raw = synth(91, w, h, 12, 3.0)
+    padded = np.zeros((h, w + 13), np.uint16)
+    padded[:, :w] = raw
+    rgb = np.empty((h, w, 3), np.float32)
+    L = D.lib()
+    rc = L.ds_demosaic_u16(ctx.handle, D._ptr(padded), w, h, w + 13, D._pat(RGGB), 128.0, D._f3((4095,) * 3), D._f3((1, 1, 1)), 0,
+                           D._ptr(rgb), None)
+    assert rc == 0
+    want, _, _ = oracle_demosaic(O, raw, RGGB, (1, 1, 1), 0, 128.0, (4095.0,) * 3)
+    assert maxabs(rgb, want) <= TOL
+
+
+def test_demosaic_rejects_bad_input(ctx):
+    import darkslide_amd as D
+    raw = synth(5, 32, 32)
+    with pytest.raises(D.DarkslideError) as e:
+        ctx.demosaic_u16(raw, (0, 0, 1, 2))
+    assert e.value.code == -1
+    with pytest.raises(D.DarkslideError):
+        ctx.demosaic_u16(raw, RGGB, algo=7)
+
+
+# ------------------------------------------------------------------ noise / NLM
+
+@pytest.mark.parametrize("w,h", [(64, 64), (3, 3), (2, 9), (640, 480), (1024, 1024)])
+def test_estimate_noise_exact(ctx, oracle_mod, w, h):
+    O = oracle_mod
+    rng = np.random.default_rng(w * 7 + h)
+    p = rng.random((h, w)).astype(np.float32)
+    assert ctx.estimate_noise(p) == O.estimate_noise(p)
+
+
+def luminance_plane(O, seed, w, h, iso=1.0):
+    raw = synth(seed, w, h, 14, iso)
+    cfa = O.cfa_normalize_u16(raw, RGGB, 512.0, list(WHITE))
+    return np.ascontiguousarray(O.oklab_from_prophoto(O.rcd_bayer(cfa))[..., 0])
+
+
+@pytest.mark.parametrize("S,P", [(3, 1), (10, 3), (1, 0), (2, 2), (5, 5), (0, 2), (20, 5)])
+@pytest.mark.parametrize("w,h", [(96, 64), (130, 70), (7, 5)])
+def test_nlm(ctx, oracle_mod, S, P, w, h):
+    O = oracle_mod
+    L = luminance_plane(O, 300 + S * 10 + P, w, h, iso=4.0)
+    s = O.estimate_noise(L) if w * h > 100 else 0.01
+    hp = max(s, 1e-4) * 0.6
+    got = ctx.nlm(L, hp, S, P)
+    want = O.nlm(L, hp, S, P)
+    err = maxabs(got, want)
+    say(f"nlm S={S} P={P} {w}x{h} max-abs {err:.3g}")
+    assert err <= TOL
+
+
+def test_nlm_negative_search_is_zero(ctx):
+    p = np.random.default_rng(1).random((20, 30)).astype(np.float32)
+    assert np.array_equal(ctx.nlm(p, 0.1, -1, 1), np.zeros_like(p))
+
+
+def test_nlm_reference_test(ctx, oracle_mod):
+    """enhance.rs:137-151 on the GPU path."""
+    from test_oracle_pins import lcg_noise_plane
+    p, noisy = lcg_noise_plane()
+    s = ctx.estimate_noise(noisy)
+    assert 0.01 < s < 0.06
+    d = ctx.nlm(noisy, s * 1.5, 3, 1)
+    err = lambda q: float(((q.astype(np.float64) - p) ** 2).sum())
+    assert err(d) < err(noisy) * 0.5
+    assert maxabs(d, oracle_mod.nlm(noisy, s * 1.5, 3, 1)) <= TOL
+
+
+# ------------------------------------------------------------------ denoise / RL
+
+def rgb_frame(O, seed, w, h, iso=4.0):
+    raw = synth(seed, w, h, 14, iso)
+    cfa = O.cfa_normalize_u16(raw, RGGB, 512.0, list(WHITE))
+    return O.rcd_bayer(cfa)
+
+
+@pytest.mark.parametrize("amount,S,P", [(0.0, 10, 3), (37.0, 3, 1), (100.0, 3, 1)])
+def test_denoise(ctx, oracle_mod, amount, S, P):
+    O = oracle_mod
+    rgb = rgb_frame(O, 400, 256, 192, iso=8.0)
+    got = ctx.denoise(rgb, amount, S, P)
+    want = O.denoise(rgb, amount, S, P)
+    err = maxabs(got, want)
+    say(f"denoise amount={amount} S={S} P={P} max-abs {err:.3g}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("sigma,iters", [(1.0, 20), (0.75, 8), (0.5, 3), (1.9, 4), (0.2, 5), (2.5, 3), (1.0, 0)])
+def test_raw_details(ctx, oracle_mod, sigma, iters):
+    O = oracle_mod
+    rgb = rgb_frame(O, 500, 200, 150)
+    got = ctx.raw_details(rgb, sigma, iters)
+    want = O.raw_details(rgb, sigma, iters)
+    err = maxabs(got, want)
+    say(f"raw_details sigma={sigma} iters={iters} max-abs {err:.3g} bit-equal {np.mean(got == want):.6f}")
+    assert err <= TOL
+
+
+def test_pipeline_config1_with_fp64_error(ctx, oracle_mod):
+    """Demosaic -> denoise (NLM 7x7 patch, 21x21 search, h = 0.6 sigma) ->
+    RL (sigma 1, 20 iterations) at 1024x1024: each stage against the FP32
+    restatement (gate) and the FP64 oracle (reported)."""
+    O = oracle_mod
+    raw = synth(0x5EED0000, 1024, 1024)
+    rgb = ctx.demosaic_u16(raw, clip=False)
+    cfa = O.cfa_normalize_u16(raw, RGGB, 512.0, list(WHITE))
+    ref = O.rcd_bayer(cfa)
+    ref64 = O.rcd_bayer(cfa.astype(np.float64))
+    say(f"stage demosaic: vs fp32 {maxabs(rgb, ref):.3g}  vs fp64 {maxabs(rgb, ref64):.3g}")
+    assert maxabs(rgb, ref) <= TOL
+    dn = ctx.denoise(ref, 0.0, 10, 3)
+    dref = O.denoise(ref, 0.0, 10, 3)
+    dref64 = O.denoise(ref.astype(np.float64), 0.0, 10, 3)
+    say(f"stage denoise: vs fp32 {maxabs(dn, dref):.3g}  vs fp64 {maxabs(dn, dref64):.3g}")
+    assert maxabs(dn, dref) <= TOL
+    rl = ctx.raw_details(dref, 1.0, 20)
+    rref = O.raw_details(dref, 1.0, 20)
+    rref64 = O.raw_details(dref.astype(np.float64), 1.0, 20)
+    say(f"stage raw_details: vs fp32 {maxabs(rl, rref):.3g}  vs fp64 {maxabs(rl, rref64):.3g}")
+    assert maxabs(rl, rref) <= TOL
+
+
+# ------------------------------------------------------------------ 6000 x 4000
+
+@pytest.fixture(scope="module")
+def frame24(ctx):
+    import darkslide_amd as D
+    w, h = 6000, 4000
+    buf = ctx.alloc(w * h * 2)
+    ctx._check(D.lib().ds_synth_bayer_dev(ctx.handle, 0x5EED0001, w, h, 14, 1.0, D._pat(RGGB), buf.ptr))
+    raw = buf.download((h, w), np.uint16)
+    buf.free()
+    return raw
+
+
+def test_full_frame_demosaic(ctx, oracle_mod, frame24):
+    O = oracle_mod
+    raw = frame24
+    rgb, clip = ctx.demosaic_u16(raw)
+    want, wclip, _ = oracle_demosaic(O, raw, RGGB, (1, 1, 1), 0)
+    err = maxabs(rgb, want)
+    say(f"rcd 6000x4000 max-abs {err:.3g}, bit-equal {np.mean(rgb == want):.7f}")
+    assert err <= TOL
+    assert np.array_equal(clip, wclip)
+    mhc = ctx.demosaic_u16(raw, algo=1, clip=False)
+    assert maxabs(mhc, oracle_demosaic(O, raw, RGGB, (1, 1, 1), 1)[0]) <= TOL
+
+
+CROPS = [(0, 0), (2000, 1700), (5744, 3744), (0, 3000)]  # (x0, y0) of 256x256 crops
+
+
+def crop_check(full, fn, x0, y0, size, margin, w, h):
+    """Runs fn on the crop grown by margin (clipped to the frame) and compares
+    the crop's interior pixels with the full-frame result."""
+    cx0, cy0 = max(x0 - margin, 0), max(y0 - margin, 0)
+    cx1, cy1 = min(x0 + size + margin, w), min(y0 + size + margin, h)
+    out = fn(cx0, cy0, cx1, cy1)
+    sub = out[y0 - cy0:y0 - cy0 + size, x0 - cx0:x0 - cx0 + size]
+    return maxabs(full[y0:y0 + size, x0:x0 + size], sub)
+
+
+def test_full_frame_nlm_and_rl(ctx, oracle_mod, frame24):
+    O = oracle_mod
+    h, w = frame24.shape
+    rgb = ctx.demosaic_u16(frame24, clip=False)
+    L = np.ascontiguousarray(O.oklab_from_prophoto(rgb)[..., 0])
+    s = ctx.estimate_noise(L)
+    assert s == O.estimate_noise(L)
+    hp = max(s, 1e-4) * 0.6
+    lf = ctx.nlm(L, hp, 10, 3)
+    for (x0, y0) in CROPS:
+        err = crop_check(lf, lambda a, b, c, d: O.nlm(np.ascontiguousarray(L[b:d, a:c]), hp, 10, 3), x0, y0, 256, 14, w, h)
+        say(f"nlm 6000x4000 crop ({x0},{y0}) max-abs {err:.3g}")
+        assert err <= TOL
+    out = ctx.raw_details(rgb, 1.0, 20)
+    for (x0, y0) in CROPS:
+        err = crop_check(out, lambda a, b, c, d: O.raw_details(np.ascontiguousarray(rgb[b:d, a:c]), 1.0, 20), x0, y0, 256, 124, w, h)
+        say(f"raw_details 6000x4000 crop ({x0},{y0}) max-abs {err:.3g}")
+        assert err <= TOL
+    # RL is flux-preserving up to its clamps: the mean luminance barely moves
+    y_in = rgb @ np.array([0.2880402, 0.7118741, 0.0000857], np.float32)
+    y_out = out @ np.array([0.2880402, 0.7118741, 0.0000857], np.float32)
+    assert abs(float(y_out.mean()) / float(y_in.mean()) - 1) < 2e-3
