@@ -61,6 +61,12 @@ _SIGS = {
     "ds_denoise": [_P, _P, _SZ, _SZ, _F, _P],
     "ds_denoise_ex": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _P],
     "ds_raw_details": [_P, _P, _SZ, _SZ, _F, _SZ, _P],
+    "ds_box_blur": [_P, _P, _SZ, _SZ, _SZ, _P],
+    "ds_gaussian_blur": [_P, _P, _SZ, _SZ, _F, _P],
+    "ds_guided_filter": [_P, _P, _P, _SZ, _SZ, _SZ, _F, _P],
+    "ds_oklab_from_prophoto": [_P, _P, _SZ, _P],
+    "ds_oklab_to_prophoto": [_P, _P, _SZ, _P],
+    "ds_luminance": [_P, _P, _SZ, _P],
     "ds_dev_alloc": [_P, _SZ, ctypes.POINTER(_P)],
     "ds_dev_free": [_P, _P],
     "ds_memcpy_h2d": [_P, _P, _P, _SZ],
@@ -224,6 +230,46 @@ class Context:
         h, w, _ = rgb.shape
         out = np.empty_like(rgb)
         self._check(lib().ds_denoise_ex(self.handle, _ptr(rgb), w, h, float(amount), int(search), int(patch), _ptr(out)))
+        return out
+
+    def box_blur(self, plane, r):
+        plane = np.ascontiguousarray(plane, np.float32)
+        h, w = plane.shape
+        out = np.empty_like(plane)
+        self._check(lib().ds_box_blur(self.handle, _ptr(plane), w, h, int(r), _ptr(out)))
+        return out
+
+    def gaussian_blur(self, plane, sigma):
+        plane = np.ascontiguousarray(plane, np.float32)
+        h, w = plane.shape
+        out = np.empty_like(plane)
+        self._check(lib().ds_gaussian_blur(self.handle, _ptr(plane), w, h, float(sigma), _ptr(out)))
+        return out
+
+    def guided_filter(self, guide, src, r, eps):
+        guide = np.ascontiguousarray(guide, np.float32)
+        src = np.ascontiguousarray(src, np.float32)
+        h, w = guide.shape
+        out = np.empty_like(guide)
+        self._check(lib().ds_guided_filter(self.handle, _ptr(guide), _ptr(src), w, h, int(r), float(eps), _ptr(out)))
+        return out
+
+    def oklab_from_prophoto(self, rgb):
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        out = np.empty_like(rgb)
+        self._check(lib().ds_oklab_from_prophoto(self.handle, _ptr(rgb), rgb.size // 3, _ptr(out)))
+        return out
+
+    def oklab_to_prophoto(self, lab):
+        lab = np.ascontiguousarray(lab, np.float32)
+        out = np.empty_like(lab)
+        self._check(lib().ds_oklab_to_prophoto(self.handle, _ptr(lab), lab.size // 3, _ptr(out)))
+        return out
+
+    def luminance(self, rgb):
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        out = np.empty(rgb.shape[:-1], np.float32)
+        self._check(lib().ds_luminance(self.handle, _ptr(rgb), rgb.size // 3, _ptr(out)))
         return out
 
     def raw_details(self, rgb, sigma, iterations):

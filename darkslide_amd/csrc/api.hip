@@ -16,7 +16,9 @@
 #include "demosaic.h"
 #include "denoise.h"
 #include "nlm.h"
+#include "box.h"
 #include "rl.h"
+#include "stages.h"
 #include "synth.h"
 
 #pragma clang fp contract(off)
@@ -133,7 +135,7 @@ struct Buf {
     size_t bytes = 0;
 };
 
-enum Slot { S_IN = 0, S_OUT, S_CLIP, S_WS, S_L, S_A, S_B, S_LF, S_SIGMA, S_NOISE, S_SCENE, S_COUNT };
+enum Slot { S_IN = 0, S_IN2, S_OUT, S_CLIP, S_WS, S_L, S_A, S_B, S_LF, S_SIGMA, S_NOISE, S_SCENE, S_COUNT };
 
 struct Pending {
     std::string name;
@@ -321,7 +323,7 @@ int denoise_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, l
     SLOT(c, B, float*, S_B, n * 4);
     SLOT(c, LF, float*, S_LF, n * 4);
     SLOT(c, sig, float*, S_SIGMA, 64);
-    SLOT(c, ws, float*, S_WS, 10 * n * 4);
+    SLOT(c, ws, float*, S_WS, 12 * n * 4);
     int rc = run(c, "oklab_fwd", [&] { return launch_oklab_fwd(rgb, n, c->ok, L, A, B, c->stream); });
     if (rc) return rc;
     if ((rc = noise_dev(c, L, w, h, sig))) return rc;
@@ -343,6 +345,28 @@ int raw_details_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float sigma
     return run(c, "raw_details", [&] { return launch_raw_details(rgb, w, h, sigma, iters, ws, out, c->stream); });
 }
 
+}  // namespace
+
+namespace {
+// Upload in0 (and in1), run f(d_in0, d_in1, d_out), download out.
+template <class F>
+int host_op(ds_ctx* c, const char* name, const float* in0, const float* in1, size_t in_floats, float* out, size_t out_floats, F&& f) {
+    if (!in0 || !out) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (in_floats == 0 || out_floats == 0) return DS_OK;
+    SLOT(c, d0, float*, S_IN, in_floats * 4);
+    float* d1 = nullptr;
+    if (in1) {
+        SLOT(c, dd, float*, S_IN2, in_floats * 4);
+        d1 = dd;
+    }
+    SLOT(c, dout, float*, S_OUT, out_floats * 4);
+    HIPCHK(c, hipMemcpyAsync(d0, in0, in_floats * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (in1) HIPCHK(c, hipMemcpyAsync(d1, in1, in_floats * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = run(c, name, [&] { return f(d0, d1, dout); })) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, dout, out_floats * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -593,6 +617,53 @@ int ds_raw_details(ds_ctx* c, const float* rgb, size_t w, size_t h, float sigma,
     HIPCHK(c, hipMemcpyAsync(rgb_out, d_out, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;
+}
+
+// ---- plane primitives (synchronous)
+
+
+int ds_box_blur(ds_ctx* c, const float* in, size_t w, size_t h, size_t r, float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    SLOT(c, ws, float*, S_WS, (w * h ? w * h : 1) * 4);
+    return host_op(c, "box_blur", in, nullptr, w * h, out, w * h,
+                   [&](float* a, float*, float* o) { return launch_box_blur(a, ws, o, w, h, r, c->stream); });
+}
+
+int ds_gaussian_blur(ds_ctx* c, const float* in, size_t w, size_t h, float sigma, float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!(sigma >= 0.0f) || sigma > 1000.0f) return fail(c, DS_ERR_INVALID, "sigma %g out of range", (double)sigma);
+    SLOT(c, ws, float*, S_WS, (w * h ? 2 * w * h : 1) * 4);
+    return host_op(c, "gaussian_blur", in, nullptr, w * h, out, w * h,
+                   [&](float* a, float*, float* o) { return launch_gaussian_blur(a, ws, o, w, h, sigma, c->stream); });
+}
+
+int ds_guided_filter(ds_ctx* c, const float* guide, const float* src, size_t w, size_t h, size_t r, float eps, float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!src) return fail(c, DS_ERR_INVALID, "null pointer");
+    SLOT(c, ws, float*, S_WS, (w * h ? 8 * w * h : 1) * 4);
+    return host_op(c, "guided_filter", guide, src, w * h, out, w * h,
+                   [&](float* a, float* b, float* o) { return launch_guided_filter(a, b, w, h, r, eps, ws, o, c->stream); });
+}
+
+int ds_oklab_from_prophoto(ds_ctx* c, const float* rgb, size_t n, float* lab_out) {
+    LOCK(c);
+    return host_op(c, "oklab_fwd", rgb, nullptr, 3 * n, lab_out, 3 * n,
+                   [&](float* a, float*, float* o) { return launch_oklab_fwd_interleaved(a, n, c->ok, o, c->stream); });
+}
+
+int ds_oklab_to_prophoto(ds_ctx* c, const float* lab, size_t n, float* rgb_out) {
+    LOCK(c);
+    return host_op(c, "oklab_inv", lab, nullptr, 3 * n, rgb_out, 3 * n,
+                   [&](float* a, float*, float* o) { return launch_oklab_inv(a, n, c->ok, o, c->stream); });
+}
+
+int ds_luminance(ds_ctx* c, const float* rgb, size_t n, float* out) {
+    LOCK(c);
+    return host_op(c, "luminance", rgb, nullptr, 3 * n, out, n,
+                   [&](float* a, float*, float* o) { return launch_luminance(a, n, o, c->stream); });
 }
 
 // ---- profiling

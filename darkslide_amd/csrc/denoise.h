@@ -15,12 +15,9 @@ size_t noise_sample_count(size_t w, size_t h);
 // estimate_noise(plane) into *sigma_out (device).  scratch: noise_sample_count u32.
 hipError_t launch_estimate_noise(const float* plane, size_t w, size_t h, uint32_t* scratch, float* sigma_out, hipStream_t st);
 
-// box_blur(in, r) -> out; tmp: w*h floats.
-hipError_t launch_box_blur(const float* in, float* tmp, float* out, size_t w, size_t h, size_t r, hipStream_t st);
-
 // The two guided filters of denoise (enhance.rs:86-91) and the inverse Oklab:
 // I = L + (LF - L) m; a2 = gf(I, A), b2 = gf(I, B) with eps = (2 max(sigma,1e-4))^2;
-// rgb = max(Oklab^-1(I, a2, b2), 0).  ws: 10 planes of w*h floats.
+// rgb = max(Oklab^-1(I, a2, b2), 0).  ws: 12 planes of w*h floats.
 hipError_t launch_guided_pair(const float* L, const float* LF, const float* A, const float* B, float m, size_t r, const float* sigma_dev,
                               const OklabM& ok, float* ws, float* rgb, size_t w, size_t h, hipStream_t st);
 

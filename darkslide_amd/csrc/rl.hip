@@ -14,9 +14,10 @@
 // Other sigmas fall back to plane-wide passes: sigma < 0.3 (blur is a copy)
 // and sigma >= 2 (three box blurs, image.rs:334-337).
 
+#include "box.h"
 #include "common.h"
-#include "denoise.h"
 #include "rl.h"
+#include "stages.h"
 
 #pragma clang fp contract(off)
 
@@ -135,21 +136,6 @@ __global__ void k_rl_ratio(const float* __restrict__ y, const float* __restrict_
 __global__ void k_rl_update(float* __restrict__ est, const float* __restrict__ corr, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) est[i] = clampr(est[i] * corr[i], 0.0f, 64.0f);
-}
-
-// Wells' three-box radii (image.rs:363-373)
-void box_radii_for_gauss(float sigma, size_t radii[3]) {
-    const int n = 3;
-    float w_ideal = sqrtf(12.0f * sigma * sigma / (float)n + 1.0f);
-    int wl = (int)floorf(w_ideal);
-    if (wl % 2 == 0) wl -= 1;
-    int wu = wl + 2;
-    float m_ideal = (12.0f * sigma * sigma - (float)(n * wl * wl) - 4.0f * (float)n * (float)wl - 3.0f * (float)n) / (-4.0f * (float)wl - 4.0f);
-    int m = (int)roundf(m_ideal);
-    for (int i = 0; i < n; i++) {
-        int v = ((i < m ? wl : wu) - 1) / 2;
-        radii[i] = (size_t)(v > 0 ? v : 0);
-    }
 }
 
 template <int R>

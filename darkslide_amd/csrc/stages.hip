@@ -1,0 +1,190 @@
+// Plane-level primitives of the detail path, each a stand-alone stage of
+// the C ABI: gaussian_blur (image.rs:326-361), guided_filter
+// (image.rs:384-405), Oklab to ProPhoto (color.rs:415-418) and luminance
+// (image.rs:174-176).  All keep the reference's operation order (no FMA
+// contraction) and are bit-reproducible against it.
+
+#include "box.h"
+#include "common.h"
+#include "rl.h"
+#include "stages.h"
+
+#pragma clang fp contract(off)
+
+namespace ds {
+
+namespace {
+
+struct Taps13 {
+    float k[13];
+};
+
+// gaussian_blur_exact passes (image.rs:347-360): taps in order, clamped edges
+__global__ void k_gauss_h(const float* __restrict__ in, float* __restrict__ out, int w, int h, int r, Taps13 taps) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const size_t row = (size_t)y * (size_t)w;
+    float acc = 0.0f;
+    for (int j = 0; j <= 2 * r; j++) acc += taps.k[j] * in[row + (size_t)clampi(x + j - r, 0, w - 1)];
+    out[row + (size_t)x] = acc;
+}
+
+__global__ void k_gauss_v(const float* __restrict__ in, float* __restrict__ out, int w, int h, int r, Taps13 taps) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    float acc = 0.0f;
+    for (int j = 0; j <= 2 * r; j++) acc += taps.k[j] * in[(size_t)clampi(y + j - r, 0, h - 1) * (size_t)w + (size_t)x];
+    out[(size_t)y * (size_t)w + (size_t)x] = acc;
+}
+
+// image.rs:395-400 with a scalar eps
+__global__ void k_gf1_coef(const float* __restrict__ mi_, const float* __restrict__ mp_, const float* __restrict__ mip_,
+                           const float* __restrict__ mii_, size_t n, float eps, float* __restrict__ a, float* __restrict__ b) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float mi = mi_[i], mp = mp_[i];
+    const float cov = mip_[i] - mi * mp;
+    const float var = mii_[i] - mi * mi;
+    const float av = cov / (var + eps);
+    a[i] = av;
+    b[i] = mp - av * mi;
+}
+
+// image.rs:404
+__global__ void k_gf1_final(const float* __restrict__ I, const float* __restrict__ ma, const float* __restrict__ mb, size_t n,
+                            float* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = ma[i] * I[i] + mb[i];
+}
+
+__global__ void k_oklab_inv(const float* __restrict__ lab, size_t n, OklabM ok, float* __restrict__ rgb) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v[3] = {lab[3 * i], lab[3 * i + 1], lab[3 * i + 2]};
+    float l[3], cube[3], p[3];
+    mul_mv(ok.m2_inv, v, l);
+    cube[0] = l[0] * l[0] * l[0];
+    cube[1] = l[1] * l[1] * l[1];
+    cube[2] = l[2] * l[2] * l[2];
+    mul_mv(ok.from_lms, cube, p);
+    rgb[3 * i] = p[0];
+    rgb[3 * i + 1] = p[1];
+    rgb[3 * i + 2] = p[2];
+}
+
+__global__ void k_oklab_fwd_il(const float* __restrict__ rgb, size_t n, OklabM ok, float* __restrict__ lab) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p[3] = {rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]};
+    float lms[3], c[3], o[3];
+    mul_mv(ok.to_lms, p, lms);
+    c[0] = cbrtf(lms[0]);
+    c[1] = cbrtf(lms[1]);
+    c[2] = cbrtf(lms[2]);
+    mul_mv(ok.m2, c, o);
+    lab[3 * i] = o[0];
+    lab[3 * i + 1] = o[1];
+    lab[3 * i + 2] = o[2];
+}
+
+__global__ void k_luminance(const float* __restrict__ rgb, size_t n, float* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = 0.2880402f * rgb[3 * i] + 0.7118741f * rgb[3 * i + 1] + 0.0000857f * rgb[3 * i + 2];
+}
+
+unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+// Wells' three-box radii (image.rs:363-373)
+void box_radii_for_gauss(float sigma, size_t radii[3]) {
+    const int n = 3;
+    float w_ideal = sqrtf(12.0f * sigma * sigma / (float)n + 1.0f);
+    int wl = (int)floorf(w_ideal);
+    if (wl % 2 == 0) wl -= 1;
+    int wu = wl + 2;
+    float m_ideal = (12.0f * sigma * sigma - (float)(n * wl * wl) - 4.0f * (float)n * (float)wl - 3.0f * (float)n) / (-4.0f * (float)wl - 4.0f);
+    int m = (int)roundf(m_ideal);
+    for (int i = 0; i < n; i++) {
+        int v = ((i < m ? wl : wu) - 1) / 2;
+        radii[i] = (size_t)(v > 0 ? v : 0);
+    }
+}
+
+hipError_t launch_gaussian_blur(const float* in, float* ws, float* out, size_t w, size_t h, float sigma, hipStream_t st) {
+    const size_t n = w * h;
+    if (n == 0) return hipSuccess;
+    if (sigma < 0.3f) return hipMemcpyAsync(out, in, n * sizeof(float), hipMemcpyDeviceToDevice, st);
+    if (sigma < 2.0f) {
+        Taps13 t{};
+        int r = gaussian_taps(sigma, t.k);
+        dim3 grid((unsigned)((w + 255) / 256), (unsigned)h);
+        hipLaunchKernelGGL(k_gauss_h, grid, dim3(256), 0, st, in, ws, (int)w, (int)h, r, t);
+        hipLaunchKernelGGL(k_gauss_v, grid, dim3(256), 0, st, ws, out, (int)w, (int)h, r, t);
+        return hipGetLastError();
+    }
+    // three boxes (image.rs:334-337): ws holds 2 planes
+    size_t radii[3];
+    box_radii_for_gauss(sigma, radii);
+    float* a = ws;
+    float* b = ws + n;
+    // box_blur needs its scratch plane distinct from input and output
+    hipError_t e = launch_box_blur(in, out, a, w, h, radii[0], st);
+    if (e == hipSuccess) e = launch_box_blur(a, out, b, w, h, radii[1], st);
+    if (e == hipSuccess) e = launch_box_blur(b, a, out, w, h, radii[2], st);
+    return e;
+}
+
+hipError_t launch_guided_filter(const float* I, const float* P, size_t w, size_t h, size_t r, float eps, float* ws, float* out,
+                                hipStream_t st) {
+    const size_t n = w * h;
+    if (n == 0) return hipSuccess;
+    float* t[8];
+    for (int k = 0; k < 8; k++) t[k] = ws + (size_t)k * n;
+    BoxPlanes h4{}, v4{}, ab{}, hab{}, vab{};
+    CBoxPlanes c4{}, cab{}, chab{};
+    for (int k = 0; k < 4; k++) {
+        h4.p[k] = t[k];
+        c4.p[k] = t[k];
+        v4.p[k] = t[4 + k];
+    }
+    hipError_t e = box_h_guide1(BoxInGuide1{I, P}, h4, w, h, r, st);
+    if (e == hipSuccess) e = box_v_planes(c4, v4, 4, w, h, r, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gf1_coef, dim3(nblk(n)), dim3(256), 0, st, t[4], t[5], t[6], t[7], n, eps, t[0], t[1]);
+    cab.p[0] = t[0];
+    cab.p[1] = t[1];
+    hab.p[0] = t[2];
+    hab.p[1] = t[3];
+    chab.p[0] = t[2];
+    chab.p[1] = t[3];
+    vab.p[0] = t[4];
+    vab.p[1] = t[5];
+    (void)ab;
+    e = hipGetLastError();
+    if (e == hipSuccess) e = box_h_planes(cab, hab, 2, w, h, r, st);
+    if (e == hipSuccess) e = box_v_planes(chab, vab, 2, w, h, r, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gf1_final, dim3(nblk(n)), dim3(256), 0, st, I, t[4], t[5], n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_oklab_fwd_interleaved(const float* rgb, size_t n, const OklabM& ok, float* lab, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_oklab_fwd_il, dim3(nblk(n)), dim3(256), 0, st, rgb, n, ok, lab);
+    return hipGetLastError();
+}
+
+hipError_t launch_oklab_inv(const float* lab, size_t n, const OklabM& ok, float* rgb, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_oklab_inv, dim3(nblk(n)), dim3(256), 0, st, lab, n, ok, rgb);
+    return hipGetLastError();
+}
+
+hipError_t launch_luminance(const float* rgb, size_t n, float* out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_luminance, dim3(nblk(n)), dim3(256), 0, st, rgb, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace ds

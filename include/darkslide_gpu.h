@@ -87,6 +87,26 @@ int ds_denoise_ex(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float amoun
 /* enhance.rs:96 raw_details(img, sigma, iterations) */
 int ds_raw_details(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float sigma, size_t iterations, float* rgb_out);
 
+/* ---- plane primitives of the path --------------------------------------- */
+
+/* image.rs:269 box_blur(p, r): running-sum window mean, clamped edges. */
+int ds_box_blur(ds_ctx* ctx, const float* in, size_t w, size_t h, size_t r, float* out);
+
+/* image.rs:327 gaussian_blur(p, sigma): copy below 0.3, exact separable
+ * (image.rs:341 gaussian_blur_exact) below 2, else three box blurs. */
+int ds_gaussian_blur(ds_ctx* ctx, const float* in, size_t w, size_t h, float sigma, float* out);
+
+/* image.rs:385 guided_filter(guide, src, r, eps) */
+int ds_guided_filter(ds_ctx* ctx, const float* guide, const float* src, size_t w, size_t h, size_t r, float eps, float* out);
+
+/* color.rs:408 Oklab::from_prophoto / color.rs:415 Oklab::to_prophoto on n
+ * interleaved triples. */
+int ds_oklab_from_prophoto(ds_ctx* ctx, const float* rgb, size_t n, float* lab_out);
+int ds_oklab_to_prophoto(ds_ctx* ctx, const float* lab, size_t n, float* rgb_out);
+
+/* image.rs:174 Image3::luminance (color.rs:196 luma_prophoto) of n pixels. */
+int ds_luminance(ds_ctx* ctx, const float* rgb, size_t n, float* out);
+
 /* ---- device-resident variants (enqueue only) ---------------------------- */
 
 int ds_dev_alloc(ds_ctx* ctx, size_t bytes, void** dptr);

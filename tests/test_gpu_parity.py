@@ -160,15 +160,92 @@ def rgb_frame(O, seed, w, h, iso=4.0):
     return O.rcd_bayer(cfa)
 
 
+def thread_envelope(O, rgb, amount, S, P):
+    """max-abs spread of the reference's own denoise over rayon pool sizes
+    1..16: NLM's chunked sums (enhance.rs:43-70) change L by ~1e-6, and the
+    guided filter's variance term amplifies that (test_oracle_pins pins it)."""
+    runs = [O.denoise(rgb, amount, S, P, nthreads=t) for t in (1, 2, 4, 8, 16)]
+    return max(maxabs(a, b) for a in runs for b in runs)
+
+
 @pytest.mark.parametrize("amount,S,P", [(0.0, 10, 3), (37.0, 3, 1), (100.0, 3, 1)])
-def test_denoise(ctx, oracle_mod, amount, S, P):
+def test_denoise_stages(ctx, oracle_mod, amount, S, P):
+    """Every stage of denoise on identical inputs within 2e-5: Oklab split,
+    noise estimate, NLM, both guided filters, Oklab back."""
+    O = oracle_mod
+    rgb = rgb_frame(O, 400, 256, 192, iso=8.0)
+    lab = O.oklab_from_prophoto(rgb)
+    glab = ctx.oklab_from_prophoto(rgb)
+    assert maxabs(glab, lab) <= TOL
+    L, A, B = (np.ascontiguousarray(lab[..., i]) for i in range(3))
+    sigma = max(O.estimate_noise(L), np.float32(1e-4))
+    assert ctx.estimate_noise(L) == O.estimate_noise(L)
+    k = np.float32(amount) / np.float32(100)
+    hp = np.float32(sigma) * (np.float32(0.6) + np.float32(1.6) * k)
+    lf = O.nlm(L, hp, S, P)
+    assert maxabs(ctx.nlm(L, hp, S, P), lf) <= TOL
+    l2 = (L + (lf - L) * (np.float32(0.5) + np.float32(0.5) * k)).astype(np.float32)
+    r = int(max((np.float32(256) / np.float32(600)) * (np.float32(1) + np.float32(2) * k), np.float32(2)))
+    eps = np.float32(np.float32(sigma) * np.float32(2)) ** 2
+    a2, b2 = O.guided_filter(l2, A, r, eps), O.guided_filter(l2, B, r, eps)
+    ga2, gb2 = ctx.guided_filter(l2, A, r, eps), ctx.guided_filter(l2, B, r, eps)
+    say(f"guided filter on identical inputs: max-abs {maxabs(ga2, a2):.3g} / {maxabs(gb2, b2):.3g}")
+    assert maxabs(ga2, a2) <= TOL and maxabs(gb2, b2) <= TOL
+    back = np.stack([l2, a2, b2], -1).astype(np.float32)
+    assert maxabs(ctx.oklab_to_prophoto(back), O.oklab_to_prophoto(back)) <= TOL
+
+
+@pytest.mark.parametrize("amount,S,P", [(0.0, 10, 3), (37.0, 3, 1), (100.0, 3, 1)])
+def test_denoise_composite(ctx, oracle_mod, amount, S, P):
+    """The composite can only be held to the reference's own reproducibility:
+    its output moves by ~2-3e-4 between rayon pool sizes, so the GPU result
+    must fall inside that envelope, and be as close to the FP64 oracle as the
+    FP32 reference is."""
     O = oracle_mod
     rgb = rgb_frame(O, 400, 256, 192, iso=8.0)
     got = ctx.denoise(rgb, amount, S, P)
     want = O.denoise(rgb, amount, S, P)
-    err = maxabs(got, want)
-    say(f"denoise amount={amount} S={S} P={P} max-abs {err:.3g}")
-    assert err <= TOL
+    env = thread_envelope(O, rgb, amount, S, P)
+    f64 = O.denoise(rgb.astype(np.float64), amount, S, P)
+    err, e64, r64 = maxabs(got, want), maxabs(got, f64), maxabs(want, f64)
+    say(f"denoise amount={amount} S={S} P={P}: vs fp32 {err:.3g} (reference thread envelope {env:.3g}); "
+        f"vs fp64 {e64:.3g} (reference vs fp64 {r64:.3g})")
+    assert err <= max(TOL, env)
+    assert e64 <= max(TOL, 1.5 * r64)
+
+
+@pytest.mark.parametrize("r", [0, 1, 4, 10, 63, 70])
+def test_box_blur_exact(ctx, oracle_mod, r):
+    """image.rs:269 with the reference's running sums: bit-identical."""
+    rng = np.random.default_rng(r)
+    p = rng.random((150, 333)).astype(np.float32)
+    got, want = ctx.box_blur(p, r), oracle_mod.box_blur(p, r)
+    say(f"box_blur r={r}: bit-equal {np.mean(got == want):.6f}")
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("sigma", [0.2, 0.5, 0.75, 1.0, 1.9, 2.5, 7.0])
+def test_gaussian_blur_exact(ctx, oracle_mod, sigma):
+    rng = np.random.default_rng(int(sigma * 10))
+    p = rng.random((97, 210)).astype(np.float32)
+    assert np.array_equal(ctx.gaussian_blur(p, sigma), oracle_mod.gaussian_blur(p, sigma))
+
+
+def test_guided_filter_exact(ctx, oracle_mod):
+    rng = np.random.default_rng(2)
+    I = rng.random((120, 230)).astype(np.float32)
+    p = (0.4 * I + 0.2 * rng.random((120, 230))).astype(np.float32)
+    for r, eps in ((2, 1e-4), (10, 1e-3), (1, 0.0)):
+        assert np.array_equal(ctx.guided_filter(I, p, r, eps), oracle_mod.guided_filter(I, p, r, eps))
+
+
+def test_oklab_and_luminance(ctx, oracle_mod):
+    O = oracle_mod
+    rgb = rgb_frame(O, 600, 160, 120)
+    assert maxabs(ctx.oklab_from_prophoto(rgb), O.oklab_from_prophoto(rgb)) <= TOL
+    lab = O.oklab_from_prophoto(rgb)
+    assert np.array_equal(ctx.oklab_to_prophoto(lab), O.oklab_to_prophoto(lab))
+    assert np.array_equal(ctx.luminance(rgb), O.luminance(rgb))
 
 
 @pytest.mark.parametrize("sigma,iters", [(1.0, 20), (0.75, 8), (0.5, 3), (1.9, 4), (0.2, 5), (2.5, 3), (1.0, 0)])
@@ -197,8 +274,10 @@ def test_pipeline_config1_with_fp64_error(ctx, oracle_mod):
     dn = ctx.denoise(ref, 0.0, 10, 3)
     dref = O.denoise(ref, 0.0, 10, 3)
     dref64 = O.denoise(ref.astype(np.float64), 0.0, 10, 3)
-    say(f"stage denoise: vs fp32 {maxabs(dn, dref):.3g}  vs fp64 {maxabs(dn, dref64):.3g}")
-    assert maxabs(dn, dref) <= TOL
+    env = max(maxabs(O.denoise(ref, 0.0, 10, 3, nthreads=t), dref) for t in (1, 4, 16))
+    say(f"stage denoise: vs fp32 {maxabs(dn, dref):.3g} (thread envelope {env:.3g})  vs fp64 {maxabs(dn, dref64):.3g} "
+        f"(fp32 reference vs fp64 {maxabs(dref, dref64):.3g})")
+    assert maxabs(dn, dref) <= max(TOL, 1.5 * env)
     rl = ctx.raw_details(dref, 1.0, 20)
     rref = O.raw_details(dref, 1.0, 20)
     rref64 = O.raw_details(dref.astype(np.float64), 1.0, 20)
