@@ -55,5 +55,17 @@ def build(verbose=False):
     return LIB
 
 
+def build_mirror_smoke():
+    """Links tests/cpp/mirror_smoke against the library (plain g++, C++17)."""
+    src = os.path.join(ROOT, "tests", "cpp", "mirror_smoke.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "mirror_smoke")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), src, "-o", out,
+           "-L" + HERE, "-ldarkslide_gpu", "-Wl,-rpath," + "$ORIGIN/../../darkslide_amd", "-pthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("mirror_smoke build failed:\n" + r.stderr)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
