@@ -10,12 +10,14 @@
 // summation order moves its output by ~1e-4 (more than the whole error budget).
 //
 // Layout: one wave owns 64 rows (horizontal) or 64 columns (vertical) of one
-// plane and walks them in 64-element chunks.  Chunks are staged through LDS
-// with coalesced loads (three chunk slots: the window spans the previous,
-// current and next chunk for r < 64); the next-but-one chunk is fetched into
-// registers while the current one is walked.  Rows/columns x planes give the
-// parallelism: ~(h/64) * planes waves for the horizontal pass and
-// ~(w/64) * planes waves for the vertical one.
+// plane and walks them in 64-element chunks.  Chunks are staged through an
+// LDS ring with coalesced loads (three chunk slots: the window spans the
+// previous, current and next chunk for r < 64); the next-but-one chunk is
+// fetched into registers while the current one is walked.  Within a chunk
+// the window differences are gathered first, so the serial part is one
+// dependent add per element.  Rows/columns x planes give the parallelism:
+// ~(h/64) * planes waves for the horizontal pass and ~(w/64) * planes waves
+// for the vertical one.
 
 #include "box.h"
 #include "common.h"
@@ -26,118 +28,146 @@ namespace ds {
 
 namespace {
 
-constexpr int BX = 64;  // chunk and group size
+constexpr int BX = 64;       // chunk length and rows / columns per wave
+constexpr int RING = 3 * BX;  // three resident chunks: previous, current, next
 
-// Horizontal pass.  grid: (ceil(h/64), planes); block: 64 threads.
+// Horizontal pass.  grid: (ceil(h/64), planes); block: one wave.  Lane t owns
+// row y0 + t.  Per 64-column chunk c (chunks c-1, c, c+1 resident in the
+// ring, x held at ring position x mod 192):
+//   1. gather d[j] = in[x+r+1] - in[x-r] for the chunk into registers;
+//   2. run the 64 dependent additions acc += d[j], keeping acc * norm;
+//   3. stage the outputs in the dead slot of chunk c-1 and store them
+//      coalesced, then refill that slot with chunk c+2, fetched into
+//      registers before step 1.
 template <class In>
-__global__ __launch_bounds__(BX) void k_boxx_h(In in, BoxPlanes out, int w, int h, int r, float norm) {
-    __shared__ float tile[3][BX][BX + 1];
-    __shared__ float otile[BX][BX + 1];
+__global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_boxx_h(In in, BoxPlanes out, int w, int h, int r, float norm) {
+    __shared__ float ring[BX][RING + 1];  // odd row stride: lanes of one column hit distinct banks
     const int t = threadIdx.x;
     const int y0 = blockIdx.x * BX;
     const int k = blockIdx.y;
-    const int ymine = min(y0 + t, h - 1);
-    const size_t myrow = (size_t)ymine * (size_t)w;
+    const size_t myrow = (size_t)min(y0 + t, h - 1) * (size_t)w;
     const int ntiles = (w + BX - 1) / BX;
-
-    // chunk c of the 64 rows: thread t fetches column 64c + t of every row
     float pre[BX];
     auto fetch = [&](int c) {
         const int xx = c * BX + t;
+        const uint32_t base = (uint32_t)y0 * (uint32_t)w + (uint32_t)xx;
 #pragma unroll
         for (int j = 0; j < BX; j++) {
             const int yy = y0 + j;
-            pre[j] = (yy < h && xx < w) ? in.get((size_t)yy * (size_t)w + (size_t)xx, k) : 0.0f;
+            pre[j] = (yy < h && xx < w) ? in.get(base + (uint32_t)j * (uint32_t)w, k) : 0.0f;
         }
     };
     auto commit = [&](int c) {
-        float(*s)[BX + 1] = tile[c % 3];
+        const int base = (c % 3) * BX + t;
 #pragma unroll
-        for (int j = 0; j < BX; j++) s[j][t] = pre[j];
+        for (int j = 0; j < BX; j++) ring[j][base] = pre[j];
     };
-
     fetch(0);
     commit(0);
     if (ntiles > 1) {
         fetch(1);
         commit(1);
     }
-    // window sum at x = 0, accumulated in the reference's order
+    // window sum at x = 0 in the reference's order
     float acc = 0.0f;
     for (int i = -r; i <= r; i++) acc += in.get(myrow + (size_t)clampi(i, 0, w - 1), k);
     __syncthreads();
-
+    const float* row = ring[t];
     for (int c = 0; c < ntiles; c++) {
         if (c + 2 < ntiles) fetch(c + 2);
         const int xb = c * BX;
-#pragma unroll 8
+        float d[BX];
+#pragma unroll
         for (int j = 0; j < BX; j++) {
             const int x = xb + j;
-            otile[t][j] = acc * norm;
             const int xn = min(x + r + 1, w - 1), xo = max(x - r, 0);
-            const float vn = tile[(xn / BX) % 3][t][xn % BX];
-            const float vo = tile[(xo / BX) % 3][t][xo % BX];
-            acc += vn - vo;
+            d[j] = row[xn % RING] - row[xo % RING];
         }
+#pragma unroll
+        for (int j = 0; j < BX; j++) {
+            const float dj = d[j];
+            d[j] = acc * norm;
+            acc += dj;
+        }
+        // chunk c-1 is no longer read: stage the outputs in its slot
+        const int stage = ((c + 2) % 3) * BX;
+#pragma unroll
+        for (int j = 0; j < BX; j++) ring[t][stage + j] = d[j];
         __syncthreads();
-        {
-            const int xx = xb + t;
-            if (xx < w)
-#pragma unroll 8
-                for (int j = 0; j < BX; j++) {
-                    const int yy = y0 + j;
-                    if (yy < h) out.p[k][(size_t)yy * (size_t)w + (size_t)xx] = otile[j][t];
-                }
-        }
+        const int xx = xb + t;
+        const uint32_t obase = (uint32_t)y0 * (uint32_t)w + (uint32_t)xx;
+        float* dst = out.p[k];
+        if (xx < w)
+#pragma unroll 16
+            for (int j = 0; j < BX; j++) {
+                const int yy = y0 + j;
+                if (yy < h) dst[obase + (uint32_t)j * (uint32_t)w] = ring[j][stage + t];
+            }
+        __syncthreads();
         if (c + 2 < ntiles) commit(c + 2);
         __syncthreads();
     }
 }
 
-// Vertical pass.  grid: (ceil(w/64), planes); block: 64 threads.  Each thread
-// walks one column; it only ever touches its own LDS column, so no barrier is
-// needed between staging and walking.
-__global__ __launch_bounds__(BX) void k_boxx_v(CBoxPlanes in, BoxPlanes out, int w, int h, int r, float norm) {
-    __shared__ float tile[3][BX][BX];
+// Vertical pass.  grid: (ceil(w/64), planes); block: one wave.  Lane t walks
+// column x0 + t; it only touches its own LDS column (ring[row][t]), so no
+// barriers are needed.  Same chunk scheme as the horizontal pass, with the
+// outputs stored straight to global memory (a wave's stores cover one
+// contiguous row piece).
+__global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_boxx_v(CBoxPlanes in, BoxPlanes out, int w, int h, int r, float norm) {
+    __shared__ float ring[RING][BX];
     const int t = threadIdx.x;
-    const int x = blockIdx.x * BX + t;
+    const int x = min(blockIdx.x * BX + t, w - 1);  // lanes past the edge shadow the last column
+    const bool live = blockIdx.x * BX + t < w;
     const int k = blockIdx.y;
-    if (x >= w) return;
-    const float* src = in.p[k];
-    float* dst = out.p[k];
+    const float* __restrict__ src = in.p[k];
+    float* __restrict__ dst = out.p[k];
     const int ntiles = (h + BX - 1) / BX;
+    const uint32_t uw = (uint32_t)w;
     float pre[BX];
-    auto fetch = [&](int c) {
+    // chunk 0 and 1 into the ring; 32-bit element offsets off a uniform base
+    // (planes stay below 2^31 elements): one VGPR per in-flight address
+    for (int c = 0; c < 2 && c < ntiles; c++) {
+        const uint32_t base = (uint32_t)(c * BX) * uw + (uint32_t)x;
 #pragma unroll
-        for (int j = 0; j < BX; j++) {
-            const int yy = c * BX + j;
-            pre[j] = yy < h ? src[(size_t)yy * (size_t)w + (size_t)x] : 0.0f;
-        }
-    };
-    auto commit = [&](int c) {
-        float(*s)[BX] = tile[c % 3];
+        for (int j = 0; j < BX; j++) pre[j] = c * BX + j < h ? src[base + (uint32_t)j * uw] : 0.0f;
 #pragma unroll
-        for (int j = 0; j < BX; j++) s[j][t] = pre[j];
-    };
-    fetch(0);
-    commit(0);
-    if (ntiles > 1) {
-        fetch(1);
-        commit(1);
+        for (int j = 0; j < BX; j++) ring[c * BX + j][t] = pre[j];
     }
     float acc = 0.0f;
-    for (int i = -r; i <= r; i++) acc += src[(size_t)clampi(i, 0, h - 1) * (size_t)w + (size_t)x];
+    for (int i = -r; i <= r; i++) acc += src[(uint32_t)clampi(i, 0, h - 1) * uw + (uint32_t)x];
     for (int c = 0; c < ntiles; c++) {
-        if (c + 2 < ntiles) fetch(c + 2);
+        const bool more = c + 2 < ntiles;
+        if (more) {
+            const uint32_t base = (uint32_t)((c + 2) * BX) * uw + (uint32_t)x;
+#pragma unroll
+            for (int j = 0; j < BX; j++) pre[j] = (c + 2) * BX + j < h ? src[base + (uint32_t)j * uw] : 0.0f;
+        }
         const int yb = c * BX;
-#pragma unroll 8
+        float d[BX];
+#pragma unroll
         for (int j = 0; j < BX; j++) {
             const int y = yb + j;
-            if (y < h) dst[(size_t)y * (size_t)w + (size_t)x] = acc * norm;
             const int yn = min(y + r + 1, h - 1), yo = max(y - r, 0);
-            acc += tile[(yn / BX) % 3][yn % BX][t] - tile[(yo / BX) % 3][yo % BX][t];
+            d[j] = ring[yn % RING][t] - ring[yo % RING][t];
         }
-        if (c + 2 < ntiles) commit(c + 2);
+#pragma unroll
+        for (int j = 0; j < BX; j++) {
+            const float dj = d[j];
+            d[j] = acc * norm;
+            acc += dj;
+        }
+        if (live) {
+            const uint32_t obase = (uint32_t)yb * uw + (uint32_t)x;
+#pragma unroll
+            for (int j = 0; j < BX; j++)
+                if (yb + j < h) dst[obase + (uint32_t)j * uw] = d[j];
+        }
+        if (more) {
+            const int slot = ((c + 2) % 3) * BX;
+#pragma unroll
+            for (int j = 0; j < BX; j++) ring[slot + j][t] = pre[j];
+        }
     }
 }
 

@@ -117,15 +117,17 @@ struct RcdLds {
     uint8_t flag[Tile<TX, TY, H_FLAG>::N];
 };
 
-template <int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_rcd(CfaSrc src, float* __restrict__ rgb, float* __restrict__ clip) {
-    static_assert((TX % 2) == 0 && (TY % 2) == 0, "tile origin must keep Bayer parity");
-    static_assert(2 * Tile<TX, TY, H_RB>::N <= 2 * Tile<TX, TY, H_BVH>::N, "r43/b43 alias");
-    __shared__ RcdLds<TX, TY> L;
+// Tiles whose 10-pixel halo lies inside [4, w-4) x [4, h-4) (INTERIOR) take
+// every stage's RCD branch, so their body drops the per-pixel border checks.
+// Stages that only touch R/B sites walk the checkerboard of those sites.
+template <int TX, int TY, int NT, bool INTERIOR>
+__device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, float* __restrict__ rgb, float* __restrict__ clip) {
     const int tid = threadIdx.x;
     const int ox = blockIdx.x * TX, oy = blockIdx.y * TY;
     const int w = src.w, h = src.h;
     const Bayer pat = src.pat;
+    // R/B sites: ((ox + tx) + (oy + ty)) & 1 == rbp, and ox, oy are even
+    const int rbp = pat.rb_parity();
 
     Tile<TX, TY, H_CM> cm{L.cm};
     Tile<TX, TY, H_BVH> bv{L.scratch_a}, bh{L.scratch_a + Tile<TX, TY, H_BVH>::N};
@@ -136,8 +138,8 @@ __global__ __launch_bounds__(NT) void k_rcd(CfaSrc src, float* __restrict__ rgb,
     Tile<TX, TY, H_G> gg{L.g};
     Tile<TX, TY, H_PQ> pq{L.pq};
 
-    auto in_img = [&](int gx, int gy) { return gx >= 0 && gx < w && gy >= 0 && gy < h; };
-    auto in_box = [&](int gx, int gy, int b) { return gx >= b && gx < w - b && gy >= b && gy < h - b; };
+    auto in_img = [&](int gx, int gy) { return INTERIOR || (gx >= 0 && gx < w && gy >= 0 && gy < h); };
+    auto in_box = [&](int gx, int gy, int b) { return INTERIOR || (gx >= b && gx < w - b && gy >= b && gy < h - b); };
     auto cval = [&](int x, int y) { return fmaxr(cm(x, y), 0.0f); };
 
     // ---- stage 0: load cfa*mul (and clip flags) with clamped coordinates
@@ -194,17 +196,23 @@ __global__ __launch_bounds__(NT) void k_rcd(CfaSrc src, float* __restrict__ rgb,
     }
     __syncthreads();
 
-    // ---- stage 3: green at R/B (demosaic.rs:222-245), MHC green off the
-    // border box; bp / bq (demosaic.rs:248-260) into the freed bv/bh space.
-    for (int i = tid; i < Tile<TX, TY, H_G>::N; i += NT) {
-        int tx = i % Tile<TX, TY, H_G>::W - H_G, ty = i / Tile<TX, TY, H_G>::W - H_G;
-        int gx = ox + tx, gy = oy + ty;
+    // ---- stage 3: green (demosaic.rs:222-245).  G sites copy c; R/B sites
+    // in the border box get the RCD estimate, outside it the MHC value.  The
+    // diagonal statistics bp / bq follow in the same phase.
+    constexpr int GW = Tile<TX, TY, H_G>::W, GH = Tile<TX, TY, H_G>::R;
+    for (int i = tid; i < (GW / 2) * GH; i += NT) {
+        const int ty = i / (GW / 2) - H_G;
+        const int off = (rbp + H_G + ty) & 1;  // first R/B column of this row
+        const int tx_rb = -H_G + 2 * (i % (GW / 2)) + off;
+        const int tx_g = -H_G + 2 * (i % (GW / 2)) + (off ^ 1);
+        const int gy = oy + ty;
+        // green site
+        gg(tx_g, ty) = in_img(ox + tx_g, gy) ? cval(tx_g, ty) : 0.0f;
+        // R/B site
+        const int tx = tx_rb, gx = ox + tx;
         float gv = 0.0f;
         if (in_img(gx, gy)) {
-            int col = pat.at(gx, gy);
-            if (col == 1) {
-                gv = cval(tx, ty);
-            } else if (in_box(gx, gy, kBorder)) {
+            if (in_box(gx, gy, kBorder)) {
                 float vd = disc(vh, tx, ty);
                 auto c = [&](int dx, int dy) { return cval(tx + dx, ty + dy); };
                 float n_grad = kEps + fabsf(c(0, -1) - c(0, 1)) + fabsf(c(0, 0) - c(0, -2)) + fabsf(c(0, -1) - c(0, -3)) + fabsf(c(0, -2) - c(0, -4));
@@ -220,12 +228,12 @@ __global__ __launch_bounds__(NT) void k_rcd(CfaSrc src, float* __restrict__ rgb,
                 float h_est = (w_grad * e_est + e_grad * w_est) / (e_grad + w_grad);
                 gv = fmaxr(vd * h_est + (1.0f - vd) * v_est, 0.0f);
             } else {
-                gv = mhc_at(cm, tx, ty, col, pat.at(gx + 1, gy), 1);
+                gv = mhc_at(cm, tx, ty, pat.at(gx, gy), pat.at(gx + 1, gy), 1);
             }
         }
         gg(tx, ty) = gv;
     }
-    __syncthreads();
+    // bp / bq (demosaic.rs:248-260) into the bv/bh space, free since stage 2
     for (int i = tid; i < Tile<TX, TY, H_G>::N; i += NT) {
         int tx = i % Tile<TX, TY, H_G>::W - H_G, ty = i / Tile<TX, TY, H_G>::W - H_G;
         int gx = ox + tx, gy = oy + ty;
@@ -259,40 +267,40 @@ __global__ __launch_bounds__(NT) void k_rcd(CfaSrc src, float* __restrict__ rgb,
     // diagonal estimate, outside it the MHC value; symmetrically for b.  The
     // pre-pass channel at the diagonal neighbours of a B (R) site is an R (B)
     // site, i.e. c itself.
-    for (int i = tid; i < Tile<TX, TY, H_RB>::N; i += NT) {
-        int tx = i % Tile<TX, TY, H_RB>::W - H_RB, ty = i / Tile<TX, TY, H_RB>::W - H_RB;
-        int gx = ox + tx, gy = oy + ty;
+    constexpr int QW = Tile<TX, TY, H_RB>::W, QR = Tile<TX, TY, H_RB>::R;
+    for (int i = tid; i < (QW / 2) * QR; i += NT) {
+        const int ty = i / (QW / 2) - H_RB;
+        const int tx = -H_RB + 2 * (i % (QW / 2)) + ((rbp + H_RB + ty) & 1);
+        const int gx = ox + tx, gy = oy + ty;
         float rv = 0.0f, bvv = 0.0f;
         if (in_img(gx, gy)) {
-            int col = pat.at(gx, gy);
-            if (col != 1) {
-                float own = cval(tx, ty);
-                float other;
-                if (in_box(gx, gy, kBorder)) {
-                    float pd = disc(pq, tx, ty);
-                    auto cc = [&](int dx, int dy) { return cval(tx + dx, ty + dy); };
-                    auto g = [&](int dx, int dy) { return gg(tx + dx, ty + dy); };
-                    float nw_g = kEps + fabsf(cc(-1, -1) - cc(1, 1)) + fabsf(cc(-1, -1) - cc(-3, -3)) + fabsf(g(0, 0) - g(-2, -2));
-                    float ne_g = kEps + fabsf(cc(1, -1) - cc(-1, 1)) + fabsf(cc(1, -1) - cc(3, -3)) + fabsf(g(0, 0) - g(2, -2));
-                    float sw_g = kEps + fabsf(cc(-1, 1) - cc(1, -1)) + fabsf(cc(-1, 1) - cc(-3, 3)) + fabsf(g(0, 0) - g(-2, 2));
-                    float se_g = kEps + fabsf(cc(1, 1) - cc(-1, -1)) + fabsf(cc(1, 1) - cc(3, 3)) + fabsf(g(0, 0) - g(2, 2));
-                    float nw_e = cc(-1, -1) - g(-1, -1);
-                    float ne_e = cc(1, -1) - g(1, -1);
-                    float sw_e = cc(-1, 1) - g(-1, 1);
-                    float se_e = cc(1, 1) - g(1, 1);
-                    float p_est = (nw_g * se_e + se_g * nw_e) / (nw_g + se_g);
-                    float q_est = (ne_g * sw_e + sw_g * ne_e) / (ne_g + sw_g);
-                    other = fmaxr(g(0, 0) + (1.0f - pd) * p_est + pd * q_est, 0.0f);
-                } else {
-                    other = mhc_at(cm, tx, ty, col, pat.at(gx + 1, gy), 2 - col);
-                }
-                if (col == 0) {
-                    rv = own;
-                    bvv = other;
-                } else {
-                    rv = other;
-                    bvv = own;
-                }
+            const int col = pat.at(gx, gy);
+            float own = cval(tx, ty);
+            float other;
+            if (in_box(gx, gy, kBorder)) {
+                float pd = disc(pq, tx, ty);
+                auto cc = [&](int dx, int dy) { return cval(tx + dx, ty + dy); };
+                auto g = [&](int dx, int dy) { return gg(tx + dx, ty + dy); };
+                float nw_g = kEps + fabsf(cc(-1, -1) - cc(1, 1)) + fabsf(cc(-1, -1) - cc(-3, -3)) + fabsf(g(0, 0) - g(-2, -2));
+                float ne_g = kEps + fabsf(cc(1, -1) - cc(-1, 1)) + fabsf(cc(1, -1) - cc(3, -3)) + fabsf(g(0, 0) - g(2, -2));
+                float sw_g = kEps + fabsf(cc(-1, 1) - cc(1, -1)) + fabsf(cc(-1, 1) - cc(-3, 3)) + fabsf(g(0, 0) - g(-2, 2));
+                float se_g = kEps + fabsf(cc(1, 1) - cc(-1, -1)) + fabsf(cc(1, 1) - cc(3, 3)) + fabsf(g(0, 0) - g(2, 2));
+                float nw_e = cc(-1, -1) - g(-1, -1);
+                float ne_e = cc(1, -1) - g(1, -1);
+                float sw_e = cc(-1, 1) - g(-1, 1);
+                float se_e = cc(1, 1) - g(1, 1);
+                float p_est = (nw_g * se_e + se_g * nw_e) / (nw_g + se_g);
+                float q_est = (ne_g * sw_e + sw_g * ne_e) / (ne_g + sw_g);
+                other = fmaxr(g(0, 0) + (1.0f - pd) * p_est + pd * q_est, 0.0f);
+            } else {
+                other = mhc_at(cm, tx, ty, col, pat.at(gx + 1, gy), 2 - col);
+            }
+            if (col == 0) {
+                rv = own;
+                bvv = other;
+            } else {
+                rv = other;
+                bvv = own;
             }
         }
         r43(tx, ty) = rv;
@@ -354,6 +362,20 @@ __global__ __launch_bounds__(NT) void k_rcd(CfaSrc src, float* __restrict__ rgb,
     }
 }
 
+template <int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_rcd(CfaSrc src, float* __restrict__ rgb, float* __restrict__ clip) {
+    static_assert((TX % 2) == 0 && (TY % 2) == 0, "tile origin must keep Bayer parity");
+    static_assert(2 * Tile<TX, TY, H_RB>::N <= 2 * Tile<TX, TY, H_BVH>::N, "r43/b43 alias");
+    __shared__ RcdLds<TX, TY> L;
+    const int ox = blockIdx.x * TX, oy = blockIdx.y * TY;
+    const bool interior = ox - H_CM >= kBorder && oy - H_CM >= kBorder && ox + TX + H_CM <= src.w - kBorder &&
+                          oy + TY + H_CM <= src.h - kBorder;
+    if (interior)
+        rcd_body<TX, TY, NT, true>(L, src, rgb, clip);
+    else
+        rcd_body<TX, TY, NT, false>(L, src, rgb, clip);
+}
+
 // Plain MHC over the whole frame (demosaic.rs:89-124) plus the clip plane:
 // a TX x TY tile with a 2-pixel clamped halo of cfa*mul in LDS.
 template <int TX, int TY, int NT>
@@ -394,7 +416,7 @@ __global__ __launch_bounds__(NT) void k_mhc(CfaSrc src, float* __restrict__ rgb,
     }
 }
 
-constexpr int RCD_TX = 64, RCD_TY = 32, RCD_NT = 256;
+constexpr int RCD_TX = 64, RCD_TY = 32, RCD_NT = 512;
 constexpr int MHC_TX = 64, MHC_TY = 16, MHC_NT = 256;
 
 }  // namespace

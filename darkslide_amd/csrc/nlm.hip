@@ -6,7 +6,7 @@
 // tile and loops over all offsets with the running (acc, wsum) of its pixels
 // in registers.  The input tile plus an (S + P)-pixel halo sits in LDS.  Per
 // offset:
-//   phase 1: every thread takes 8-column row segments of the tile rows
+//   phase 1: every thread takes two 8-column row segments of the tile rows
 //            [-P, TH+P), forms the squared differences and their horizontal
 //            (2P+1)-sums with a sliding window, and writes them to LDS;
 //   phase 2: every thread slides a vertical (2P+1)-window down one column
@@ -32,105 +32,136 @@ namespace ds {
 
 namespace {
 
-constexpr int NLM_TW = 64;
-constexpr int NLM_TH = 64;
+constexpr int NLM_TW = 64;        // tile columns
+constexpr int NLM_HR = 64;        // rows of horizontal sums per tile (TH + 2P)
 constexpr int NLM_NT = 256;
 constexpr int NLM_SEG = 8;        // columns per phase-1 task
-constexpr int NLM_ROWS = NLM_TH / (NLM_NT / NLM_TW);  // rows per phase-2 thread
+constexpr int NLM_HS = NLM_TW + 4;  // padded row stride of the horizontal-sum buffer
 
+__host__ __device__ constexpr int nlm_th(int P) { return NLM_HR - 2 * P; }
+// odd row stride of the input tile: lanes walking down one column hit distinct banks
+__host__ __device__ inline int nlm_pw(int SP) { return (NLM_TW + 2 * SP) | 1; }
+__host__ __device__ inline int nlm_pt_floats(int SP, int P) { return ((nlm_th(P) + 2 * SP) * nlm_pw(SP) + 3) & ~3; }
+
+// Tile kernel.  Phase 1: thread t owns hs row (t % 64) and the 8-column
+// segments (t / 64) and (t / 64 + 4): 512 tasks, exactly two per thread.  The
+// centre samples of its two segments do not depend on the offset and stay in
+// registers for the whole offset loop.  Phase 2: thread t owns column t % 64
+// and a quarter of the tile rows.
 template <int P>
-__global__ __launch_bounds__(NLM_NT) void k_nlm_tile(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S,
-                                                     NlmWeight wp) {
-    const float wscale = wp.scale();
+__global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S,
+                                                        NlmWeight wp) {
     extern __shared__ float lds[];
-    constexpr int TW = NLM_TW, TH = NLM_TH, NT = NLM_NT;
-    constexpr int HR = TH + 2 * P;  // rows of horizontal sums
+    constexpr int TW = NLM_TW, TH = nlm_th(P), NT = NLM_NT, SEG = NLM_SEG, HS = NLM_HS;
+    constexpr int DW = SEG + 2 * P;           // differences per phase-1 task
+    constexpr int RPT = (TH + 3) / 4;         // phase-2 rows per thread
+    const float wscale = wp.scale();
     const int SP = S + P;
-    const int PW = TW + 2 * SP;  // input tile row length
-    float* pt = lds;                          // input tile, (TH + 2SP) x PW
-    float* hs0 = lds + (TH + 2 * SP) * PW;    // horizontal sums, 2 x HR x TW
+    const int PWL = TW + 2 * SP;              // loaded tile row length
+    const int PW = nlm_pw(SP);
+    const int PH = TH + 2 * SP;
+    float* pt = lds;                          // input tile, PH x PW
+    float* hs0 = lds + nlm_pt_floats(SP, P);  // horizontal sums, 2 x 64 x HS
     const int tid = threadIdx.x;
     const int ox = blockIdx.x * TW, oy = blockIdx.y * TH;
 
     // input tile with clamped coordinates: slot (u, v) holds p[clamp(ox+u), clamp(oy+v)]
-    for (int i = tid; i < (TH + 2 * SP) * PW; i += NT) {
-        int u = i % PW - SP, v = i / PW - SP;
+    for (int i = tid; i < PH * PWL; i += NT) {
+        int u = i % PWL - SP, v = i / PWL - SP;
         int gx = clampi(ox + u, 0, w - 1), gy = clampi(oy + v, 0, h - 1);
-        pt[i] = in[(size_t)gy * w + gx];
+        pt[(v + SP) * PW + (u + SP)] = in[(size_t)gy * w + gx];
     }
     auto P_at = [&](int u, int v) -> float { return pt[(v + SP) * PW + (u + SP)]; };
     const bool interior = ox - SP >= 0 && oy - SP >= 0 && ox + TW + SP <= w && oy + TH + SP <= h;
+    __syncthreads();
 
-    const int col = tid % TW;
-    const int row0 = (tid / TW) * NLM_ROWS;
-    float acc[NLM_ROWS], wsum[NLM_ROWS];
+    // phase-1 geometry
+    const int v1 = tid % NLM_HR - P;  // tile row of this thread's horizontal sums
+    const int u0a = (tid / NLM_HR) * SEG, u0b = u0a + 4 * SEG;
+    // window positions (tile coords) of the two segments; at the frame edge the
+    // difference plane is evaluated at the clamped position (box_blur's clamp)
+    auto cu_of = [&](int u) { return interior ? u : clampi(ox + u, 0, w - 1) - ox; };
+    const int cv = interior ? v1 : clampi(oy + v1, 0, h - 1) - oy;
+    float ca[DW], cb[DW];
 #pragma unroll
-    for (int k = 0; k < NLM_ROWS; k++) {
+    for (int i = 0; i < DW; i++) {
+        ca[i] = P_at(cu_of(u0a - P + i), cv);
+        cb[i] = P_at(cu_of(u0b - P + i), cv);
+    }
+
+    // phase-2 geometry
+    const int col = tid % TW;
+    const int row0 = (tid / TW) * RPT;
+    float acc[RPT], wsum[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
         acc[k] = 0.0f;
         wsum[k] = 0.0f;
     }
-    __syncthreads();
 
     const int side = 2 * S + 1;
     const int noff = side * side;
     for (int o = 0; o < noff; o++) {
         const int dy = o / side - S, dx = o % side - S;
-        float* hs = hs0 + (o & 1) * (HR * TW);
+        float* hs = hs0 + (o & 1) * (NLM_HR * HS);
         // ---- phase 1: horizontal window sums of the squared differences
-        for (int t = tid; t < HR * (TW / NLM_SEG); t += NT) {
-            const int v = t / (TW / NLM_SEG) - P;
-            const int u0 = (t % (TW / NLM_SEG)) * NLM_SEG;
-            float d[NLM_SEG + 2 * P];
-            if (interior) {
 #pragma unroll
-                for (int i = 0; i < NLM_SEG + 2 * P; i++) {
-                    float e = P_at(u0 - P + i, v) - P_at(u0 - P + i + dx, v + dy);
+        for (int task = 0; task < 2; task++) {
+            const float* c = task == 0 ? ca : cb;
+            const int u0 = task == 0 ? u0a : u0b;
+            float d[DW];
+            if (interior) {
+                const float* sp = pt + (cv + dy + SP) * PW + (u0 - P + dx + SP);
+#pragma unroll
+                for (int i = 0; i < DW; i++) {
+                    float e = c[i] - sp[i];
                     d[i] = e * e;
                 }
             } else {
-                const int gv = clampi(oy + v, 0, h - 1);
-                const int sv = clampi(gv + dy, 0, h - 1) - oy;
+                const int sv = clampi(oy + cv + dy, 0, h - 1) - oy;
 #pragma unroll
-                for (int i = 0; i < NLM_SEG + 2 * P; i++) {
-                    const int gu = clampi(ox + u0 - P + i, 0, w - 1);
-                    const int su = clampi(gu + dx, 0, w - 1) - ox;
-                    float e = P_at(gu - ox, gv - oy) - P_at(su, sv);
+                for (int i = 0; i < DW; i++) {
+                    const int su = clampi(ox + cu_of(u0 - P + i) + dx, 0, w - 1) - ox;
+                    float e = c[i] - P_at(su, sv);
                     d[i] = e * e;
                 }
             }
+            float hsum[SEG];
             float s = 0.0f;
 #pragma unroll
             for (int i = 0; i < 2 * P + 1; i++) s += d[i];
-            float* dst = hs + (v + P) * TW + u0;
 #pragma unroll
-            for (int j = 0; j < NLM_SEG; j++) {
-                dst[j] = s;
-                if (j + 1 < NLM_SEG) s += d[j + 2 * P + 1] - d[j];
+            for (int j = 0; j < SEG; j++) {
+                hsum[j] = s;
+                if (j + 1 < SEG) s += d[j + 2 * P + 1] - d[j];
             }
+            float4* dst = reinterpret_cast<float4*>(hs + (v1 + P) * HS + u0);
+            dst[0] = make_float4(hsum[0], hsum[1], hsum[2], hsum[3]);
+            dst[1] = make_float4(hsum[4], hsum[5], hsum[6], hsum[7]);
         }
         __syncthreads();
         // ---- phase 2: vertical window sums, weights, accumulation
         {
-            const float* hc = hs + row0 * TW + col;  // hs row (row0 - P) + P
+            const float* hc = hs + row0 * HS + col;  // hs row r holds tile row r - P
+            const float* qp = pt + (row0 + dy + SP) * PW + (col + dx + SP);
             float s = 0.0f;
 #pragma unroll
-            for (int j = 0; j < 2 * P + 1; j++) s += hc[j * TW];
+            for (int j = 0; j < 2 * P + 1; j++) s += hc[j * HS];
 #pragma unroll
-            for (int k = 0; k < NLM_ROWS; k++) {
+            for (int k = 0; k < RPT; k++) {
                 const float wt = __builtin_amdgcn_exp2f(s * wscale);
-                const float q = P_at(col + dx, row0 + k + dy);
-                acc[k] += wt * q;
+                acc[k] += wt * qp[k * PW];
                 wsum[k] += wt;
-                if (k + 1 < NLM_ROWS) s += hc[(k + 2 * P + 1) * TW] - hc[k * TW];
+                if (k + 1 < RPT && row0 + k + 1 < TH) s += hc[(k + 2 * P + 1) * HS] - hc[k * HS];
             }
         }
-        // next offset writes the other hs buffer; the barrier after its
-        // phase 1 orders it against this phase 2
+        // the other hs buffer is written next; the barrier after that phase 1
+        // orders it against this phase 2
     }
 #pragma unroll
-    for (int k = 0; k < NLM_ROWS; k++) {
+    for (int k = 0; k < RPT; k++) {
         const int gx = ox + col, gy = oy + row0 + k;
-        if (gx < w && gy < h) out[(size_t)gy * w + gx] = acc[k] / fmaxf(wsum[k], 1e-12f);
+        if (row0 + k < TH && gx < w && gy < h) out[(size_t)gy * w + gx] = acc[k] / fmaxf(wsum[k], 1e-12f);
     }
 }
 
@@ -170,13 +201,12 @@ constexpr int NLM_MAX_SP = 24;  // halo limit of the tile kernel
 
 template <int P>
 hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeight wp, hipStream_t stream) {
-    const int SP = S + P;
-    size_t lds = (size_t)((NLM_TH + 2 * SP) * (NLM_TW + 2 * SP) + 2 * (NLM_TH + 2 * P) * NLM_TW) * sizeof(float);
-    dim3 grid((unsigned)((w + NLM_TW - 1) / NLM_TW), (unsigned)((h + NLM_TH - 1) / NLM_TH));
+    size_t lds = nlm_tile_lds_bytes(S, P);
+    dim3 grid((unsigned)((w + NLM_TW - 1) / NLM_TW), (unsigned)((h + nlm_th(P) - 1) / nlm_th(P)));
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void*)k_nlm_tile<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)nlm_tile_lds_bytes(NLM_MAX_SP, P));
+                                           (int)nlm_tile_lds_bytes(NLM_MAX_SP - P, P));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
@@ -187,8 +217,7 @@ hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeig
 }  // namespace
 
 size_t nlm_tile_lds_bytes(int S, int P) {
-    const int SP = S + P;
-    return (size_t)((NLM_TH + 2 * SP) * (NLM_TW + 2 * SP) + 2 * (NLM_TH + 2 * P) * NLM_TW) * sizeof(float);
+    return (size_t)(nlm_pt_floats(S + P, P) + 2 * NLM_HR * NLM_HS) * sizeof(float);
 }
 
 hipError_t launch_nlm(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, hipStream_t stream) {

@@ -55,11 +55,12 @@ __global__ void k_rl_apply(const float* __restrict__ rgb, const float* __restric
 template <int R>
 __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est, const float* __restrict__ yp, float* __restrict__ est_out,
                                                    int w, int h, Taps taps) {
-    constexpr int TX = RL_TX, TY = RL_TY, NT = RL_NT, K = 2 * R + 1;
+    constexpr int TX = RL_TX, TY = RL_TY, NT = RL_NT, K = 2 * R + 1, B = 8;
     // est: [-2R, T+2R) both ways; y, ratio: [-R, T+R); tmp: cols [-R, TX+R), rows [-2R, TY+2R)
     constexpr int EW = TX + 4 * R, EH = TY + 4 * R;
     constexpr int YW = TX + 2 * R, YH = TY + 2 * R;
     constexpr int MW = TX + 2 * R, MH = TY + 4 * R;
+    constexpr int QW = TX, QH = TY + 2 * R;
     __shared__ float s_est[EH * EW];
     __shared__ float s_y[YH * YW];
     __shared__ float s_tmp[MH * MW];
@@ -80,12 +81,92 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
     }
     __syncthreads();
     auto E = [&](int u, int v) -> float { return s_est[(v + 2 * R) * EW + (u + 2 * R)]; };
-    // Each stage evaluates at the clamped image coordinate of its slot, so a
-    // slot outside the image holds the value of the edge pixel it clamps to.
+    const bool interior = ox >= 2 * R && oy >= 2 * R && ox + TX + 2 * R <= w && oy + TY + 2 * R <= h;
+
+    if (interior) {
+        // Register-blocked passes: each task produces B consecutive outputs
+        // from B + 2R staged samples.  Every output still accumulates its
+        // taps in order 0..2R with separate multiply and add.
+        // horizontal blur of est: rows [-2R, TY+2R), cols [-R, TX+R)
+        constexpr int H1R = (MW + B - 1) / B;
+        for (int t = tid; t < MH * H1R; t += NT) {
+            const int v = t / H1R - 2 * R, u0 = (t % H1R) * B - R;
+            const float* src = s_est + (v + 2 * R) * EW + (u0 - R + 2 * R);
+            float x[B + 2 * R];
+#pragma unroll
+            for (int i = 0; i < B + 2 * R; i++) x[i] = src[min(i, EW - 1 - (u0 - R + 2 * R))];
+            float* dst = s_tmp + (v + 2 * R) * MW + (u0 + R);
+#pragma unroll
+            for (int j = 0; j < B; j++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
+                if (u0 + j < TX + R) dst[j] = acc;
+            }
+        }
+        __syncthreads();
+        // vertical blur -> conv, ratio: rows [-R, TY+R), cols [-R, TX+R)
+        constexpr int V1R = (YH + B - 1) / B;
+        for (int t = tid; t < YW * V1R; t += NT) {
+            const int u = t % YW - R, v0 = (t / YW) * B - R;
+            const float* src = s_tmp + (v0 - R + 2 * R) * MW + (u + R);
+            float x[B + 2 * R];
+#pragma unroll
+            for (int i = 0; i < B + 2 * R; i++) x[i] = src[min(i, MH - 1 - (v0 - R + 2 * R)) * MW];
+#pragma unroll
+            for (int j = 0; j < B; j++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
+                if (v0 + j < TY + R) {
+                    const int idx = (v0 + j + R) * YW + (u + R);
+                    s_rat[idx] = s_y[idx] / fmaxr(acc, 1e-6f);
+                }
+            }
+        }
+        __syncthreads();
+        // horizontal blur of ratio: rows [-R, TY+R), cols [0, TX)
+        constexpr int H2R = QW / B;
+        for (int t = tid; t < QH * H2R; t += NT) {
+            const int v = t / H2R - R, u0 = (t % H2R) * B;
+            const float* src = s_rat + (v + R) * YW + u0;
+            float x[B + 2 * R];
+#pragma unroll
+            for (int i = 0; i < B + 2 * R; i++) x[i] = src[i];
+            float* dst = s_tmp + (v + R) * QW + u0;
+#pragma unroll
+            for (int j = 0; j < B; j++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
+                dst[j] = acc;
+            }
+        }
+        __syncthreads();
+        // vertical blur -> corr, est update: rows [0, TY), cols [0, TX)
+        constexpr int V2R = TY / B;
+        for (int t = tid; t < QW * V2R; t += NT) {
+            const int u = t % QW, v0 = (t / QW) * B;
+            const float* src = s_tmp + v0 * QW + u;
+            float x[B + 2 * R];
+#pragma unroll
+            for (int i = 0; i < B + 2 * R; i++) x[i] = src[i * QW];
+#pragma unroll
+            for (int j = 0; j < B; j++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
+                est_out[(size_t)(oy + v0 + j) * w + (ox + u)] = clampr(E(u, v0 + j) * acc, 0.0f, 64.0f);
+            }
+        }
+        return;
+    }
+
+    // Frame-edge tiles: each stage evaluates at the clamped image coordinate
+    // of its slot, so a slot outside the image holds the value of the edge
+    // pixel it clamps to.
     auto cu_of = [&](int u) { return clampi(ox + u, 0, w - 1) - ox; };
     auto cv_of = [&](int v) { return clampi(oy + v, 0, h - 1) - oy; };
-
-    // horizontal blur of est
     for (int i = tid; i < MH * MW; i += NT) {
         int u = i % MW - R, v = i / MW - 2 * R;
         int cu = cu_of(u);
@@ -95,7 +176,6 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         s_tmp[i] = acc;
     }
     __syncthreads();
-    // vertical blur -> conv, ratio = y / max(conv, 1e-6)
     for (int i = tid; i < YH * YW; i += NT) {
         int u = i % YW - R, v = i / YW - R;
         int cv = cv_of(v);
@@ -105,8 +185,6 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         s_rat[i] = s_y[i] / fmaxr(acc, 1e-6f);
     }
     __syncthreads();
-    // horizontal blur of ratio over rows [-R, TY+R), cols [0, TX) (reuses s_tmp)
-    constexpr int QW = TX, QH = TY + 2 * R;
     for (int i = tid; i < QH * QW; i += NT) {
         int u = i % QW, v = i / QW - R;
         int cu = cu_of(u);
@@ -116,7 +194,6 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         s_tmp[i] = acc;
     }
     __syncthreads();
-    // vertical blur -> corr, est update
     for (int i = tid; i < TX * TY; i += NT) {
         int u = i % TX, v = i / TX;
         int gx = ox + u, gy = oy + v;
