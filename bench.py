@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=1024)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one plain device copy of this size (known-size reference for HBM counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -108,6 +110,14 @@ def main():
         ctx._check(L.ds_denoise_ex_dev(ctx.handle, rgb.ptr, w, h, 0.0, 10, 3, den.ptr))
         ctx._check(L.ds_raw_details_dev(ctx.handle, den.ptr, w, h, 1.0, 20, out.ptr))
 
+    if args.copy_calibration_mib:
+        nb = args.copy_calibration_mib << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        ctx._check(L.ds_dev_copy(ctx.handle, b.ptr, a.ptr, nb))
+        ctx.sync()
+        a.free()
+        b.free()
+        log(f"copy calibration: {args.copy_calibration_mib} MiB read + written")
     for i in range(args.warmup):
         step()
         ctx.sync()

@@ -72,6 +72,7 @@ _SIGS = {
     "ds_memcpy_h2d": [_P, _P, _P, _SZ],
     "ds_memcpy_d2h": [_P, _P, _P, _SZ],
     "ds_sync": [_P],
+    "ds_dev_copy": [_P, _P, _P, _SZ],
     "ds_demosaic_u16_dev": [_P, _P, _SZ, _SZ, _SZ, _P, _F, _P, _P, _I, _P, _P],
     "ds_demosaic_f32_dev": [_P, _P, _SZ, _SZ, _P, _P, _I, _P, _P],
     "ds_estimate_noise_dev": [_P, _P, _SZ, _SZ, _P],

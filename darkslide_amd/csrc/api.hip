@@ -114,6 +114,13 @@ __global__ void k_synth(const SynthScene* __restrict__ sp, uint16_t* __restrict_
     out[i] = synth_dn(s, (int)(i % (size_t)s.w), (int)(i / (size_t)s.w));
 }
 
+// plain copy of n floats (also the known-size reference for HBM counters)
+__global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n4; i += stride) dst[i] = src[i];
+}
+
 // plain fill; kept as a kernel so the enqueued work stays graph-capturable
 __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -458,6 +465,18 @@ int ds_memcpy_d2h(ds_ctx* c, void* dst, const void* src, size_t bytes) {
     HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "hipMemcpy D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;
+}
+
+int ds_dev_copy(ds_ctx* c, void* dst, const void* src, size_t bytes) {
+    LOCK(c);
+    if (!dst || !src || (bytes % 16) || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
+        return fail(c, DS_ERR_INVALID, "ds_dev_copy needs 16-byte aligned pointers and size");
+    const size_t n4 = bytes / 16;
+    if (n4 == 0) return DS_OK;
+    return run(c, "copy", [&] {
+        hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, (const float4*)src, (float4*)dst, n4);
+        return hipGetLastError();
+    });
 }
 
 int ds_sync(ds_ctx* c) {

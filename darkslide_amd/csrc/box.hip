@@ -10,14 +10,13 @@
 // summation order moves its output by ~1e-4 (more than the whole error budget).
 //
 // Layout: one wave owns 64 rows (horizontal) or 64 columns (vertical) of one
-// plane and walks them in 64-element chunks.  Chunks are staged through an
-// LDS ring with coalesced loads (three chunk slots: the window spans the
-// previous, current and next chunk for r < 64); the next-but-one chunk is
-// fetched into registers while the current one is walked.  Within a chunk
-// the window differences are gathered first, so the serial part is one
-// dependent add per element.  Rows/columns x planes give the parallelism:
-// ~(h/64) * planes waves for the horizontal pass and ~(w/64) * planes waves
-// for the vertical one.
+// plane and walks them in 64-element chunks.  The window differences
+// in[x+r+1] - in[x-r] are formed with coalesced loads before the walk (each
+// is exactly the reference's `at(x + r + 1) - at(x - r)`), so the serial part
+// is one dependent add per element; the next chunk's loads are in flight
+// while the current chunk is walked.  Rows/columns x planes give the
+// parallelism: ~(h/64) * planes waves for the horizontal pass and
+// ~(w/64) * planes waves for the vertical one.
 
 #include "box.h"
 #include "common.h"
@@ -28,94 +27,82 @@ namespace ds {
 
 namespace {
 
-constexpr int BX = 64;       // chunk length and rows / columns per wave
-constexpr int RING = 3 * BX;  // three resident chunks: previous, current, next
+constexpr int BX = 64;  // chunk length and rows / columns per wave
 
-// Horizontal pass.  grid: (ceil(h/64), planes); block: one wave.  Lane t owns
-// row y0 + t.  Per 64-column chunk c (chunks c-1, c, c+1 resident in the
-// ring, x held at ring position x mod 192):
-//   1. gather d[j] = in[x+r+1] - in[x-r] for the chunk into registers;
-//   2. run the 64 dependent additions acc += d[j], keeping acc * norm;
-//   3. stage the outputs in the dead slot of chunk c-1 and store them
-//      coalesced, then refill that slot with chunk c+2, fetched into
-//      registers before step 1.
+// Horizontal pass.  grid: (ceil(h/64), planes); block: one wave.  Per
+// 64-column chunk:
+//   1. lane t forms the window differences d = in[x+r+1] - in[x-r] of column
+//      x = 64c + t for the wave's 64 rows (coalesced loads, clamped indices
+//      computed once per chunk) and stores them transposed into LDS;
+//   2. lane t walks row y0 + t through the chunk: out = acc * norm;
+//      acc += d -- the only sequential work, one add per element, reading
+//      LDS at immediate offsets;
+//   3. the outputs go back through the same LDS tile and out coalesced.
+// The next chunk's loads are issued before the walk, and two LDS tiles
+// alternate, so each chunk needs two barriers.
 template <class In>
-__global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_boxx_h(In in, BoxPlanes out, int w, int h, int r, float norm) {
-    __shared__ float ring[BX][RING + 1];  // odd row stride: lanes of one column hit distinct banks
+__global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_boxx_h(In in, BoxPlanes out, int w, int h, int r,
+                                                                                          float norm) {
+    __shared__ float tile[2][BX][BX + 1];
     const int t = threadIdx.x;
     const int y0 = blockIdx.x * BX;
     const int k = blockIdx.y;
-    const size_t myrow = (size_t)min(y0 + t, h - 1) * (size_t)w;
+    const uint32_t uw = (uint32_t)w;
     const int ntiles = (w + BX - 1) / BX;
-    float pre[BX];
-    auto fetch = [&](int c) {
-        const int xx = c * BX + t;
-        const uint32_t base = (uint32_t)y0 * (uint32_t)w + (uint32_t)xx;
+    float a[BX], b[BX];
+    auto issue = [&](int c) {
+        const int x = min(c * BX + t, w - 1);
+        const uint32_t xn = (uint32_t)min(x + r + 1, w - 1), xo = (uint32_t)max(x - r, 0);
 #pragma unroll
         for (int j = 0; j < BX; j++) {
-            const int yy = y0 + j;
-            pre[j] = (yy < h && xx < w) ? in.get(base + (uint32_t)j * (uint32_t)w, k) : 0.0f;
+            const uint32_t row = (uint32_t)min(y0 + j, h - 1) * uw;
+            a[j] = in.get(row + xn, k);
+            b[j] = in.get(row + xo, k);
         }
     };
     auto commit = [&](int c) {
-        const int base = (c % 3) * BX + t;
 #pragma unroll
-        for (int j = 0; j < BX; j++) ring[j][base] = pre[j];
+        for (int j = 0; j < BX; j++) tile[c & 1][j][t] = a[j] - b[j];
     };
-    fetch(0);
-    commit(0);
-    if (ntiles > 1) {
-        fetch(1);
-        commit(1);
-    }
-    // window sum at x = 0 in the reference's order
+    // window sum at x = 0 of row y0 + t, in the reference's order
     float acc = 0.0f;
-    for (int i = -r; i <= r; i++) acc += in.get(myrow + (size_t)clampi(i, 0, w - 1), k);
+    {
+        const uint32_t row = (uint32_t)min(y0 + t, h - 1) * uw;
+        for (int i = -r; i <= r; i++) acc += in.get(row + (uint32_t)clampi(i, 0, w - 1), k);
+    }
+    issue(0);
+    commit(0);
     __syncthreads();
-    const float* row = ring[t];
+    float* dst = out.p[k];
     for (int c = 0; c < ntiles; c++) {
-        if (c + 2 < ntiles) fetch(c + 2);
-        const int xb = c * BX;
-        float d[BX];
+        const bool more = c + 1 < ntiles;
+        if (more) issue(c + 1);
+        float* rowt = tile[c & 1][t];
 #pragma unroll
         for (int j = 0; j < BX; j++) {
-            const int x = xb + j;
-            const int xn = min(x + r + 1, w - 1), xo = max(x - r, 0);
-            d[j] = row[xn % RING] - row[xo % RING];
+            const float d = rowt[j];
+            rowt[j] = acc * norm;
+            acc += d;
         }
+        __syncthreads();
+        const int x = c * BX + t;
+        if (x < w) {
+            const uint32_t obase = (uint32_t)y0 * uw + (uint32_t)x;
 #pragma unroll
-        for (int j = 0; j < BX; j++) {
-            const float dj = d[j];
-            d[j] = acc * norm;
-            acc += dj;
+            for (int j = 0; j < BX; j++)
+                if (y0 + j < h) dst[obase + (uint32_t)j * uw] = tile[c & 1][j][t];
         }
-        // chunk c-1 is no longer read: stage the outputs in its slot
-        const int stage = ((c + 2) % 3) * BX;
-#pragma unroll
-        for (int j = 0; j < BX; j++) ring[t][stage + j] = d[j];
-        __syncthreads();
-        const int xx = xb + t;
-        const uint32_t obase = (uint32_t)y0 * (uint32_t)w + (uint32_t)xx;
-        float* dst = out.p[k];
-        if (xx < w)
-#pragma unroll 16
-            for (int j = 0; j < BX; j++) {
-                const int yy = y0 + j;
-                if (yy < h) dst[obase + (uint32_t)j * (uint32_t)w] = ring[j][stage + t];
-            }
-        __syncthreads();
-        if (c + 2 < ntiles) commit(c + 2);
+        if (more) commit(c + 1);
         __syncthreads();
     }
 }
 
 // Vertical pass.  grid: (ceil(w/64), planes); block: one wave.  Lane t walks
-// column x0 + t; it only touches its own LDS column (ring[row][t]), so no
-// barriers are needed.  Same chunk scheme as the horizontal pass, with the
-// outputs stored straight to global memory (a wave's stores cover one
-// contiguous row piece).
-__global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_boxx_v(CBoxPlanes in, BoxPlanes out, int w, int h, int r, float norm) {
-    __shared__ float ring[RING][BX];
+// column x0 + t; its window differences come from two coalesced row loads,
+// so the pass needs no LDS: differences and outputs stay in registers and
+// the stores of a wave cover one contiguous row piece.
+__global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_boxx_v(CBoxPlanes in, BoxPlanes out, int w, int h, int r,
+                                                                                          float norm) {
     const int t = threadIdx.x;
     const int x = min(blockIdx.x * BX + t, w - 1);  // lanes past the edge shadow the last column
     const bool live = blockIdx.x * BX + t < w;
@@ -124,49 +111,31 @@ __global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     float* __restrict__ dst = out.p[k];
     const int ntiles = (h + BX - 1) / BX;
     const uint32_t uw = (uint32_t)w;
-    float pre[BX];
-    // chunk 0 and 1 into the ring; 32-bit element offsets off a uniform base
-    // (planes stay below 2^31 elements): one VGPR per in-flight address
-    for (int c = 0; c < 2 && c < ntiles; c++) {
-        const uint32_t base = (uint32_t)(c * BX) * uw + (uint32_t)x;
+    float a[BX], b[BX];
+    auto issue = [&](int c) {
 #pragma unroll
-        for (int j = 0; j < BX; j++) pre[j] = c * BX + j < h ? src[base + (uint32_t)j * uw] : 0.0f;
-#pragma unroll
-        for (int j = 0; j < BX; j++) ring[c * BX + j][t] = pre[j];
-    }
+        for (int j = 0; j < BX; j++) {
+            const int y = c * BX + j;
+            a[j] = src[(uint32_t)min(y + r + 1, h - 1) * uw + (uint32_t)x];
+            b[j] = src[(uint32_t)max(y - r, 0) * uw + (uint32_t)x];
+        }
+    };
+    // window sum at y = 0 in the reference's order
     float acc = 0.0f;
     for (int i = -r; i <= r; i++) acc += src[(uint32_t)clampi(i, 0, h - 1) * uw + (uint32_t)x];
+    issue(0);
     for (int c = 0; c < ntiles; c++) {
-        const bool more = c + 2 < ntiles;
-        if (more) {
-            const uint32_t base = (uint32_t)((c + 2) * BX) * uw + (uint32_t)x;
-#pragma unroll
-            for (int j = 0; j < BX; j++) pre[j] = (c + 2) * BX + j < h ? src[base + (uint32_t)j * uw] : 0.0f;
-        }
-        const int yb = c * BX;
         float d[BX];
 #pragma unroll
-        for (int j = 0; j < BX; j++) {
-            const int y = yb + j;
-            const int yn = min(y + r + 1, h - 1), yo = max(y - r, 0);
-            d[j] = ring[yn % RING][t] - ring[yo % RING][t];
-        }
+        for (int j = 0; j < BX; j++) d[j] = a[j] - b[j];
+        if (c + 1 < ntiles) issue(c + 1);
+        const int yb = c * BX;
+        const uint32_t obase = (uint32_t)yb * uw + (uint32_t)x;
 #pragma unroll
         for (int j = 0; j < BX; j++) {
-            const float dj = d[j];
-            d[j] = acc * norm;
-            acc += dj;
-        }
-        if (live) {
-            const uint32_t obase = (uint32_t)yb * uw + (uint32_t)x;
-#pragma unroll
-            for (int j = 0; j < BX; j++)
-                if (yb + j < h) dst[obase + (uint32_t)j * uw] = d[j];
-        }
-        if (more) {
-            const int slot = ((c + 2) % 3) * BX;
-#pragma unroll
-            for (int j = 0; j < BX; j++) ring[slot + j][t] = pre[j];
+            const float o = acc * norm;
+            acc += d[j];
+            if (live && yb + j < h) dst[obase + (uint32_t)j * uw] = o;
         }
     }
 }

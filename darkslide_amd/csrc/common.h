@@ -43,4 +43,42 @@ __host__ __device__ __forceinline__ void mul_mv(const float m[3][3], const float
     o[2] = m[2][0] * v[0] + m[2][1] * v[1] + m[2][2] * v[2];
 }
 
+// Stages N elements into LDS with NT threads: every thread issues all of its
+// global loads before its first LDS store, so a block pays one memory latency
+// for the whole tile instead of one per loop trip.  src(i) returns element i
+// (the caller clamps coordinates), dst(i, v) stores it.
+template <int N, int NT, class Src, class Dst>
+__device__ __forceinline__ void stage_tile(Src src, Dst dst) {
+    constexpr int K = (N + NT - 1) / NT;
+    float r[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = (int)threadIdx.x + k * NT;
+        r[k] = (i < N) ? src(i) : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = (int)threadIdx.x + k * NT;
+        if (i < N) dst(i, r[k]);
+    }
+}
+
+// Runtime-sized variant: batches of KB loads per thread.
+template <int KB, int NT, class Src, class Dst>
+__device__ __forceinline__ void stage_tile_n(int n, Src src, Dst dst) {
+    for (int base = 0; base < n; base += KB * NT) {
+        float r[KB];
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const int i = base + (int)threadIdx.x + k * NT;
+            r[k] = (i < n) ? src(i) : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const int i = base + (int)threadIdx.x + k * NT;
+            if (i < n) dst(i, r[k]);
+        }
+    }
+}
+
 }  // namespace ds

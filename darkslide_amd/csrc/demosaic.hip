@@ -143,15 +143,20 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
     auto cval = [&](int x, int y) { return fmaxr(cm(x, y), 0.0f); };
 
     // ---- stage 0: load cfa*mul (and clip flags) with clamped coordinates
-    for (int i = tid; i < Tile<TX, TY, H_CM>::N; i += NT) {
-        int lx = i % Tile<TX, TY, H_CM>::W, ly = i / Tile<TX, TY, H_CM>::W;
-        int tx = lx - H_CM, ty = ly - H_CM;
-        int gx = clampi(ox + tx, 0, w - 1), gy = clampi(oy + ty, 0, h - 1);
-        int c = pat.at(gx, gy);
-        float v = src.cfa(gx, gy, c);
-        cm(tx, ty) = v * src.mul[c];
-        if (clip && tx >= -H_FLAG && tx < TX + H_FLAG && ty >= -H_FLAG && ty < TY + H_FLAG)
-            L.flag[(ty + H_FLAG) * Tile<TX, TY, H_FLAG>::W + (tx + H_FLAG)] = v >= kClip ? 1 : 0;
+    {
+        constexpr int CW = Tile<TX, TY, H_CM>::W;
+        stage_tile<Tile<TX, TY, H_CM>::N, NT>(
+            [&](int i) {
+                int gx = clampi(ox + i % CW - H_CM, 0, w - 1), gy = clampi(oy + i / CW - H_CM, 0, h - 1);
+                return src.cfa(gx, gy, pat.at(gx, gy));
+            },
+            [&](int i, float v) {
+                int tx = i % CW - H_CM, ty = i / CW - H_CM;
+                int gx = clampi(ox + tx, 0, w - 1), gy = clampi(oy + ty, 0, h - 1);
+                cm(tx, ty) = v * src.mul[pat.at(gx, gy)];
+                if (clip && tx >= -H_FLAG && tx < TX + H_FLAG && ty >= -H_FLAG && ty < TY + H_FLAG)
+                    L.flag[(ty + H_FLAG) * Tile<TX, TY, H_FLAG>::W + (tx + H_FLAG)] = v >= kClip ? 1 : 0;
+            });
     }
     __syncthreads();
 
@@ -388,13 +393,19 @@ __global__ __launch_bounds__(NT) void k_mhc(CfaSrc src, float* __restrict__ rgb,
     const int ox = blockIdx.x * TX, oy = blockIdx.y * TY;
     const int w = src.w, h = src.h;
     const Bayer pat = src.pat;
-    for (int i = tid; i < Tile<TX, TY, H>::N; i += NT) {
-        int tx = i % Tile<TX, TY, H>::W - H, ty = i / Tile<TX, TY, H>::W - H;
-        int gx = clampi(ox + tx, 0, w - 1), gy = clampi(oy + ty, 0, h - 1);
-        int c = pat.at(gx, gy);
-        float v = src.cfa(gx, gy, c);
-        cm(tx, ty) = v * src.mul[c];
-        if (clip && tx >= -1 && tx <= TX && ty >= -1 && ty <= TY) s_flag[(ty + 1) * Tile<TX, TY, 1>::W + (tx + 1)] = v >= kClip ? 1 : 0;
+    {
+        constexpr int CW = Tile<TX, TY, H>::W;
+        stage_tile<Tile<TX, TY, H>::N, NT>(
+            [&](int i) {
+                int gx = clampi(ox + i % CW - H, 0, w - 1), gy = clampi(oy + i / CW - H, 0, h - 1);
+                return src.cfa(gx, gy, pat.at(gx, gy));
+            },
+            [&](int i, float v) {
+                int tx = i % CW - H, ty = i / CW - H;
+                int gx = clampi(ox + tx, 0, w - 1), gy = clampi(oy + ty, 0, h - 1);
+                cm(tx, ty) = v * src.mul[pat.at(gx, gy)];
+                if (clip && tx >= -1 && tx <= TX && ty >= -1 && ty <= TY) s_flag[(ty + 1) * Tile<TX, TY, 1>::W + (tx + 1)] = v >= kClip ? 1 : 0;
+            });
     }
     __syncthreads();
     for (int i = tid; i < TX * TY; i += NT) {

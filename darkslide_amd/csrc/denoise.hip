@@ -105,6 +105,21 @@ __global__ __launch_bounds__(SEL_NT) void k_noise_select(const uint32_t* __restr
     if (tid == 0) *sigma_out = __uint_as_float(s_prefix) / 0.6745f / sqrt20;
 }
 
+// The guide I = L + (lf - L) m (enhance.rs:86) and the products the guided
+// filters box (image.rs:388-389): I, I a, I b, I I.
+__global__ void k_gf_inputs(const float* __restrict__ L, const float* __restrict__ LF, const float* __restrict__ A,
+                            const float* __restrict__ B, float m, size_t n, float* __restrict__ I_, float* __restrict__ IA,
+                            float* __restrict__ IB, float* __restrict__ II) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float o = L[i];
+    const float I = o + (LF[i] - o) * m;
+    I_[i] = I;
+    IA[i] = I * A[i];
+    IB[i] = I * B[i];
+    II[i] = I * I;
+}
+
 // image.rs:395-400 for the a and b planes, eps = (2 max(sigma,1e-4))^2
 // (enhance.rs:82, :88)
 __global__ void k_gf_coef(const float* __restrict__ mi_, const float* __restrict__ mpa_, const float* __restrict__ mpb_,
@@ -196,15 +211,44 @@ hipError_t launch_guided_pair(const float* L, const float* LF, const float* A, c
         return o;
     };
     const unsigned nb = (unsigned)((n + 255) / 256);
-    // means of I, a, b, I a, I b, I I (image.rs:386-391) -> t6..t11
-    hipError_t e = box_h_guided(BoxInGuided{L, LF, A, B, m}, planes(0, 6), w, h, r, st);
-    if (e == hipSuccess) e = box_v_planes(cplanes(0, 6), planes(6, 6), 6, w, h, r, st);
+    // the guide I = l2 and its products (image.rs:386-391) -> t0..t3
+    hipLaunchKernelGGL(k_gf_inputs, dim3(nb), dim3(256), 0, st, L, LF, A, B, m, n, t[0], t[1], t[2], t[3]);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // coefficients (image.rs:395-400) -> t0..t3
-    hipLaunchKernelGGL(k_gf_coef, dim3(nb), dim3(256), 0, st, t[6], t[7], t[8], t[9], t[10], t[11], n, sigma_dev, t[0], t[1], t[2], t[3]);
+    // means of I, a, b, I a, I b, I I -> t6..t11 (horizontal pass into t4, t5, t8..t11 first)
+    CBoxPlanes hin{};
+    hin.p[0] = t[0];
+    hin.p[1] = A;
+    hin.p[2] = B;
+    hin.p[3] = t[1];
+    hin.p[4] = t[2];
+    hin.p[5] = t[3];
+    BoxPlanes hout{};
+    hout.p[0] = t[4];
+    hout.p[1] = t[5];
+    hout.p[2] = t[8];
+    hout.p[3] = t[9];
+    hout.p[4] = t[10];
+    hout.p[5] = t[11];
+    e = box_h_planes(hin, hout, 6, w, h, r, st);
+    if (e != hipSuccess) return e;
+    CBoxPlanes vin{};
+    for (int k = 0; k < 6; k++) vin.p[k] = hout.p[k];
+    BoxPlanes vout{};
+    vout.p[0] = t[6];
+    vout.p[1] = t[7];
+    vout.p[2] = t[0];
+    vout.p[3] = t[1];
+    vout.p[4] = t[2];
+    vout.p[5] = t[3];
+    e = box_v_planes(vin, vout, 6, w, h, r, st);
+    if (e != hipSuccess) return e;
+    // means now: I t6, a t7, b t0, I a t1, I b t2, I I t3
+    // coefficients (image.rs:395-400) -> t8..t11
+    hipLaunchKernelGGL(k_gf_coef, dim3(nb), dim3(256), 0, st, t[6], t[7], t[0], t[1], t[2], t[3], n, sigma_dev, t[8], t[9], t[10], t[11]);
     e = hipGetLastError();
-    // their means (image.rs:402-403) -> t8..t11
-    if (e == hipSuccess) e = box_h_planes(cplanes(0, 4), planes(4, 4), 4, w, h, r, st);
+    // their means (image.rs:402-403) -> t8..t11 (via t4..t7)
+    if (e == hipSuccess) e = box_h_planes(cplanes(8, 4), planes(4, 4), 4, w, h, r, st);
     if (e == hipSuccess) e = box_v_planes(cplanes(4, 4), planes(8, 4), 4, w, h, r, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_gf_final, dim3(nb), dim3(256), 0, st, L, LF, m, t[8], t[9], t[10], t[11], n, ok, rgb);

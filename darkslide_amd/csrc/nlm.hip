@@ -37,11 +37,15 @@ constexpr int NLM_HR = 64;        // rows of horizontal sums per tile (TH + 2P)
 constexpr int NLM_NT = 256;
 constexpr int NLM_SEG = 8;        // columns per phase-1 task
 constexpr int NLM_HS = NLM_TW + 4;  // padded row stride of the horizontal-sum buffer
+constexpr int NLM_MAX_SP = 24;    // halo limit of the tile kernel
+// Input tile row stride: fixed (so every LDS offset inside the offset loop
+// is an immediate) and odd (lanes walking down one column hit distinct banks).
+constexpr int NLM_PW = NLM_TW + 2 * NLM_MAX_SP + 1;
+constexpr int NLM_HSR = NLM_HR + 4;  // sum rows incl. padding read by the last row segment
 
 __host__ __device__ constexpr int nlm_th(int P) { return NLM_HR - 2 * P; }
-// odd row stride of the input tile: lanes walking down one column hit distinct banks
-__host__ __device__ inline int nlm_pw(int SP) { return (NLM_TW + 2 * SP) | 1; }
-__host__ __device__ inline int nlm_pt_floats(int SP, int P) { return ((nlm_th(P) + 2 * SP) * nlm_pw(SP) + 3) & ~3; }
+// input tile plus two spare rows read (not stored) by the last row segment
+__host__ __device__ inline int nlm_pt_floats(int SP, int P) { return ((nlm_th(P) + 2 * SP + 2) * NLM_PW + 3) & ~3; }
 
 // Tile kernel.  Phase 1: thread t owns hs row (t % 64) and the 8-column
 // segments (t / 64) and (t / 64 + 4): 512 tasks, exactly two per thread.  The
@@ -58,19 +62,21 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
     const float wscale = wp.scale();
     const int SP = S + P;
     const int PWL = TW + 2 * SP;              // loaded tile row length
-    const int PW = nlm_pw(SP);
+    constexpr int PW = NLM_PW;
     const int PH = TH + 2 * SP;
     float* pt = lds;                          // input tile, PH x PW
-    float* hs0 = lds + nlm_pt_floats(SP, P);  // horizontal sums, 2 x 64 x HS
+    float* hs0 = lds + nlm_pt_floats(SP, P);  // horizontal sums, 2 x HSR x HS
     const int tid = threadIdx.x;
     const int ox = blockIdx.x * TW, oy = blockIdx.y * TH;
 
     // input tile with clamped coordinates: slot (u, v) holds p[clamp(ox+u), clamp(oy+v)]
-    for (int i = tid; i < PH * PWL; i += NT) {
-        int u = i % PWL - SP, v = i / PWL - SP;
-        int gx = clampi(ox + u, 0, w - 1), gy = clampi(oy + v, 0, h - 1);
-        pt[(v + SP) * PW + (u + SP)] = in[(size_t)gy * w + gx];
-    }
+    stage_tile_n<16, NT>(
+        PH * PWL,
+        [&](int i) {
+            int gx = clampi(ox + i % PWL - SP, 0, w - 1), gy = clampi(oy + i / PWL - SP, 0, h - 1);
+            return in[(size_t)gy * w + gx];
+        },
+        [&](int i, float v) { pt[(i / PWL) * PW + (i % PWL)] = v; });
     auto P_at = [&](int u, int v) -> float { return pt[(v + SP) * PW + (u + SP)]; };
     const bool interior = ox - SP >= 0 && oy - SP >= 0 && ox + TW + SP <= w && oy + TH + SP <= h;
     __syncthreads();
@@ -101,9 +107,9 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
 
     const int side = 2 * S + 1;
     const int noff = side * side;
+    int dx = -S, dy = -S;  // offsets in the reference's order: dy outer, dx inner
     for (int o = 0; o < noff; o++) {
-        const int dy = o / side - S, dx = o % side - S;
-        float* hs = hs0 + (o & 1) * (NLM_HR * HS);
+        float* hs = hs0 + (o & 1) * (NLM_HSR * HS);
         // ---- phase 1: horizontal window sums of the squared differences
 #pragma unroll
         for (int task = 0; task < 2; task++) {
@@ -147,16 +153,23 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
             float s = 0.0f;
 #pragma unroll
             for (int j = 0; j < 2 * P + 1; j++) s += hc[j * HS];
+            // every row segment runs RPT rows (the last one past the tile
+            // reads padding rows of hs and is not stored), so the loop has
+            // no per-lane branches
 #pragma unroll
             for (int k = 0; k < RPT; k++) {
                 const float wt = __builtin_amdgcn_exp2f(s * wscale);
                 acc[k] += wt * qp[k * PW];
                 wsum[k] += wt;
-                if (k + 1 < RPT && row0 + k + 1 < TH) s += hc[(k + 2 * P + 1) * HS] - hc[k * HS];
+                if (k + 1 < RPT) s += hc[(k + 2 * P + 1) * HS] - hc[k * HS];
             }
         }
         // the other hs buffer is written next; the barrier after that phase 1
         // orders it against this phase 2
+        if (++dx > S) {
+            dx = -S;
+            ++dy;
+        }
     }
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
@@ -197,8 +210,6 @@ __global__ void k_nlm_direct(const float* __restrict__ in, float* __restrict__ o
     out[(size_t)y * w + x] = acc / fmaxf(wsum, 1e-12f);
 }
 
-constexpr int NLM_MAX_SP = 24;  // halo limit of the tile kernel
-
 template <int P>
 hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeight wp, hipStream_t stream) {
     size_t lds = nlm_tile_lds_bytes(S, P);
@@ -217,7 +228,7 @@ hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeig
 }  // namespace
 
 size_t nlm_tile_lds_bytes(int S, int P) {
-    return (size_t)(nlm_pt_floats(S + P, P) + 2 * NLM_HR * NLM_HS) * sizeof(float);
+    return (size_t)(nlm_pt_floats(S + P, P) + 2 * NLM_HSR * NLM_HS) * sizeof(float);
 }
 
 hipError_t launch_nlm(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, hipStream_t stream) {

@@ -71,14 +71,18 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
 #pragma unroll
     for (int i = 0; i < K; i++) kk[i] = taps.k[i];
 
-    for (int i = tid; i < EH * EW; i += NT) {
-        int u = i % EW - 2 * R, v = i / EW - 2 * R;
-        s_est[i] = est[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
-    }
-    for (int i = tid; i < YH * YW; i += NT) {
-        int u = i % YW - R, v = i / YW - R;
-        s_y[i] = yp[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
-    }
+    stage_tile<EH * EW, NT>(
+        [&](int i) {
+            int u = i % EW - 2 * R, v = i / EW - 2 * R;
+            return est[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
+        },
+        [&](int i, float v) { s_est[i] = v; });
+    stage_tile<YH * YW, NT>(
+        [&](int i) {
+            int u = i % YW - R, v = i / YW - R;
+            return yp[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
+        },
+        [&](int i, float v) { s_y[i] = v; });
     __syncthreads();
     auto E = [&](int u, int v) -> float { return s_est[(v + 2 * R) * EW + (u + 2 * R)]; };
     const bool interior = ox >= 2 * R && oy >= 2 * R && ox + TX + 2 * R <= w && oy + TY + 2 * R <= h;

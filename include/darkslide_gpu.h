@@ -114,6 +114,8 @@ int ds_dev_free(ds_ctx* ctx, void* dptr);
 int ds_memcpy_h2d(ds_ctx* ctx, void* dst, const void* src, size_t bytes); /* synchronous */
 int ds_memcpy_d2h(ds_ctx* ctx, void* dst, const void* src, size_t bytes); /* synchronous */
 int ds_sync(ds_ctx* ctx);
+/* Device-to-device copy with a plain streaming kernel (16-byte aligned). */
+int ds_dev_copy(ds_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 int ds_demosaic_u16_dev(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
                         const float white[3], const float mul[3], int algo, float* rgb_out, float* clip_out);
