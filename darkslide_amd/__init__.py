@@ -13,12 +13,13 @@ There is no CPU fallback: without the built library or a GPU the calls raise.
 import ctypes
 import os
 import threading
+import weakref
 
 import numpy as np
 
 __all__ = [
     "DarkslideError", "Context", "lib", "default_context", "RGGB", "ALGO_RCD", "ALGO_MHC",
-    "demosaic", "demosaic_u16", "estimate_noise", "nlm", "denoise", "raw_details", "synth_bayer",
+    "demosaic", "demosaic_u16", "estimate_noise", "nlm", "denoise", "raw_details", "synth_bayer", "CNN", "LearnedStats",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -40,6 +41,14 @@ class DarkslideError(RuntimeError):
 
 class ProfEntry(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("total_ms", ctypes.c_double), ("launches", ctypes.c_long)]
+
+
+class LearnedStats(ctypes.Structure):
+    _fields_ = [("regions", ctypes.c_size_t), ("fallback", ctypes.c_size_t), ("mean_gap_db", ctypes.c_float),
+                ("max_gap_db", ctypes.c_float)]
+
+    def as_dict(self):
+        return {"regions": self.regions, "fallback": self.fallback, "mean_gap_db": self.mean_gap_db, "max_gap_db": self.max_gap_db}
 
 
 _P = ctypes.c_void_p
@@ -84,8 +93,24 @@ _SIGS = {
     "ds_prof_get": [_P, _P, _I],
     "ds_synth_bayer_host": [ctypes.c_uint64, _SZ, _SZ, _I, _F, _P, _P],
     "ds_synth_bayer_dev": [_P, ctypes.c_uint64, _SZ, _SZ, _I, _F, _P, _P],
+    "ds_cnn_param_count": [],
+    "ds_cnn_create": [_P, ctypes.c_uint64, ctypes.POINTER(_P)],
+    "ds_cnn_destroy": [_P],
+    "ds_cnn_get_params": [_P, _P, _SZ],
+    "ds_cnn_set_params": [_P, _P, _SZ],
+    "ds_cnn_step": [_P],
+    "ds_cnn_save": [_P, ctypes.c_char_p],
+    "ds_cnn_load": [_P, ctypes.c_char_p],
+    "ds_cnn_forward": [_P, _P, _SZ, _SZ, _SZ, _P],
+    "ds_cnn_loss_grad": [_P, _P, _P, _SZ, _SZ, _SZ, _SZ, ctypes.POINTER(ctypes.c_double), _P],
+    "ds_cnn_train_step_dev": [_P, _P, _P, _SZ, _SZ, _SZ, _SZ, _F, ctypes.POINTER(ctypes.c_double)],
+    "ds_cnn_make_pairs_dev": [_P, ctypes.c_uint64, _SZ, _SZ, _F, _L, _SZ, _SZ, _SZ, _P, _P],
+    "ds_cnn_denoise_plane": [_P, _P, _SZ, _SZ, _P],
+    "ds_denoise_learned": [_P, _P, _P, _SZ, _SZ, _F, _L, _SZ, _F, _P, _P],
+    "ds_denoise_learned_dev": [_P, _P, _P, _SZ, _SZ, _F, _L, _SZ, _F, _P, _P],
 }
-_RESTYPES = {"ds_ctx_destroy": None, "ds_last_error": ctypes.c_char_p, "ds_version": ctypes.c_char_p}
+_RESTYPES = {"ds_ctx_destroy": None, "ds_last_error": ctypes.c_char_p, "ds_version": ctypes.c_char_p,
+             "ds_cnn_param_count": ctypes.c_size_t, "ds_cnn_destroy": None, "ds_cnn_step": ctypes.c_long}
 
 
 def lib():
@@ -126,6 +151,12 @@ class DeviceBuffer:
         ctx._check(lib().ds_dev_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
         self.ptr = p
 
+    def at(self, offset):
+        """Device address `offset` bytes into the buffer."""
+        if not 0 <= offset <= self.nbytes:
+            raise ValueError("offset outside the buffer")
+        return ctypes.c_void_p(self.ptr.value + int(offset))
+
     def upload(self, arr):
         arr = np.ascontiguousarray(arr)
         if arr.nbytes > self.nbytes:
@@ -155,9 +186,12 @@ class Context:
         if rc != 0:
             raise DarkslideError(rc, L.ds_last_error(None).decode())
         self.handle = h
+        self._nets = weakref.WeakSet()  # networks are destroyed before their context
 
     def close(self):
         if self.handle:
+            for net in list(self._nets):
+                net.close()
             lib().ds_ctx_destroy(self.handle)
             self.handle = None
 
@@ -278,6 +312,105 @@ class Context:
         h, w, _ = rgb.shape
         out = np.empty_like(rgb)
         self._check(lib().ds_raw_details(self.handle, _ptr(rgb), w, h, float(sigma), int(iterations), _ptr(out)))
+        return out
+
+    def denoise_learned(self, net, rgb, amount, search=10, patch=3, threshold_db=0.3):
+        """denoise with the learned luminance and the per-region NLM fallback -> (rgb, stats)."""
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        h, w, _ = rgb.shape
+        out = np.empty_like(rgb)
+        st = LearnedStats()
+        self._check(lib().ds_denoise_learned(self.handle, net.handle, _ptr(rgb), w, h, float(amount), int(search), int(patch),
+                                             float(threshold_db), _ptr(out), ctypes.byref(st)))
+        return out, st.as_dict()
+
+    def make_pairs(self, seed, frame_w, frame_h, iso, crop, n_crops, search=10, patch=3):
+        """Training pairs (x, t), each (n_crops, crop, crop), generated on the GPU."""
+        nb = n_crops * crop * crop * 4
+        bx, bt = self.alloc(nb), self.alloc(nb)
+        try:
+            self._check(lib().ds_cnn_make_pairs_dev(self.handle, int(seed), frame_w, frame_h, float(iso), int(search), int(patch), crop,
+                                                    n_crops, bx.ptr, bt.ptr))
+            return bx.download((n_crops, crop, crop), np.float32), bt.download((n_crops, crop, crop), np.float32)
+        finally:
+            bx.free()
+            bt.free()
+
+
+class CNN:
+    """The learned luminance denoiser (include/darkslide_gpu.h, ds_cnn_*).
+
+    Parameters, forward, loss gradient and Adam steps all run on the GPU;
+    the host sees FP32 parameter vectors and float32 batches."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx._check(lib().ds_cnn_create(ctx.handle, int(seed), ctypes.byref(h)))
+        self.handle = h
+        ctx._nets.add(self)
+
+    def close(self):
+        if self.handle:
+            lib().ds_cnn_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def param_count():
+        return int(lib().ds_cnn_param_count())
+
+    @property
+    def step(self):
+        return int(lib().ds_cnn_step(self.handle))
+
+    def params(self):
+        p = np.empty(self.param_count(), np.float32)
+        self.ctx._check(lib().ds_cnn_get_params(self.handle, _ptr(p), p.size))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self.ctx._check(lib().ds_cnn_set_params(self.handle, _ptr(p), p.size))
+
+    def save(self, path):
+        self.ctx._check(lib().ds_cnn_save(self.handle, os.fsencode(path)))
+
+    def load(self, path):
+        self.ctx._check(lib().ds_cnn_load(self.handle, os.fsencode(path)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        n, h, w = x.shape
+        y = np.empty_like(x)
+        self.ctx._check(lib().ds_cnn_forward(self.handle, _ptr(x), n, h, w, _ptr(y)))
+        return y
+
+    def loss_grad(self, x, t, margin):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        n, h, w = x.shape
+        g = np.empty(self.param_count(), np.float32)
+        loss = ctypes.c_double()
+        self.ctx._check(lib().ds_cnn_loss_grad(self.handle, _ptr(x), _ptr(t), n, h, w, int(margin), ctypes.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def train_step_dev(self, x_dev, t_dev, n, h, w, margin, lr, want_loss=False):
+        loss = ctypes.c_double()
+        self.ctx._check(lib().ds_cnn_train_step_dev(self.handle, x_dev, t_dev, n, h, w, int(margin), float(lr),
+                                                    ctypes.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def denoise_plane(self, L):
+        L = np.ascontiguousarray(L, np.float32)
+        h, w = L.shape
+        out = np.empty_like(L)
+        self.ctx._check(lib().ds_cnn_denoise_plane(self.handle, _ptr(L), w, h, _ptr(out)))
         return out
 
 

@@ -20,6 +20,7 @@
 #include "rl.h"
 #include "stages.h"
 #include "synth.h"
+#include "cnn.h"
 
 #pragma clang fp contract(off)
 
@@ -142,7 +143,7 @@ struct Buf {
     size_t bytes = 0;
 };
 
-enum Slot { S_IN = 0, S_IN2, S_OUT, S_CLIP, S_WS, S_L, S_A, S_B, S_LF, S_SIGMA, S_NOISE, S_SCENE, S_COUNT };
+enum Slot { S_IN = 0, S_IN2, S_OUT, S_CLIP, S_WS, S_L, S_A, S_B, S_LF, S_SIGMA, S_NOISE, S_SCENE, S_X, S_T, S_REGIONS, S_COUNT };
 
 struct Pending {
     std::string name;
@@ -351,6 +352,280 @@ int raw_details_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float sigma
     SLOT(c, ws, float*, S_WS, rl_workspace_floats(w, h) * 4);
     return run(c, "raw_details", [&] { return launch_raw_details(rgb, w, h, sigma, iters, ws, out, c->stream); });
 }
+
+int synth_dev(ds_ctx* c, uint64_t seed, size_t w, size_t h, int bits, float iso, const uint8_t* pattern, uint16_t* raw_out) {
+    if (!raw_out || !valid_bayer(pattern) || (bits != 12 && bits != 14) || !(iso >= 1.0f || iso == 0.0f) || w > (1u << 20) || h > (1u << 20))
+        return fail(c, DS_ERR_INVALID, "invalid synthetic frame arguments");
+    SynthScene s;
+    synth_scene_init(&s, seed, (int)w, (int)h, bits, iso, pattern);
+    size_t n = w * h;
+    if (n == 0) return DS_OK;
+    SLOT(c, d_scene, SynthScene*, S_SCENE, sizeof(SynthScene));
+    HIPCHK(c, hipMemcpyAsync(d_scene, &s, sizeof(SynthScene), hipMemcpyHostToDevice, c->stream), "H2D");
+    int rc = run(c, "synth_bayer", [&] {
+        hipLaunchKernelGGL(k_synth, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d_scene, raw_out);
+        return hipGetLastError();
+    });
+    // the scene lives on this stack frame until the copy has been consumed
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return rc;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// learned denoiser
+
+struct ds_cnn {
+    ds_ctx* ctx = nullptr;
+    float* state = nullptr;    // params | grad | adam m | adam v, CNN_NPARAM floats each
+    uint16_t* wpk = nullptr;   // bf16 middle weights: forward layout | data-gradient layout
+    long step = 0;
+    uint64_t seed = 0;
+    bool packed = false;
+    size_t tr_px = 0;          // training buffers, sized for tr_px pixels
+    uint16_t* tr_act = nullptr;  // 7 activations + 2 activation gradients, 32 bf16 per pixel each
+    float* tr_f = nullptr;       // x | tgt | out | backward scratch
+    size_t tr_fcap = 0;          // floats in tr_f
+    size_t inf_px = 0;         // inference buffers
+    uint16_t* inf_act = nullptr;  // 2 ping-pong activations
+    float* params() { return state; }
+    float* grad() { return state + CNN_NPARAM; }
+    float* adam_m() { return state + 2 * CNN_NPARAM; }
+    float* adam_v() { return state + 3 * CNN_NPARAM; }
+};
+
+namespace {
+
+constexpr size_t CNN_ACT_BYTES = 32 * sizeof(uint16_t);  // per pixel
+
+int cnn_pack_now(ds_cnn* net) {
+    ds_ctx* c = net->ctx;
+    if (net->packed) return DS_OK;
+    int rc = run(c, "cnn_pack", [&] { return cnn_pack(net->params(), net->wpk, net->wpk + CNN_WBF16, c->stream); });
+    if (rc == DS_OK) net->packed = true;
+    return rc;
+}
+
+size_t tr_floats(size_t npx, int n, int H, int W) { return 3 * npx + cnn_scratch_floats(n, H, W); }
+
+int cnn_ensure_train(ds_cnn* net, int n, int H, int W) {
+    ds_ctx* c = net->ctx;
+    const size_t npx = (size_t)n * H * W;
+    // the backward scratch depends on the batch shape, not only on its pixel count
+    const size_t fl = tr_floats(npx, n, H, W);
+    if (net->tr_px >= npx && net->tr_fcap >= fl) return DS_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (net->tr_act) hipFree(net->tr_act);
+    if (net->tr_f) hipFree(net->tr_f);
+    net->tr_act = nullptr;
+    net->tr_f = nullptr;
+    const size_t px = std::max(npx, net->tr_px), cap = std::max(fl, net->tr_fcap);
+    net->tr_px = 0;
+    net->tr_fcap = 0;
+    HIPCHK(c, hipMalloc(&net->tr_act, 9 * px * CNN_ACT_BYTES), "hipMalloc (CNN activations)");
+    HIPCHK(c, hipMalloc(&net->tr_f, cap * sizeof(float)), "hipMalloc (CNN training buffers)");
+    net->tr_px = px;
+    net->tr_fcap = cap;
+    return DS_OK;
+}
+
+int cnn_ensure_infer(ds_cnn* net, size_t npx) {
+    ds_ctx* c = net->ctx;
+    if (net->inf_px >= npx && net->inf_act) return DS_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (net->inf_act) hipFree(net->inf_act);
+    net->inf_act = nullptr;
+    net->inf_px = 0;
+    HIPCHK(c, hipMalloc(&net->inf_act, 2 * npx * CNN_ACT_BYTES), "hipMalloc (CNN activations)");
+    net->inf_px = npx;
+    return DS_OK;
+}
+
+int cnn_check_shape(ds_ctx* c, size_t n, size_t h, size_t w) {
+    if (n == 0 || h == 0 || w == 0) return fail(c, DS_ERR_INVALID, "empty CNN batch");
+    if (n > 65535 || h > (1u << 16) || w > (1u << 16) || n * h * w > ((size_t)1 << 31))
+        return fail(c, DS_ERR_INVALID, "CNN batch %zux%zux%zu too large", n, h, w);
+    return DS_OK;
+}
+
+// x (n*h*w, device) -> out; training layout keeps all 7 activations.
+int cnn_forward_train(ds_cnn* net, const float* x, int n, int H, int W, float* out) {
+    ds_ctx* c = net->ctx;
+    const size_t npx = (size_t)n * H * W;
+    if (int rc = cnn_pack_now(net)) return rc;
+    uint16_t* acts[7];
+    for (int k = 0; k < 7; k++) acts[k] = net->tr_act + (size_t)k * npx * 32;
+    return run(c, "cnn_forward", [&] { return cnn_forward(net->params(), net->wpk, x, n, H, W, acts, out, c->stream); });
+}
+
+// Loss partials of the last backward (host sum in double).
+int cnn_loss_host(ds_cnn* net, size_t npx, int n, int H, int W, int margin, double* loss) {
+    ds_ctx* c = net->ctx;
+    const size_t nparts = (npx + 255) / 256;
+    std::vector<float> parts(nparts);
+    float* lossp = net->tr_f + 3 * npx + npx;  // scratch: dout | loss partials | ...
+    HIPCHK(c, hipMemcpyAsync(parts.data(), lossp, nparts * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    double s = 0.0;
+    for (float v : parts) s += v;
+    const long nvalid = (long)n * (H - 2 * margin) * (W - 2 * margin);
+    *loss = s / (double)(nvalid > 0 ? nvalid : 1);
+    return DS_OK;
+}
+
+// Forward + backward of a device batch already in tr_f (x at 0, tgt at npx):
+// gradient into net->grad().
+int cnn_grad_dev(ds_cnn* net, int n, int H, int W, int margin) {
+    ds_ctx* c = net->ctx;
+    const size_t npx = (size_t)n * H * W;
+    float* x = net->tr_f;
+    float* tgt = x + npx;
+    float* out = tgt + npx;
+    float* scratch = out + npx;
+    if (int rc = cnn_forward_train(net, x, n, H, W, out)) return rc;
+    uint16_t* acts[7];
+    for (int k = 0; k < 7; k++) acts[k] = net->tr_act + (size_t)k * npx * 32;
+    uint16_t* gacts[2] = {net->tr_act + 7 * npx * 32, net->tr_act + 8 * npx * 32};
+    return run(c, "cnn_backward", [&] {
+        return cnn_backward(net->params(), net->wpk + CNN_WBF16, x, tgt, n, H, W, margin, acts, out, gacts, scratch, net->grad(), nullptr,
+                            c->stream);
+    });
+}
+
+int cnn_adam_step(ds_cnn* net, float lr) {
+    ds_ctx* c = net->ctx;
+    net->step += 1;
+    net->packed = false;
+    int rc = run(c, "cnn_adam", [&] { return cnn_adam(net->params(), net->grad(), net->adam_m(), net->adam_v(), (int)net->step, lr, c->stream); });
+    if (rc) return rc;
+    return cnn_pack_now(net);
+}
+
+int cnn_check_batch(ds_cnn* net, size_t n, size_t h, size_t w, size_t margin) {
+    ds_ctx* c = net->ctx;
+    if (int rc = cnn_check_shape(c, n, h, w)) return rc;
+    if (2 * margin >= h || 2 * margin >= w) return fail(c, DS_ERR_INVALID, "margin %zu leaves no pixels of %zux%zu", margin, w, h);
+    return cnn_ensure_train(net, (int)n, (int)h, (int)w);
+}
+
+// Learned luminance: LF = L + s (net(x) - x), x = cnn_input(L, s), s from sig.
+int cnn_plane_dev(ds_cnn* net, const float* L, size_t w, size_t h, const float* sig, float* lf) {
+    ds_ctx* c = net->ctx;
+    const size_t n = w * h;
+    if (int rc = cnn_check_shape(c, 1, h, w)) return rc;
+    if (int rc = cnn_ensure_infer(net, n)) return rc;
+    if (int rc = cnn_pack_now(net)) return rc;
+    SLOT(c, x, float*, S_X, n * 4);
+    SLOT(c, o, float*, S_T, n * 4);
+    int rc = run(c, "cnn_input", [&] { return cnn_input(L, (int)w, (int)h, sig, x, c->stream); });
+    if (rc) return rc;
+    uint16_t* a = net->inf_act;
+    uint16_t* b = net->inf_act + n * 32;
+    uint16_t* acts[7] = {a, b, a, b, a, b, a};
+    rc = run(c, "cnn_forward", [&] { return cnn_forward(net->params(), net->wpk, x, 1, (int)h, (int)w, acts, o, c->stream); });
+    if (rc) return rc;
+    return run(c, "cnn_output", [&] { return cnn_output(o, x, L, n, sig, lf, c->stream); });
+}
+
+// denoise with the learned luminance stand-in for NLM, the per-region NLM
+// spot check and fallback, then the reference's composite (enhance.rs:74-92).
+int denoise_learned_dev(ds_ctx* c, ds_cnn* net, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float thr_db,
+                        float* out, ds_learned_stats* stats) {
+    const size_t n = w * h;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (n == 0) return DS_OK;
+    if (search < 0 || search > 1024 || patch > 64) return fail(c, DS_ERR_INVALID, "search %ld / patch %zu out of range", search, patch);
+    SLOT(c, L, float*, S_L, n * 4);
+    SLOT(c, A, float*, S_A, n * 4);
+    SLOT(c, B, float*, S_B, n * 4);
+    SLOT(c, LF, float*, S_LF, n * 4);
+    SLOT(c, sig, float*, S_SIGMA, 64);
+    SLOT(c, ws, float*, S_WS, 12 * n * 4);
+    const size_t nreg = nlm_regions(w, h);
+    SLOT(c, regbuf, float*, S_REGIONS, nreg * 5 + 64);
+    float* gap = regbuf;
+    uint8_t* flags = reinterpret_cast<uint8_t*>(regbuf + nreg);
+    int rc = run(c, "oklab_fwd", [&] { return launch_oklab_fwd(rgb, n, c->ok, L, A, B, c->stream); });
+    if (rc) return rc;
+    if ((rc = noise_dev(c, L, w, h, sig))) return rc;
+    if ((rc = cnn_plane_dev(net, L, w, h, sig, LF))) return rc;
+    const float k = amount / 100.0f;
+    NlmH hp{0.0f, sig, 0.6f + 1.6f * k};
+    rc = run(c, "nlm_spot_check", [&] { return launch_nlm_spot_check(L, LF, w, h, hp, search, patch, thr_db, gap, flags, c->stream); });
+    if (rc) return rc;
+    rc = run(c, "nlm_fallback", [&] { return launch_nlm_regions(L, LF, w, h, hp, search, patch, flags, c->stream); });
+    if (rc) return rc;
+    const float m = 0.5f + 0.5f * k;
+    float rf = std::fmax(((float)(w > h ? w : h) / 600.0f) * (1.0f + 2.0f * k), 2.0f);
+    size_t r = (size_t)rf;
+    rc = run(c, "guided_filter", [&] { return launch_guided_pair(L, LF, A, B, m, r, sig, c->ok, ws, out, w, h, c->stream); });
+    if (rc || !stats) return rc;
+    std::vector<float> g(nreg);
+    std::vector<uint8_t> f(nreg);
+    HIPCHK(c, hipMemcpyAsync(g.data(), gap, nreg * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipMemcpyAsync(f.data(), flags, nreg, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    stats->regions = nreg;
+    double sum = 0.0;
+    float mx = 0.0f;
+    for (size_t i = 0; i < nreg; i++) {
+        stats->fallback += f[i] ? 1 : 0;
+        sum += g[i];
+        mx = std::max(mx, g[i]);
+    }
+    stats->mean_gap_db = (float)(sum / (double)nreg);
+    stats->max_gap_db = mx;
+    return DS_OK;
+}
+
+// Training pairs from one synthetic frame: demosaic (RCD), Oklab L, noise
+// sigma, NLM at h = 0.6 sigma; x = cnn_input(L), t = x + (LF - L) / s;
+// n_crops crops of crop x crop at positions drawn from seed.
+int make_pairs_dev(ds_ctx* c, uint64_t seed, size_t fw, size_t fh, float iso, long search, size_t patch, size_t crop, size_t n_crops,
+                   float* x_out, float* t_out) {
+    if (crop == 0 || crop > fw || crop > fh || n_crops == 0) return fail(c, DS_ERR_INVALID, "crop %zu of %zux%zu", crop, fw, fh);
+    if (!(iso >= 1.0f)) return fail(c, DS_ERR_INVALID, "iso %g < 1", (double)iso);
+    const size_t n = fw * fh;
+    static const uint8_t rggb[4] = {0, 1, 1, 2};
+    const float white[3] = {16383.0f, 16383.0f, 16383.0f};
+    const float ones[3] = {1.0f, 1.0f, 1.0f};
+    SLOT(c, raw, uint16_t*, S_IN, n * 2);
+    SLOT(c, rgb, float*, S_OUT, n * 12);
+    SLOT(c, L, float*, S_L, n * 4);
+    SLOT(c, A, float*, S_A, n * 4);
+    SLOT(c, B, float*, S_B, n * 4);
+    SLOT(c, LF, float*, S_LF, n * 4);
+    SLOT(c, sig, float*, S_SIGMA, 64);
+    SLOT(c, x, float*, S_X, n * 4);
+    SLOT(c, t, float*, S_T, n * 4);
+    int rc = synth_dev(c, seed, fw, fh, 14, iso, rggb, raw);
+    if (rc) return rc;
+    if ((rc = demosaic_dev(c, nullptr, raw, fw, fh, fw, rggb, 512.0f, white, ones, DS_ALGO_RCD, rgb, nullptr))) return rc;
+    if ((rc = run(c, "oklab_fwd", [&] { return launch_oklab_fwd(rgb, n, c->ok, L, A, B, c->stream); }))) return rc;
+    if ((rc = noise_dev(c, L, fw, fh, sig))) return rc;
+    if ((rc = nlm_dev(c, L, fw, fh, NlmH{0.0f, sig, 0.6f}, search, patch, LF))) return rc;
+    if ((rc = run(c, "cnn_input", [&] { return cnn_input(L, (int)fw, (int)fh, sig, x, c->stream); }))) return rc;
+    if ((rc = run(c, "cnn_target", [&] { return cnn_target(x, L, LF, n, sig, t, c->stream); }))) return rc;
+    return run(c, "cnn_crops", [&] { return cnn_crops(x, t, (int)fw, (int)fh, (int)crop, (int)n_crops, seed, x_out, t_out, c->stream); });
+}
+
+// He-normal initial weights from a seeded splitmix64 / Box-Muller stream;
+// zero biases.
+void cnn_init_params(uint64_t seed, std::vector<float>& p) {
+    p.assign(CNN_NPARAM, 0.0f);
+    SplitMix64 r{seed ^ 0x5DEECE66DULL};
+    auto normal = [&]() {
+        const double u1 = r.uniform(), u2 = r.uniform();
+        return std::sqrt(-2.0 * std::log(u1 + 1e-300)) * std::cos(6.283185307179586 * u2);
+    };
+    for (size_t i = 0; i < 32 * 9; i++) p[CNN_OFF_IN + i] = (float)(normal() * std::sqrt(2.0 / 9.0));
+    for (int l = 0; l < 6; l++)
+        for (size_t i = 0; i < 32 * 9 * 32; i++) p[CNN_OFF_MID + l * CNN_MID_STRIDE + i] = (float)(normal() * std::sqrt(2.0 / 288.0));
+    // a small output layer: the untrained network starts close to the identity
+    for (size_t i = 0; i < 9 * 32; i++) p[CNN_OFF_OUT + i] = (float)(normal() * 0.1 * std::sqrt(2.0 / 288.0));
+}
+
 
 }  // namespace
 
@@ -720,7 +995,7 @@ int ds_prof_get(ds_ctx* c, ds_prof_entry* out, int cap) {
 // ---- synthetic frames
 
 int ds_synth_bayer_host(uint64_t seed, size_t w, size_t h, int bits, float iso, const uint8_t pattern[4], uint16_t* raw_out) {
-    if (!raw_out || !valid_bayer(pattern) || (bits != 12 && bits != 14) || !(iso >= 1.0f)) return DS_ERR_INVALID;
+    if (!raw_out || !valid_bayer(pattern) || (bits != 12 && bits != 14) || !(iso >= 1.0f || iso == 0.0f)) return DS_ERR_INVALID;
     if (w > (1u << 20) || h > (1u << 20)) return DS_ERR_INVALID;
     SynthScene* s = new SynthScene;
     synth_scene_init(s, seed, (int)w, (int)h, bits, iso, pattern);
@@ -732,21 +1007,238 @@ int ds_synth_bayer_host(uint64_t seed, size_t w, size_t h, int bits, float iso, 
 
 int ds_synth_bayer_dev(ds_ctx* c, uint64_t seed, size_t w, size_t h, int bits, float iso, const uint8_t pattern[4], uint16_t* raw_out) {
     LOCK(c);
-    if (!raw_out || !valid_bayer(pattern) || (bits != 12 && bits != 14) || !(iso >= 1.0f) || w > (1u << 20) || h > (1u << 20))
-        return fail(c, DS_ERR_INVALID, "invalid synthetic frame arguments");
-    SynthScene s;
-    synth_scene_init(&s, seed, (int)w, (int)h, bits, iso, pattern);
-    size_t n = w * h;
-    if (n == 0) return DS_OK;
-    SLOT(c, d_scene, SynthScene*, S_SCENE, sizeof(SynthScene));
-    HIPCHK(c, hipMemcpyAsync(d_scene, &s, sizeof(SynthScene), hipMemcpyHostToDevice, c->stream), "H2D");
-    int rc = run(c, "synth_bayer", [&] {
-        hipLaunchKernelGGL(k_synth, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d_scene, raw_out);
-        return hipGetLastError();
-    });
-    // the scene lives on this stack frame until the copy has been consumed
+    return synth_dev(c, seed, w, h, bits, iso, pattern, raw_out);
+}
+
+// ---- learned denoiser
+
+size_t ds_cnn_param_count(void) { return CNN_NPARAM; }
+
+int ds_cnn_create(ds_ctx* c, uint64_t seed, ds_cnn** out) {
+    LOCK(c);
+    if (!out) return fail(c, DS_ERR_INVALID, "out is null");
+    *out = nullptr;
+    ds_cnn* net = new ds_cnn();
+    net->ctx = c;
+    net->seed = seed;
+    hipError_t e = hipMalloc(&net->state, 4 * CNN_NPARAM * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&net->wpk, 2 * CNN_WBF16 * sizeof(uint16_t));
+    if (e != hipSuccess) {
+        if (net->state) hipFree(net->state);
+        delete net;
+        return hip_fail(c, e, "hipMalloc (CNN state)");
+    }
+    std::vector<float> p;
+    cnn_init_params(seed, p);
+    std::vector<float> zero(3 * CNN_NPARAM, 0.0f);
+    e = hipMemcpyAsync(net->state, p.data(), CNN_NPARAM * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(net->state + CNN_NPARAM, zero.data(), 3 * CNN_NPARAM * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        hipFree(net->state);
+        hipFree(net->wpk);
+        delete net;
+        return hip_fail(c, e, "CNN initialisation");
+    }
+    *out = net;
+    return DS_OK;
+}
+
+void ds_cnn_destroy(ds_cnn* net) {
+    if (!net) return;
+    ds_ctx* c = net->ctx;
+    {
+        std::lock_guard<std::recursive_mutex> g(c->mu);
+        hipSetDevice(c->device);
+        hipStreamSynchronize(c->stream);
+        hipFree(net->state);
+        hipFree(net->wpk);
+        if (net->tr_act) hipFree(net->tr_act);
+        if (net->tr_f) hipFree(net->tr_f);
+        if (net->inf_act) hipFree(net->inf_act);
+    }
+    delete net;
+}
+
+int ds_cnn_get_params(ds_cnn* net, float* params, size_t n) {
+    if (!net) return DS_ERR_INVALID;
+    ds_ctx* c = net->ctx;
+    LOCK(c);
+    if (!params || n != CNN_NPARAM) return fail(c, DS_ERR_INVALID, "expected %zu parameters", CNN_NPARAM);
+    HIPCHK(c, hipMemcpyAsync(params, net->params(), n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    return rc;
+    return DS_OK;
+}
+
+int ds_cnn_set_params(ds_cnn* net, const float* params, size_t n) {
+    if (!net) return DS_ERR_INVALID;
+    ds_ctx* c = net->ctx;
+    LOCK(c);
+    if (!params || n != CNN_NPARAM) return fail(c, DS_ERR_INVALID, "expected %zu parameters", CNN_NPARAM);
+    std::vector<float> zero(3 * CNN_NPARAM, 0.0f);
+    HIPCHK(c, hipMemcpyAsync(net->params(), params, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    HIPCHK(c, hipMemcpyAsync(net->grad(), zero.data(), 3 * CNN_NPARAM * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    net->step = 0;
+    net->packed = false;
+    return DS_OK;
+}
+
+long ds_cnn_step(const ds_cnn* net) { return net ? net->step : -1; }
+
+// Checkpoint file: "DSCNN001", u32 nparam, u32 0, i64 step, u64 seed,
+// then params, adam m, adam v (nparam f32 each), little-endian.
+int ds_cnn_save(ds_cnn* net, const char* path) {
+    if (!net) return DS_ERR_INVALID;
+    ds_ctx* c = net->ctx;
+    LOCK(c);
+    if (!path) return fail(c, DS_ERR_INVALID, "path is null");
+    std::vector<float> buf(4 * CNN_NPARAM);
+    HIPCHK(c, hipMemcpyAsync(buf.data(), net->state, buf.size() * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(c, DS_ERR_INVALID, "cannot open %s for writing", path);
+    const char magic[8] = {'D', 'S', 'C', 'N', 'N', '0', '0', '1'};
+    const uint32_t hdr[2] = {(uint32_t)CNN_NPARAM, 0u};
+    const int64_t step = net->step;
+    const uint64_t seed = net->seed;
+    bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 2, f) == 2 && std::fwrite(&step, 8, 1, f) == 1 &&
+              std::fwrite(&seed, 8, 1, f) == 1 && std::fwrite(buf.data(), 4, CNN_NPARAM, f) == CNN_NPARAM &&  // params
+              std::fwrite(buf.data() + 2 * CNN_NPARAM, 4, 2 * CNN_NPARAM, f) == 2 * CNN_NPARAM;              // m, v
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? DS_OK : fail(c, DS_ERR_INVALID, "write to %s failed", path);
+}
+
+int ds_cnn_load(ds_cnn* net, const char* path) {
+    if (!net) return DS_ERR_INVALID;
+    ds_ctx* c = net->ctx;
+    LOCK(c);
+    if (!path) return fail(c, DS_ERR_INVALID, "path is null");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(c, DS_ERR_INVALID, "cannot open %s", path);
+    char magic[8];
+    uint32_t hdr[2];
+    int64_t step = 0;
+    uint64_t seed = 0;
+    std::vector<float> buf(4 * CNN_NPARAM, 0.0f);
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "DSCNN001", 8) == 0 && std::fread(hdr, 4, 2, f) == 2 &&
+              hdr[0] == CNN_NPARAM && std::fread(&step, 8, 1, f) == 1 && std::fread(&seed, 8, 1, f) == 1 &&
+              std::fread(buf.data(), 4, CNN_NPARAM, f) == CNN_NPARAM &&
+              std::fread(buf.data() + 2 * CNN_NPARAM, 4, 2 * CNN_NPARAM, f) == 2 * CNN_NPARAM && step >= 0;
+    std::fclose(f);
+    if (!ok) return fail(c, DS_ERR_INVALID, "%s is not a checkpoint of this network", path);
+    HIPCHK(c, hipMemcpyAsync(net->state, buf.data(), buf.size() * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    net->step = (long)step;
+    net->seed = seed;
+    net->packed = false;
+    return DS_OK;
+}
+
+int ds_cnn_forward(ds_cnn* net, const float* x, size_t n_img, size_t h, size_t w, float* out) {
+    if (!net) return DS_ERR_INVALID;
+    ds_ctx* c = net->ctx;
+    LOCK(c);
+    if (!x || !out) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (int rc = cnn_check_batch(net, n_img, h, w, 0)) return rc;
+    const size_t npx = n_img * h * w;
+    float* dx = net->tr_f;
+    float* dout = net->tr_f + 2 * npx;
+    HIPCHK(c, hipMemcpyAsync(dx, x, npx * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = cnn_forward_train(net, dx, (int)n_img, (int)h, (int)w, dout)) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, dout, npx * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_cnn_loss_grad(ds_cnn* net, const float* x, const float* tgt, size_t n_img, size_t h, size_t w, size_t margin, double* loss,
+                     float* grad) {
+    if (!net) return DS_ERR_INVALID;
+    ds_ctx* c = net->ctx;
+    LOCK(c);
+    if (!x || !tgt) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (int rc = cnn_check_batch(net, n_img, h, w, margin)) return rc;
+    const size_t npx = n_img * h * w;
+    HIPCHK(c, hipMemcpyAsync(net->tr_f, x, npx * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    HIPCHK(c, hipMemcpyAsync(net->tr_f + npx, tgt, npx * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = cnn_grad_dev(net, (int)n_img, (int)h, (int)w, (int)margin)) return rc;
+    if (grad) HIPCHK(c, hipMemcpyAsync(grad, net->grad(), CNN_NPARAM * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    double l = 0.0;
+    if (int rc = cnn_loss_host(net, npx, (int)n_img, (int)h, (int)w, (int)margin, &l)) return rc;
+    if (loss) *loss = l;
+    return DS_OK;
+}
+
+int ds_cnn_train_step_dev(ds_cnn* net, const float* x, const float* tgt, size_t n_img, size_t h, size_t w, size_t margin, float lr,
+                          double* loss_out) {
+    if (!net) return DS_ERR_INVALID;
+    ds_ctx* c = net->ctx;
+    LOCK(c);
+    if (!x || !tgt) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (!(lr > 0.0f) || lr > 1.0f) return fail(c, DS_ERR_INVALID, "learning rate %g out of range", (double)lr);
+    if (int rc = cnn_check_batch(net, n_img, h, w, margin)) return rc;
+    const size_t npx = n_img * h * w;
+    HIPCHK(c, hipMemcpyAsync(net->tr_f, x, npx * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    HIPCHK(c, hipMemcpyAsync(net->tr_f + npx, tgt, npx * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    if (int rc = cnn_grad_dev(net, (int)n_img, (int)h, (int)w, (int)margin)) return rc;
+    if (int rc = cnn_adam_step(net, lr)) return rc;
+    if (loss_out) return cnn_loss_host(net, npx, (int)n_img, (int)h, (int)w, (int)margin, loss_out);
+    return DS_OK;
+}
+
+int ds_cnn_make_pairs_dev(ds_ctx* c, uint64_t seed, size_t frame_w, size_t frame_h, float iso, long search, size_t patch, size_t crop,
+                          size_t n_crops, float* x_out, float* t_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, frame_w, frame_h)) return rc;
+    if (!x_out || !t_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (search < 0 || search > 1024 || patch > 64) return fail(c, DS_ERR_INVALID, "search %ld / patch %zu out of range", search, patch);
+    return make_pairs_dev(c, seed, frame_w, frame_h, iso, search, patch, crop, n_crops, x_out, t_out);
+}
+
+int ds_cnn_denoise_plane(ds_cnn* net, const float* L, size_t w, size_t h, float* lf_out) {
+    if (!net) return DS_ERR_INVALID;
+    ds_ctx* c = net->ctx;
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    if (!L || !lf_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    SLOT(c, dL, float*, S_IN, n * 4);
+    SLOT(c, dLF, float*, S_OUT, n * 4);
+    SLOT(c, sig, float*, S_SIGMA, 64);
+    HIPCHK(c, hipMemcpyAsync(dL, L, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = noise_dev(c, dL, w, h, sig)) return rc;
+    if (int rc = cnn_plane_dev(net, dL, w, h, sig, dLF)) return rc;
+    HIPCHK(c, hipMemcpyAsync(lf_out, dLF, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_denoise_learned_dev(ds_ctx* c, ds_cnn* net, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch,
+                           float threshold_db, float* rgb_out, ds_learned_stats* stats) {
+    LOCK(c);
+    if (!net || net->ctx != c) return fail(c, DS_ERR_INVALID, "network does not belong to this context");
+    if (int rc = check_dims(c, w, h)) return rc;
+    if ((!rgb || !rgb_out) && w * h) return fail(c, DS_ERR_INVALID, "null pointer");
+    return denoise_learned_dev(c, net, rgb, w, h, amount, search, patch, threshold_db, rgb_out, stats);
+}
+
+int ds_denoise_learned(ds_ctx* c, ds_cnn* net, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch,
+                       float threshold_db, float* rgb_out, ds_learned_stats* stats) {
+    LOCK(c);
+    if (!net || net->ctx != c) return fail(c, DS_ERR_INVALID, "network does not belong to this context");
+    if (int rc = check_dims(c, w, h)) return rc;
+    const size_t n = w * h;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (n == 0) return DS_OK;
+    if (!rgb || !rgb_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    SLOT(c, d_in, float*, S_IN, n * 12);
+    SLOT(c, d_out, float*, S_OUT, n * 12);
+    HIPCHK(c, hipMemcpyAsync(d_in, rgb, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = denoise_learned_dev(c, net, d_in, w, h, amount, search, patch, threshold_db, d_out, stats)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb_out, d_out, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
 }
 
 }  // extern "C"

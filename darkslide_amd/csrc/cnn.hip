@@ -1,0 +1,598 @@
+// Learned luminance denoiser: a residual CNN that stands in for the NLM
+// stage of denoise.  It has no counterpart in the reference; its training
+// targets are the NLM outputs of this library.
+//
+// Network (input x, 1 channel; 32 channels inside; 8 conv3x3 layers):
+//     a1 = relu(conv_in(x))            1 -> 32   (FP32 VALU)
+//     a{k+1} = relu(conv_k(a_k))       32 -> 32, k = 1..6   (BF16 MFMA)
+//     out = x + conv_out(a7)           32 -> 1   (FP32 VALU, residual epilogue)
+// with zero padding.  Activations are NHWC bf16 (32 channels = 64 bytes per
+// pixel); every accumulation is FP32.
+//
+// The 32 -> 32 convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16:
+// D[co][px] = sum over (tap, ci) of W[co][tap][ci] * X[px + tap][ci], with the
+// weights (A operand, 16 x 32 per MFMA) held in registers for the whole
+// block and the input tile plus its 1-pixel halo (B operand) in LDS.  The
+// data-gradient pass is the same kernel with the weights flipped and
+// transposed; the weight gradient is a GEMM over pixels through an im2col
+// tile in LDS.
+//
+// MFMA 16x16x32 operand layout (lane l of the wave, g = l / 16):
+//   A: lane holds A[m = l % 16][k = 8g .. 8g + 7]
+//   B: lane holds B[k = 8g .. 8g + 7][n = l % 16]
+//   D: lane holds D[m = 4g + i][n = l % 16], i = 0..3
+
+#include "cnn.h"
+#include "common.h"
+
+namespace ds {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int C = 32;       // channels
+constexpr int CP = 40;      // LDS pixel stride in bf16 (80 bytes: 16-byte aligned, spreads banks)
+constexpr int TW = 64;      // output tile width
+constexpr int TH = 4;       // output tile rows (one per wave)
+constexpr int NT = 256;
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    __bf16 b = (__bf16)f;  // round to nearest even
+    return __builtin_bit_cast(uint16_t, b);
+}
+__device__ __forceinline__ float bf2f(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
+
+// ---------------------------------------------------------------- input / output layers (VALU, FP32)
+
+// a1[p][co] = relu(b[co] + sum_tap W[co][tap] * x[p + tap])
+__global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, const float* __restrict__ W, const float* __restrict__ b,
+                                                 uint16_t* __restrict__ act, int H, int Wd, int n_img) {
+    __shared__ float sw[C * 9 + C];
+    for (int i = threadIdx.x; i < C * 9 + C; i += blockDim.x) sw[i] = i < C * 9 ? W[i] : b[i - C * 9];
+    __syncthreads();
+    const size_t npx = (size_t)H * Wd;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npx * n_img) return;
+    const size_t img = p / npx, q = p % npx;
+    const int y = (int)(q / Wd), xx = (int)(q % Wd);
+    const float* xi = x + img * npx;
+    float v[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
+        v[t] = (yy >= 0 && yy < H && xc >= 0 && xc < Wd) ? xi[(size_t)yy * Wd + xc] : 0.0f;
+    }
+    uint16_t o[C];
+#pragma unroll
+    for (int co = 0; co < C; co++) {
+        float a = sw[C * 9 + co];
+#pragma unroll
+        for (int t = 0; t < 9; t++) a += sw[co * 9 + t] * v[t];
+        o[co] = f2bf(fmaxf(a, 0.0f));
+    }
+    uint4* dst = reinterpret_cast<uint4*>(act + p * C);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint4 u;
+        u.x = o[8 * k] | ((uint32_t)o[8 * k + 1] << 16);
+        u.y = o[8 * k + 2] | ((uint32_t)o[8 * k + 3] << 16);
+        u.z = o[8 * k + 4] | ((uint32_t)o[8 * k + 5] << 16);
+        u.w = o[8 * k + 6] | ((uint32_t)o[8 * k + 7] << 16);
+        dst[k] = u;
+    }
+}
+
+__device__ __forceinline__ void load_px32(const uint16_t* p, float* v) {
+    const uint4* s = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint4 u = s[k];
+        uint32_t w4[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            v[8 * k + 2 * j] = bf2f((uint16_t)(w4[j] & 0xffff));
+            v[8 * k + 2 * j + 1] = bf2f((uint16_t)(w4[j] >> 16));
+        }
+    }
+}
+
+// out[p] = x[p] + b + sum_{tap, ci} W[tap][ci] * a7[p + tap][ci]
+__global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ act, const float* __restrict__ x, const float* __restrict__ W,
+                                                  const float* __restrict__ b, float* __restrict__ out, int H, int Wd, int n_img) {
+    __shared__ float sw[9 * C + 1];
+    for (int i = threadIdx.x; i < 9 * C + 1; i += blockDim.x) sw[i] = i < 9 * C ? W[i] : b[0];
+    __syncthreads();
+    const size_t npx = (size_t)H * Wd;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npx * n_img) return;
+    const size_t img = p / npx, q = p % npx;
+    const int y = (int)(q / Wd), xx = (int)(q % Wd);
+    float acc = sw[9 * C];
+    for (int t = 0; t < 9; t++) {
+        const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
+        if (yy < 0 || yy >= H || xc < 0 || xc >= Wd) continue;
+        float v[C];
+        load_px32(act + (img * npx + (size_t)yy * Wd + xc) * C, v);
+#pragma unroll
+        for (int ci = 0; ci < C; ci++) acc += sw[t * C + ci] * v[ci];
+    }
+    out[p] = x[p] + acc;
+}
+
+// ---------------------------------------------------------------- 32 -> 32 conv on MFMA
+
+// MODE 0: forward, epilogue relu(acc + bias).
+// MODE 1: data gradient, epilogue acc * (mask > 0) (mask = the layer input's activation).
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_conv_mid(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wgt,
+                                                 const float* __restrict__ bias, const uint16_t* __restrict__ mask,
+                                                 uint16_t* __restrict__ out, int H, int Wd) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[(TH + 2) * (TW + 2) * CP];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const size_t img_off = (size_t)blockIdx.z * H * Wd;
+    // stage the input tile + 1-pixel halo, 16 bytes per load, zero outside the image
+    {
+        constexpr int N16 = (TH + 2) * (TW + 2) * (C / 8);
+        constexpr int K = (N16 + NT - 1) / NT;
+        uint4 r[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = tid + k * NT;
+            const int pix = i >> 2, part = i & 3;
+            const int ty = pix / (TW + 2), tx = pix % (TW + 2);
+            const int gy = y0 + ty - 1, gx = x0 + tx - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (i < N16 && gy >= 0 && gy < H && gx >= 0 && gx < Wd)
+                v = reinterpret_cast<const uint4*>(in + (img_off + (size_t)gy * Wd + gx) * C)[part];
+            r[k] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = tid + k * NT;
+            if (i < N16) reinterpret_cast<uint4*>(s_in + (i >> 2) * CP)[i & 3] = r[k];
+        }
+    }
+    // weights as A operands: rows co = mt*16 + lane%16, k = tap*32 + 8*(lane/16) .. +7
+    const int g = lane >> 4, ln = lane & 15;
+    bf16x8 wa[2][9];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+            wa[mt][t] = *reinterpret_cast<const bf16x8*>(wgt + ((mt * 16 + ln) * 9 + t) * C + 8 * g);
+    float bco[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) bco[mt][i] = MODE == 0 ? bias[mt * 16 + 4 * g + i] : 0.0f;
+    __syncthreads();
+
+    const int oy = y0 + wave;
+#pragma unroll
+    for (int nt = 0; nt < TW / 16; nt++) {
+        f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            const int ky = t / 3, kx = t % 3;
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(s_in + ((wave + ky) * (TW + 2) + nt * 16 + ln + kx) * CP + 8 * g);
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[0][t], bv, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[1][t], bv, acc[1], 0, 0, 0);
+        }
+        const int ox = x0 + nt * 16 + ln;
+        if (oy < H && ox < Wd) {
+            const size_t p = img_off + (size_t)oy * Wd + ox;
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++) {
+                float v[4];
+                if (MODE == 0) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) v[i] = fmaxf(acc[mt][i] + bco[mt][i], 0.0f);
+                } else {
+                    const uint2 mk = *reinterpret_cast<const uint2*>(mask + p * C + mt * 16 + 4 * g);
+                    const uint16_t m4[4] = {(uint16_t)(mk.x & 0xffff), (uint16_t)(mk.x >> 16), (uint16_t)(mk.y & 0xffff), (uint16_t)(mk.y >> 16)};
+#pragma unroll
+                    for (int i = 0; i < 4; i++) v[i] = bf2f(m4[i]) > 0.0f ? acc[mt][i] : 0.0f;
+                }
+                uint2 o;
+                o.x = f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                o.y = f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                *reinterpret_cast<uint2*>(out + p * C + mt * 16 + 4 * g) = o;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- weight gradient of a 32 -> 32 layer (MFMA)
+
+// dW[co][tap][ci] = sum_p dY[p][co] * X[p + tap][ci];  db[co] = sum_p dY[p][co].
+// Each block walks `segs` 64-pixel row segments (image row order), builds for
+// each the im2col tile Xc[n = tap*32 + ci][64 px] and dY^T[co][64 px] in LDS,
+// and accumulates D[n][co] (18 x 2 MFMA tiles, split over the 4 waves).
+// Partial sums per block go to `part` (C*9*C + C floats per block).
+constexpr int WG_PX = 64;
+constexpr int WG_XS = WG_PX + 8;  // LDS row stride of the [n][px] tiles (bf16)
+__global__ __launch_bounds__(NT) void k_wgrad_mid(const uint16_t* __restrict__ X, const uint16_t* __restrict__ dY, int H, int Wd, int n_img,
+                                                  int segs, float* __restrict__ part) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_xc[9 * C * WG_XS];
+    __shared__ __attribute__((aligned(16))) uint16_t s_dy[C * WG_XS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, ln = lane & 15;
+    const int segs_per_row = (Wd + WG_PX - 1) / WG_PX;
+    const long total_segs = (long)segs_per_row * H * n_img;
+    // wave w owns n-tiles {w, w+4, w+8, ...} of the 18, both co tiles
+    f32x4 acc[5][2];
+#pragma unroll
+    for (int a = 0; a < 5; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+    float dbacc = 0.0f;
+    for (int s = 0; s < segs; s++) {
+        const long seg = (long)blockIdx.x * segs + s;
+        if (seg >= total_segs) break;
+        const int img = (int)(seg / ((long)segs_per_row * H));
+        const int rem = (int)(seg % ((long)segs_per_row * H));
+        const int y = rem / segs_per_row, x0 = (rem % segs_per_row) * WG_PX;
+        const size_t ioff = (size_t)img * H * Wd;
+        __syncthreads();
+        // dY^T tile: s_dy[co][px]; and the bias gradient
+        for (int i = tid; i < WG_PX * C; i += NT) {
+            const int px = i / C, co = i % C;
+            const int gx = x0 + px;
+            uint16_t v = 0;
+            if (gx < Wd) v = dY[(ioff + (size_t)y * Wd + gx) * C + co];
+            s_dy[co * WG_XS + px] = v;
+        }
+        // im2col: s_xc[tap*32 + ci][px] = X[y + ky - 1][x0 + px + kx - 1][ci]
+        for (int i = tid; i < 9 * WG_PX * C; i += NT) {
+            const int t = i / (WG_PX * C), r = i % (WG_PX * C);
+            const int px = r / C, ci = r % C;
+            const int gy = y + t / 3 - 1, gx = x0 + px + t % 3 - 1;
+            uint16_t v = 0;
+            if (gy >= 0 && gy < H && gx >= 0 && gx < Wd && x0 + px < Wd) v = X[(ioff + (size_t)gy * Wd + gx) * C + ci];
+            s_xc[(t * C + ci) * WG_XS + px] = v;
+        }
+        __syncthreads();
+        if (tid < C) {
+            for (int px = 0; px < WG_PX; px++) dbacc += bf2f(s_dy[tid * WG_XS + px]);
+        }
+        // D[n][co] += sum_px Xc[n][px] * dY[px][co]; K = 64 px = 2 MFMA steps
+#pragma unroll
+        for (int a = 0; a < 5; a++) {
+            const int nt = wave + 4 * a;
+            if (nt >= 18) break;
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++) {
+                const bf16x8 av = *reinterpret_cast<const bf16x8*>(s_xc + (nt * 16 + ln) * WG_XS + kk * 32 + 8 * g);
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const bf16x8 bv = *reinterpret_cast<const bf16x8*>(s_dy + (b * 16 + ln) * WG_XS + kk * 32 + 8 * g);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[a][b], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // write partials: D[n][co] at lane: n = nt*16 + 4g + i, co = b*16 + ln -> dW[co][tap][ci] with n = tap*32 + ci
+    float* pb = part + (size_t)blockIdx.x * (C * 9 * C + C);
+#pragma unroll
+    for (int a = 0; a < 5; a++) {
+        const int nt = wave + 4 * a;
+        if (nt >= 18) break;
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int n = nt * 16 + 4 * g + i, co = b * 16 + ln;
+                pb[co * 9 * C + n] = acc[a][b][i];
+            }
+    }
+    if (tid < C) pb[C * 9 * C + tid] = dbacc;
+}
+
+// Sum block partials: out[j] = sum_b part[b][j].  With split > 0, entries
+// j >= split go to out2[j - split] instead.
+__global__ void k_reduce_parts(const float* __restrict__ part, int nblk, int n, float* __restrict__ out, int split, float* __restrict__ out2) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float s = 0.0f;
+    for (int b = 0; b < nblk; b++) s += part[(size_t)b * n + j];
+    if (split > 0 && j >= split)
+        out2[j - split] = s;
+    else
+        out[j] = s;
+}
+
+// ---------------------------------------------------------------- loss and the VALU layers' gradients
+
+// dout = 2 (out - tgt) / n_valid inside the crop margin, 0 outside; loss partials.
+__global__ void k_mse(const float* __restrict__ out, const float* __restrict__ tgt, int H, int Wd, int n_img, int margin, float inv_n,
+                      float* __restrict__ dout, float* __restrict__ loss_part) {
+    __shared__ float red[256];
+    const size_t npx = (size_t)H * Wd;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (p < npx * n_img) {
+        const size_t q = p % npx;
+        const int y = (int)(q / Wd), x = (int)(q % Wd);
+        const bool in = y >= margin && y < H - margin && x >= margin && x < Wd - margin;
+        const float e = out[p] - tgt[p];
+        dout[p] = in ? 2.0f * e * inv_n : 0.0f;
+        l = in ? e * e : 0.0f;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// g7[p][ci] = (a7[p][ci] > 0) * sum_tap W[tap][ci] * dout[p - tap]
+__global__ __launch_bounds__(256) void k_out_dgrad(const float* __restrict__ dout, const float* __restrict__ W, const uint16_t* __restrict__ a7,
+                                                   uint16_t* __restrict__ g7, int H, int Wd, int n_img) {
+    __shared__ float sw[9 * C];
+    for (int i = threadIdx.x; i < 9 * C; i += blockDim.x) sw[i] = W[i];
+    __syncthreads();
+    const size_t npx = (size_t)H * Wd;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npx * n_img) return;
+    const size_t img = p / npx, q = p % npx;
+    const int y = (int)(q / Wd), x = (int)(q % Wd);
+    float d[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        // out[p'] used a7[p' + off(t)]; so a7[p] feeds out[p - off(t)]
+        const int yy = y - (t / 3 - 1), xx = x - (t % 3 - 1);
+        d[t] = (yy >= 0 && yy < H && xx >= 0 && xx < Wd) ? dout[img * npx + (size_t)yy * Wd + xx] : 0.0f;
+    }
+    float a[C];
+    load_px32(a7 + p * C, a);
+    uint16_t o[C];
+#pragma unroll
+    for (int ci = 0; ci < C; ci++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 9; t++) s += sw[t * C + ci] * d[t];
+        o[ci] = f2bf(a[ci] > 0.0f ? s : 0.0f);
+    }
+    uint4* dst = reinterpret_cast<uint4*>(g7 + p * C);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint4 u;
+        u.x = o[8 * k] | ((uint32_t)o[8 * k + 1] << 16);
+        u.y = o[8 * k + 2] | ((uint32_t)o[8 * k + 3] << 16);
+        u.z = o[8 * k + 4] | ((uint32_t)o[8 * k + 5] << 16);
+        u.w = o[8 * k + 6] | ((uint32_t)o[8 * k + 7] << 16);
+        dst[k] = u;
+    }
+}
+
+// Weight gradients of the output layer (dW[tap][ci] = sum_p dout[p] a7[p + tap][ci],
+// db = sum dout) and of the input layer (dW[co][tap] = sum_p g1[p][co] x[p + tap],
+// db[co] = sum g1[p][co]).  One thread per (block, parameter) sums a pixel range.
+__global__ void k_io_wgrad(const float* __restrict__ dout, const uint16_t* __restrict__ a7, const uint16_t* __restrict__ g1,
+                           const float* __restrict__ x, int H, int Wd, int n_img, int px_per_blk, float* __restrict__ part) {
+    // parameters of this pass: out layer 9*C + 1, in layer C*9 + C
+    constexpr int NOUT = 9 * C + 1, NIN = C * 9 + C, NP = NOUT + NIN;
+    const int j = threadIdx.x + blockDim.x * blockIdx.y;  // parameter index
+    if (j >= NP) return;
+    const size_t npx = (size_t)H * Wd, tot = npx * n_img;
+    const size_t p0 = (size_t)blockIdx.x * px_per_blk, p1 = p0 + px_per_blk < tot ? p0 + px_per_blk : tot;
+    float s = 0.0f;
+    if (j < NOUT) {
+        if (j == 9 * C) {
+            for (size_t p = p0; p < p1; p++) s += dout[p];
+        } else {
+            const int t = j / C, ci = j % C, dy = t / 3 - 1, dx = t % 3 - 1;
+            for (size_t p = p0; p < p1; p++) {
+                const size_t img = p / npx, q = p % npx;
+                const int y = (int)(q / Wd) + dy, xx = (int)(q % Wd) + dx;
+                if (y < 0 || y >= H || xx < 0 || xx >= Wd) continue;
+                s += dout[p] * bf2f(a7[(img * npx + (size_t)y * Wd + xx) * C + ci]);
+            }
+        }
+    } else {
+        const int jj = j - NOUT;
+        if (jj >= C * 9) {
+            const int co = jj - C * 9;
+            for (size_t p = p0; p < p1; p++) s += bf2f(g1[p * C + co]);
+        } else {
+            const int co = jj / 9, t = jj % 9, dy = t / 3 - 1, dx = t % 3 - 1;
+            for (size_t p = p0; p < p1; p++) {
+                const size_t img = p / npx, q = p % npx;
+                const int y = (int)(q / Wd) + dy, xx = (int)(q % Wd) + dx;
+                if (y < 0 || y >= H || xx < 0 || xx >= Wd) continue;
+                s += bf2f(g1[p * C + co]) * x[img * npx + (size_t)y * Wd + xx];
+            }
+        }
+    }
+    part[(size_t)blockIdx.x * NP + j] = s;
+}
+
+// ---------------------------------------------------------------- Adam and weight packing
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ gr, float* __restrict__ m, float* __restrict__ v, int n, float lr,
+                       float b1, float b2, float eps, float bc1, float bc2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = gr[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * g;
+    const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// bf16 copies of the six middle layers: forward layout [co][tap][ci] and the
+// data-gradient layout W'[ci][tap'][co] = W[co][8 - tap'][ci].
+__global__ void k_pack_mid(const float* __restrict__ wmid, uint16_t* __restrict__ fwd, uint16_t* __restrict__ bwd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int PER = C * 9 * C;
+    if (i >= 6 * PER) return;
+    const int l = i / PER, r = i % PER;
+    const int co = r / (9 * C), t = (r / C) % 9, ci = r % C;
+    const uint16_t b = f2bf(wmid[(size_t)l * (PER + C) + r]);
+    fwd[(size_t)l * PER + r] = b;
+    bwd[(size_t)l * PER + ((ci * 9) + (8 - t)) * C + co] = b;
+}
+
+// ---------------------------------------------------------------- input normalisation and crops
+
+// x = clamp((L - mean5x5(L)) / s, -64, 64) with s = max(sigma, 1e-4); the
+// 5x5 mean uses clamped coordinates.
+__global__ void k_cnn_input(const float* __restrict__ L, int w, int h, const float* __restrict__ sigma_dev, float* __restrict__ x) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)w * h) return;
+    const int px = (int)(i % w), py = (int)(i / w);
+    float s = 0.0f;
+    for (int dy = -2; dy <= 2; dy++)
+        for (int dx = -2; dx <= 2; dx++) s += L[(size_t)clampi(py + dy, 0, h - 1) * w + clampi(px + dx, 0, w - 1)];
+    const float sg = fmaxf(*sigma_dev, 1e-4f);
+    x[i] = fminf(fmaxf((L[i] - s * (1.0f / 25.0f)) / sg, -64.0f), 64.0f);
+}
+
+// target of training: x + (LF - L) / s
+__global__ void k_cnn_target(const float* __restrict__ x, const float* __restrict__ L, const float* __restrict__ LF, size_t n,
+                             const float* __restrict__ sigma_dev, float* __restrict__ t) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    t[i] = x[i] + (LF[i] - L[i]) / fmaxf(*sigma_dev, 1e-4f);
+}
+
+// lf = L + s * (out - x)
+__global__ void k_cnn_output(const float* __restrict__ out, const float* __restrict__ x, const float* __restrict__ L, size_t n,
+                             const float* __restrict__ sigma_dev, float* __restrict__ lf) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    lf[i] = L[i] + fmaxf(*sigma_dev, 1e-4f) * (out[i] - x[i]);
+}
+
+// cut n_crops crops of c x c from two w x h planes at seeded positions
+__global__ void k_crops(const float* __restrict__ a, const float* __restrict__ b, int w, int h, int c, int n_crops, uint64_t seed,
+                        float* __restrict__ ca, float* __restrict__ cb) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t per = (size_t)c * c;
+    if (i >= per * n_crops) return;
+    const int k = (int)(i / per), r = (int)(i % per);
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + (uint64_t)k * 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 31;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 29;
+    const int ox = (int)((z & 0xffffffffull) % (uint64_t)(w - c + 1)), oy = (int)((z >> 32) % (uint64_t)(h - c + 1));
+    const size_t src = (size_t)(oy + r / c) * w + ox + r % c;
+    ca[i] = a[src];
+    cb[i] = b[src];
+}
+
+unsigned nb(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+
+size_t cnn_param_count() { return CNN_NPARAM; }
+
+hipError_t cnn_pack(const float* params, uint16_t* wfwd, uint16_t* wbwd, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack_mid, dim3(nb(6 * C * 9 * C)), dim3(256), 0, st, params + CNN_OFF_MID, wfwd, wbwd);
+    return hipGetLastError();
+}
+
+// Forward of n_img images H x W: x (fp32) -> out (fp32); acts: 7 activation
+// buffers of n_img*H*W*32 bf16 (kept for the backward pass).
+hipError_t cnn_forward(const float* params, const uint16_t* wfwd, const float* x, int n_img, int H, int W, uint16_t* const* acts, float* out,
+                       hipStream_t st) {
+    const size_t npx = (size_t)n_img * H * W;
+    hipLaunchKernelGGL(k_conv_in, dim3(nb(npx)), dim3(256), 0, st, x, params + CNN_OFF_IN, params + CNN_OFF_IN + C * 9, acts[0], H, W, n_img);
+    dim3 grid((unsigned)((W + TW - 1) / TW), (unsigned)((H + TH - 1) / TH), (unsigned)n_img);
+    for (int l = 0; l < 6; l++) {
+        const float* bias = params + CNN_OFF_MID + (size_t)l * (C * 9 * C + C) + C * 9 * C;
+        hipLaunchKernelGGL(k_conv_mid<0>, grid, dim3(NT), 0, st, acts[l], wfwd + (size_t)l * C * 9 * C, bias, (const uint16_t*)nullptr, acts[l + 1],
+                           H, W);
+    }
+    hipLaunchKernelGGL(k_conv_out, dim3(nb(npx)), dim3(256), 0, st, acts[6], x, params + CNN_OFF_OUT, params + CNN_OFF_OUT + 9 * C, out, H, W,
+                       n_img);
+    return hipGetLastError();
+}
+
+size_t cnn_wgrad_blocks(int n_img, int H, int W) {
+    const long segs = (long)((W + WG_PX - 1) / WG_PX) * H * n_img;
+    const long per = 16;
+    return (size_t)((segs + per - 1) / per);
+}
+
+size_t cnn_scratch_floats(int n_img, int H, int W) {
+    const size_t npx = (size_t)n_img * H * W;
+    const size_t wg = cnn_wgrad_blocks(n_img, H, W) * (C * 9 * C + C);
+    const size_t io_blk = (npx + 4095) / 4096;
+    const size_t io = io_blk * (9 * C + 1 + C * 9 + C);
+    size_t m = wg > io ? wg : io;
+    return npx /* dout */ + nb(npx) /* loss partials */ + m;
+}
+
+// Loss (MSE over the crop interior) and gradients of all parameters into grad.
+// gacts: 2 scratch activation-gradient buffers; scratch: cnn_scratch_floats.
+hipError_t cnn_backward(const float* params, const uint16_t* wbwd, const float* x, const float* tgt, int n_img, int H, int W, int margin,
+                        uint16_t* const* acts, const float* out, uint16_t* const* gacts, float* scratch, float* grad, float* loss_part_out,
+                        hipStream_t st) {
+    const size_t npx = (size_t)n_img * H * W;
+    const long nvalid = (long)n_img * (H - 2 * margin) * (W - 2 * margin);
+    float* dout = scratch;
+    float* lossp = scratch + npx;
+    float* part = lossp + nb(npx);
+    hipLaunchKernelGGL(k_mse, dim3(nb(npx)), dim3(256), 0, st, out, tgt, H, W, n_img, margin, 1.0f / (float)(nvalid > 0 ? nvalid : 1), dout,
+                       loss_part_out ? loss_part_out : lossp);
+    // a7 -> g7
+    hipLaunchKernelGGL(k_out_dgrad, dim3(nb(npx)), dim3(256), 0, st, dout, params + CNN_OFF_OUT, acts[6], gacts[0], H, W, n_img);
+    dim3 grid((unsigned)((W + TW - 1) / TW), (unsigned)((H + TH - 1) / TH), (unsigned)n_img);
+    const int wblk = (int)cnn_wgrad_blocks(n_img, H, W);
+    const int npar = C * 9 * C + C;
+    int cur = 0;
+    for (int l = 5; l >= 0; l--) {
+        // layer l maps acts[l] -> acts[l+1]; gacts[cur] holds dL/d(pre-activation of l+1)
+        hipLaunchKernelGGL(k_wgrad_mid, dim3(wblk), dim3(NT), 0, st, acts[l], gacts[cur], H, W, n_img, 16, part);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(nb(npar)), dim3(256), 0, st, part, wblk, npar, grad + CNN_OFF_MID + (size_t)l * npar, 0,
+                           (float*)nullptr);
+        hipLaunchKernelGGL(k_conv_mid<1>, grid, dim3(NT), 0, st, gacts[cur], wbwd + (size_t)l * C * 9 * C, (const float*)nullptr, acts[l],
+                           gacts[cur ^ 1], H, W);
+        cur ^= 1;
+    }
+    // input and output layers
+    constexpr int NOUT = 9 * C + 1, NIN = C * 9 + C, NP = NOUT + NIN;
+    const int px_per = 4096;
+    const int iob = (int)((npx + px_per - 1) / px_per);
+    hipLaunchKernelGGL(k_io_wgrad, dim3(iob, (NP + 255) / 256), dim3(256), 0, st, dout, acts[6], gacts[cur], x, H, W, n_img, px_per, part);
+    // part layout [blk][NP]: out-layer params first, then in-layer params
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nb(NP)), dim3(256), 0, st, part, iob, NP, grad + CNN_OFF_OUT, NOUT, grad + CNN_OFF_IN);
+    return hipGetLastError();
+}
+
+hipError_t cnn_adam(float* params, const float* grad, float* m, float* v, int step, float lr, hipStream_t st) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float bc1 = 1.0f - powf(b1, (float)step), bc2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(nb(CNN_NPARAM)), dim3(256), 0, st, params, grad, m, v, (int)CNN_NPARAM, lr, b1, b2, eps, bc1, bc2);
+    return hipGetLastError();
+}
+
+hipError_t cnn_input(const float* L, int w, int h, const float* sigma_dev, float* x, hipStream_t st) {
+    hipLaunchKernelGGL(k_cnn_input, dim3(nb((size_t)w * h)), dim3(256), 0, st, L, w, h, sigma_dev, x);
+    return hipGetLastError();
+}
+
+hipError_t cnn_target(const float* x, const float* L, const float* LF, size_t n, const float* sigma_dev, float* t, hipStream_t st) {
+    hipLaunchKernelGGL(k_cnn_target, dim3(nb(n)), dim3(256), 0, st, x, L, LF, n, sigma_dev, t);
+    return hipGetLastError();
+}
+
+hipError_t cnn_output(const float* out, const float* x, const float* L, size_t n, const float* sigma_dev, float* lf, hipStream_t st) {
+    hipLaunchKernelGGL(k_cnn_output, dim3(nb(n)), dim3(256), 0, st, out, x, L, n, sigma_dev, lf);
+    return hipGetLastError();
+}
+
+hipError_t cnn_crops(const float* a, const float* b, int w, int h, int c, int n, uint64_t seed, float* ca, float* cb, hipStream_t st) {
+    hipLaunchKernelGGL(k_crops, dim3(nb((size_t)c * c * n)), dim3(256), 0, st, a, b, w, h, c, n, seed, ca, cb);
+    return hipGetLastError();
+}
+
+}  // namespace ds

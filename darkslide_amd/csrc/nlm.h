@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
 
 namespace ds {
 
@@ -27,9 +28,26 @@ struct NlmWeight {
     __device__ __forceinline__ float scale() const { return -k / h2(); }
 };
 
+// Regions of the learned denoiser's NLM spot check and fallback: square
+// blocks of NLM_REGION pixels, row-major, ceil(w / NLM_REGION) per row.
+constexpr int NLM_REGION = 256;
+constexpr int NLM_SPOT = 16;  // spot-check lattice per region side
+
 size_t nlm_tile_lds_bytes(int S, int P);
+size_t nlm_regions(size_t w, size_t h);
 
 // nlm(p, h, search, patch); search >= 0.
 hipError_t launch_nlm(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, hipStream_t stream);
+
+// NLM over the pixels of the flagged regions only (regions[r] != 0); out
+// keeps its values elsewhere.  regions == nullptr: every pixel.
+hipError_t launch_nlm_regions(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, const uint8_t* regions,
+                              hipStream_t stream);
+
+// Per-region deviation of `cand` from NLM in dB (gap_db) and the fallback
+// flags (gap_db > thr_db, or every region when thr_db <= 0); hp.sigma_dev
+// must be set.
+hipError_t launch_nlm_spot_check(const float* in, const float* cand, size_t w, size_t h, NlmH hp, long search, size_t patch, float thr_db,
+                                 float* gap_db, uint8_t* flags, hipStream_t stream);
 
 }  // namespace ds

@@ -54,7 +54,7 @@ __host__ __device__ inline int nlm_pt_floats(int SP, int P) { return ((nlm_th(P)
 // and a quarter of the tile rows.
 template <int P>
 __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S,
-                                                        NlmWeight wp) {
+                                                        NlmWeight wp, const uint8_t* __restrict__ regions) {
     extern __shared__ float lds[];
     constexpr int TW = NLM_TW, TH = nlm_th(P), NT = NLM_NT, SEG = NLM_SEG, HS = NLM_HS;
     constexpr int DW = SEG + 2 * P;           // differences per phase-1 task
@@ -68,6 +68,17 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
     float* hs0 = lds + nlm_pt_floats(SP, P);  // horizontal sums, 2 x HSR x HS
     const int tid = threadIdx.x;
     const int ox = blockIdx.x * TW, oy = blockIdx.y * TH;
+    // region mode: only tiles that touch a flagged region run, and they store
+    // only the pixels of flagged regions
+    const int nrx = (w + NLM_REGION - 1) / NLM_REGION;
+    if (regions) {
+        const int rx0 = ox / NLM_REGION, rx1 = (min(ox + TW, w) - 1) / NLM_REGION;
+        const int ry0 = oy / NLM_REGION, ry1 = (min(oy + TH, h) - 1) / NLM_REGION;
+        bool any = false;
+        for (int ry = ry0; ry <= ry1; ry++)
+            for (int rx = rx0; rx <= rx1; rx++) any = any || regions[ry * nrx + rx] != 0;
+        if (!any) return;
+    }
 
     // input tile with clamped coordinates: slot (u, v) holds p[clamp(ox+u), clamp(oy+v)]
     stage_tile_n<16, NT>(
@@ -174,21 +185,17 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         const int gx = ox + col, gy = oy + row0 + k;
-        if (row0 + k < TH && gx < w && gy < h) out[(size_t)gy * w + gx] = acc[k] / fmaxf(wsum[k], 1e-12f);
+        if (row0 + k < TH && gx < w && gy < h && (!regions || regions[(gy / NLM_REGION) * nrx + gx / NLM_REGION]))
+            out[(size_t)gy * w + gx] = acc[k] / fmaxf(wsum[k], 1e-12f);
     }
 }
 
-// Any search / patch radius: direct per-pixel patch distances with the same
-// two-level clamp.  Used when the tile kernel's halo or patch limit is
-// exceeded.
-__global__ void k_nlm_direct(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S, int P, NlmWeight wp,
-                             float norm) {
-    const float inv_h2 = 1.0f / wp.h2();
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= w || y >= h) return;
+// NLM at one pixel by direct patch distances, with the two-level clamp of the
+// reference.  Also returns the weight sums (sum w, sum w^2).
+__device__ __forceinline__ float nlm_pixel(const float* __restrict__ in, int w, int h, int S, int P, float inv_h2, float norm, int x, int y,
+                                           float& wsum_out, float& w2sum_out) {
     auto at = [&](int u, int v) { return in[(size_t)clampi(v, 0, h - 1) * w + clampi(u, 0, w - 1)]; };
-    float acc = 0.0f, wsum = 0.0f;
+    float acc = 0.0f, wsum = 0.0f, w2sum = 0.0f;
     for (int dy = -S; dy <= S; dy++)
         for (int dx = -S; dx <= S; dx++) {
             float dist = 0.0f;
@@ -206,12 +213,69 @@ __global__ void k_nlm_direct(const float* __restrict__ in, float* __restrict__ o
             float wt = expf(-(dist * inv_h2));
             acc += wt * at(x + dx, y + dy);
             wsum += wt;
+            w2sum += wt * wt;
         }
-    out[(size_t)y * w + x] = acc / fmaxf(wsum, 1e-12f);
+    wsum_out = wsum;
+    w2sum_out = w2sum;
+    return acc / fmaxf(wsum, 1e-12f);
+}
+
+// Any search / patch radius: one pixel per thread.  Used when the tile
+// kernel's halo or patch limit is exceeded.
+__global__ void k_nlm_direct(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S, int P, NlmWeight wp,
+                             float norm, const uint8_t* __restrict__ regions) {
+    const float inv_h2 = 1.0f / wp.h2();
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    if (regions && !regions[(y / NLM_REGION) * ((w + NLM_REGION - 1) / NLM_REGION) + x / NLM_REGION]) return;
+    float ws, w2;
+    out[(size_t)y * w + x] = nlm_pixel(in, w, h, S, P, inv_h2, norm, x, y, ws, w2);
+}
+
+// Spot check of a candidate result (the learned denoiser's) against NLM.
+// One block per NLM_REGION x NLM_REGION region; its NLM_SPOT x NLM_SPOT
+// threads take a lattice of pixels spread evenly over the region.  Each
+// computes NLM there and the variance NLM leaves of white noise of the
+// estimated sigma, sigma^2 sum w^2 / (sum w)^2.  The region's deviation
+// from NLM, in dB of PSNR, is 10 log10(1 + mean (cand - nlm)^2 / mean var):
+// the PSNR the candidate gives up if its deviation from NLM is independent
+// of NLM's own residual noise, taking that residual at its variance part.
+__global__ __launch_bounds__(NLM_SPOT* NLM_SPOT) void k_nlm_spot(const float* __restrict__ in, const float* __restrict__ cand, int w, int h,
+                                                                 int S, int P, NlmWeight wp, float norm, float thr_db,
+                                                                 float* __restrict__ gap_db, uint8_t* __restrict__ flags) {
+    constexpr int NT = NLM_SPOT * NLM_SPOT;
+    __shared__ float red[2][NT];
+    const int rx = blockIdx.x, ry = blockIdx.y;
+    const int x0 = rx * NLM_REGION, y0 = ry * NLM_REGION;
+    const int rw = min(NLM_REGION, w - x0), rh = min(NLM_REGION, h - y0);
+    const int i = threadIdx.x % NLM_SPOT, j = threadIdx.x / NLM_SPOT;
+    const int x = x0 + ((2 * i + 1) * rw) / (2 * NLM_SPOT), y = y0 + ((2 * j + 1) * rh) / (2 * NLM_SPOT);
+    const float inv_h2 = 1.0f / wp.h2();
+    float ws, w2;
+    const float v = nlm_pixel(in, w, h, S, P, inv_h2, norm, x, y, ws, w2);
+    const float sg = wp.sigma_dev ? fmaxf(*wp.sigma_dev, 1e-4f) : 1e-4f;
+    const float e = cand[(size_t)y * w + x] - v;
+    red[0][threadIdx.x] = e * e;
+    red[1][threadIdx.x] = sg * sg * w2 / fmaxf(ws * ws, 1e-30f);
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float g = 10.0f * log10f(1.0f + red[0][0] / fmaxf(red[1][0], 1e-30f));
+        const int r = ry * gridDim.x + rx;
+        gap_db[r] = g;
+        flags[r] = (thr_db <= 0.0f || !(g <= thr_db)) ? 1 : 0;  // NaN falls back too
+    }
 }
 
 template <int P>
-hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeight wp, hipStream_t stream) {
+hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeight wp, const uint8_t* regions, hipStream_t stream) {
     size_t lds = nlm_tile_lds_bytes(S, P);
     dim3 grid((unsigned)((w + NLM_TW - 1) / NLM_TW), (unsigned)((h + nlm_th(P) - 1) / nlm_th(P)));
     static bool attr_set = false;
@@ -221,7 +285,7 @@ hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeig
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_nlm_tile<P>, grid, dim3(NLM_NT), lds, stream, in, out, w, h, S, wp);
+    hipLaunchKernelGGL(k_nlm_tile<P>, grid, dim3(NLM_NT), lds, stream, in, out, w, h, S, wp, regions);
     return hipGetLastError();
 }
 
@@ -231,30 +295,56 @@ size_t nlm_tile_lds_bytes(int S, int P) {
     return (size_t)(nlm_pt_floats(S + P, P) + 2 * NLM_HSR * NLM_HS) * sizeof(float);
 }
 
-hipError_t launch_nlm(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, hipStream_t stream) {
-    if (w == 0 || h == 0) return hipSuccess;
-    const int S = (int)search;
-    const int P = (int)patch;
-    const double norm = 1.0 / (double)(2 * P + 1);
+namespace {
+NlmWeight make_weight(NlmH hp, int P, double& norm) {
+    norm = 1.0 / (double)(2 * P + 1);
     NlmWeight wp;
     wp.h_param = hp.h_param;
     wp.sigma_dev = hp.sigma_dev;
     wp.sigma_mul = hp.sigma_mul;
     // weight = exp(-(norm^2 * window_sum) / h2) = exp2(window_sum * scale)
     wp.k = (float)(norm * norm * 1.4426950408889634);
+    return wp;
+}
+}  // namespace
+
+hipError_t launch_nlm_regions(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, const uint8_t* regions,
+                              hipStream_t stream) {
+    if (w == 0 || h == 0) return hipSuccess;
+    const int S = (int)search;
+    const int P = (int)patch;
+    double norm;
+    const NlmWeight wp = make_weight(hp, P, norm);
     if (S + P <= NLM_MAX_SP) {
         switch (P) {
-            case 0: return launch_tile<0>(in, out, (int)w, (int)h, S, wp, stream);
-            case 1: return launch_tile<1>(in, out, (int)w, (int)h, S, wp, stream);
-            case 2: return launch_tile<2>(in, out, (int)w, (int)h, S, wp, stream);
-            case 3: return launch_tile<3>(in, out, (int)w, (int)h, S, wp, stream);
-            case 4: return launch_tile<4>(in, out, (int)w, (int)h, S, wp, stream);
-            case 5: return launch_tile<5>(in, out, (int)w, (int)h, S, wp, stream);
+            case 0: return launch_tile<0>(in, out, (int)w, (int)h, S, wp, regions, stream);
+            case 1: return launch_tile<1>(in, out, (int)w, (int)h, S, wp, regions, stream);
+            case 2: return launch_tile<2>(in, out, (int)w, (int)h, S, wp, regions, stream);
+            case 3: return launch_tile<3>(in, out, (int)w, (int)h, S, wp, regions, stream);
+            case 4: return launch_tile<4>(in, out, (int)w, (int)h, S, wp, regions, stream);
+            case 5: return launch_tile<5>(in, out, (int)w, (int)h, S, wp, regions, stream);
             default: break;
         }
     }
     dim3 grid((unsigned)((w + 255) / 256), (unsigned)h);
-    hipLaunchKernelGGL(k_nlm_direct, grid, dim3(256), 0, stream, in, out, (int)w, (int)h, S, P, wp, (float)norm);
+    hipLaunchKernelGGL(k_nlm_direct, grid, dim3(256), 0, stream, in, out, (int)w, (int)h, S, P, wp, (float)norm, regions);
+    return hipGetLastError();
+}
+
+hipError_t launch_nlm(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, hipStream_t stream) {
+    return launch_nlm_regions(in, out, w, h, hp, search, patch, nullptr, stream);
+}
+
+size_t nlm_regions(size_t w, size_t h) { return ((w + NLM_REGION - 1) / NLM_REGION) * ((h + NLM_REGION - 1) / NLM_REGION); }
+
+hipError_t launch_nlm_spot_check(const float* in, const float* cand, size_t w, size_t h, NlmH hp, long search, size_t patch, float thr_db,
+                                 float* gap_db, uint8_t* flags, hipStream_t stream) {
+    if (w == 0 || h == 0) return hipSuccess;
+    double norm;
+    const NlmWeight wp = make_weight(hp, (int)patch, norm);
+    dim3 grid((unsigned)((w + NLM_REGION - 1) / NLM_REGION), (unsigned)((h + NLM_REGION - 1) / NLM_REGION));
+    hipLaunchKernelGGL(k_nlm_spot, grid, dim3(NLM_SPOT * NLM_SPOT), 0, stream, in, cand, (int)w, (int)h, (int)search, (int)patch, wp, (float)norm,
+                       thr_db, gap_db, flags);
     return hipGetLastError();
 }
 

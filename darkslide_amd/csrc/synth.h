@@ -65,7 +65,8 @@ __host__ __device__ inline double synth_u01(uint64_t seed, uint64_t idx, uint32_
     return (double)(splitmix64_mix(splitmix64_mix(z)) >> 11) * (1.0 / 9007199254740992.0);
 }
 
-// Host-side scene description.  bits: 12 or 14; iso: noise multiplier >= 1.
+// Host-side scene description.  bits: 12 or 14; iso: noise multiplier >= 1,
+// or 0 for the noise-free frame of the same scene.
 inline void synth_scene_init(SynthScene* s, uint64_t seed, int w, int h, int bits, float iso, const uint8_t pat[4]) {
     SplitMix64 r{seed ^ 0xA5A5A5A55A5A5A5Aull};
     s->seed = seed;
@@ -193,6 +194,10 @@ __host__ __device__ inline double synth_poisson(uint64_t seed, uint64_t idx, uin
 __host__ __device__ inline uint16_t synth_dn(const SynthScene& s, int x, int y) {
     int c = s.pat[((y & 1) << 1) | (x & 1)];
     double sig = (double)synth_value(s, x, y, c) * (double)(s.white - s.black);
+    if (s.gain <= 0.0f) {  // noise-free rendering
+        double dn = floor((double)s.black + sig + 0.5);
+        return (uint16_t)(dn > (double)s.white ? (double)s.white : dn);
+    }
     uint64_t idx = (uint64_t)y * (uint64_t)s.w + (uint64_t)x;
     uint32_t k = 0;
     double e = synth_poisson(s.seed, idx, k, sig / (double)s.gain);

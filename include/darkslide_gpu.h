@@ -148,9 +148,73 @@ int ds_prof_get(ds_ctx* ctx, ds_prof_entry* out, int cap);
 /* Seeded synthetic Bayer frame (gradients + polygons + sinusoids, linear in
  * [0.02, 0.85] of range, Poisson-Gaussian noise with gain 0.5*iso DN/e and
  * read sigma 4 DN).  bits 14: black 512, white 16383; bits 12: black 128,
- * white 4095.  raw_out: w*h u16.  The _dev form writes device memory. */
+ * white 4095.  iso 0 renders the same scene without noise.  raw_out: w*h
+ * u16.  The _dev form writes device memory. */
 int ds_synth_bayer_host(uint64_t seed, size_t w, size_t h, int bits, float iso, const uint8_t pattern[4], uint16_t* raw_out);
 int ds_synth_bayer_dev(ds_ctx* ctx, uint64_t seed, size_t w, size_t h, int bits, float iso, const uint8_t pattern[4], uint16_t* raw_out);
+
+/* ---- learned luminance denoiser ---------------------------------------- */
+
+/* No counterpart in the reference: a residual CNN (8 conv3x3 layers, 32
+ * channels, ReLU; BF16 weights and activations, FP32 accumulation) that
+ * stands in for the NLM call of denoise (enhance.rs:84), trained on the
+ * GPU on pairs (noisy luminance, NLM output) that the pipeline generates
+ * from synthetic frames.  Its input is x = clamp((L - mean5x5(L)) / s, -64,
+ * 64) with s = max(estimate_noise(L), 1e-4) (enhance.rs:82); its output y
+ * gives the luminance L + s (y - x).  Parameters: ds_cnn_param_count()
+ * FP32 values, laid out W_in[32][9], b_in[32], six times W[co 32][tap 9]
+ * [ci 32], b[32], then W_out[tap 9][ci 32], b_out (tap = 3 (dy+1) + dx+1). */
+typedef struct ds_cnn ds_cnn;
+
+size_t ds_cnn_param_count(void);
+/* He-normal initial weights from `seed`; Adam state zero. */
+int ds_cnn_create(ds_ctx* ctx, uint64_t seed, ds_cnn** out);
+void ds_cnn_destroy(ds_cnn* net);
+int ds_cnn_get_params(ds_cnn* net, float* params, size_t n);
+/* Replaces the weights and resets the Adam state and step count. */
+int ds_cnn_set_params(ds_cnn* net, const float* params, size_t n);
+long ds_cnn_step(const ds_cnn* net);
+/* Training checkpoint: weights, Adam moments and step ("DSCNN001" file). */
+int ds_cnn_save(ds_cnn* net, const char* path);
+int ds_cnn_load(ds_cnn* net, const char* path);
+
+/* Host batches of n_img images h x w (network input x, target t, output y).
+ * ds_cnn_loss_grad: loss = mean over pixels at least `margin` from the
+ * image border of (y - t)^2, and its gradient (ds_cnn_param_count floats). */
+int ds_cnn_forward(ds_cnn* net, const float* x, size_t n_img, size_t h, size_t w, float* y_out);
+int ds_cnn_loss_grad(ds_cnn* net, const float* x, const float* t, size_t n_img, size_t h, size_t w, size_t margin, double* loss,
+                     float* grad);
+/* One Adam step (lr, beta 0.9 / 0.999, eps 1e-8) on a device batch; with
+ * loss_out set, waits for and returns the batch loss before the step. */
+int ds_cnn_train_step_dev(ds_cnn* net, const float* x, const float* t, size_t n_img, size_t h, size_t w, size_t margin, float lr,
+                          double* loss_out);
+/* Training pairs: the synthetic frame `seed` (frame_w x frame_h RGGB, 14-bit,
+ * noise level iso) through RCD demosaic, Oklab L, estimate_noise and NLM
+ * (h = 0.6 sigma, search / patch radii); n_crops seeded crop x crop crops of
+ * x and of the target t = x + (NLM(L) - L) / s, into device buffers. */
+int ds_cnn_make_pairs_dev(ds_ctx* ctx, uint64_t seed, size_t frame_w, size_t frame_h, float iso, long search, size_t patch, size_t crop,
+                          size_t n_crops, float* x_out, float* t_out);
+/* The learned luminance of one plane (host buffers): L + s (y - x). */
+int ds_cnn_denoise_plane(ds_cnn* net, const float* L, size_t w, size_t h, float* lf_out);
+
+/* denoise (enhance.rs:74-92) with the learned luminance in place of NLM.
+ * Every 256 x 256 region is spot-checked against NLM on a 16 x 16 lattice
+ * of pixels: deviation = 10 log10(1 + mean (learned - NLM)^2 / mean var),
+ * var = sigma^2 sum w^2 / (sum w)^2 being the white-noise variance NLM
+ * leaves at that pixel.  Regions whose deviation exceeds threshold_db (all
+ * regions when threshold_db <= 0) are recomputed with NLM.  stats may be
+ * NULL; when set, the call waits for the result. */
+typedef struct ds_learned_stats {
+    size_t regions;
+    size_t fallback;
+    float mean_gap_db;
+    float max_gap_db;
+} ds_learned_stats;
+
+int ds_denoise_learned(ds_ctx* ctx, ds_cnn* net, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch,
+                       float threshold_db, float* rgb_out, ds_learned_stats* stats);
+int ds_denoise_learned_dev(ds_ctx* ctx, ds_cnn* net, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch,
+                           float threshold_db, float* rgb_out, ds_learned_stats* stats);
 
 #ifdef __cplusplus
 }

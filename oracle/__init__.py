@@ -211,3 +211,62 @@ def box_radii_for_gauss(sigma):
     radii = (ctypes.c_size_t * 3)()
     lib().or_box_radii_for_gauss(ctypes.c_float(sigma), radii)
     return list(radii)
+
+
+# ---- learned denoiser (cnn_oracle.cpp): FP64 forward / loss / gradient
+
+
+def cnn_param_count():
+    f = lib().or_cnn_param_count
+    f.restype = ctypes.c_size_t
+    return int(f())
+
+
+def bf16_round(a):
+    """Round float64 values to the nearest bfloat16 (ties to even)."""
+    f = lib().or_cnn_bf16_round
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_double]
+    return np.vectorize(lambda v: f(float(v)))(np.asarray(a, dtype=np.float64))
+
+
+def cnn_forward(params, x, bf16=False):
+    """x: (n, H, W) float32 -> out (n, H, W) float64."""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n, H, W = x.shape
+    out = np.empty((n, H, W), np.float64)
+    f = lib().or_cnn_forward_f64
+    f.restype = None
+    f(_p(params), _p(x), n, H, W, int(bool(bf16)), _p(out))
+    return out
+
+
+def cnn_loss_grad(params, x, tgt, margin, bf16=False, grad=True):
+    """Mean squared error inside the margin and its gradient (float64)."""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    tgt = np.ascontiguousarray(tgt, dtype=np.float32)
+    n, H, W = x.shape
+    g = np.empty(cnn_param_count(), np.float64) if grad else None
+    f = lib().or_cnn_loss_grad_f64
+    f.restype = ctypes.c_double
+    loss = f(_p(params), _p(x), _p(tgt), n, H, W, int(margin), int(bool(bf16)), _p(g))
+    return (loss, g) if grad else loss
+
+
+def cnn_adam(w, g, m, v, step, lr):
+    """One Adam step in place on float64 arrays."""
+    f = lib().or_cnn_adam_f64
+    f.restype = None
+    f(_p(w), _p(g), _p(m), _p(v), ctypes.c_size_t(w.size), int(step), ctypes.c_double(lr))
+
+
+def cnn_input(L, sigma):
+    L = np.ascontiguousarray(L, dtype=np.float32)
+    h, w = L.shape
+    x = np.empty((h, w), np.float64)
+    f = lib().or_cnn_input_f64
+    f.restype = None
+    f(_p(L), w, h, ctypes.c_double(sigma), _p(x))
+    return x
