@@ -15,6 +15,7 @@ Prints one JSON line on stdout.
 """
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -68,6 +69,98 @@ def cpu_baseline(size, reps):
     }
 
 
+def _psnr(a, ref):
+    import numpy as np
+    d = a.astype(np.float64) - ref.astype(np.float64)
+    mse = float(np.mean(d * d))
+    return 10.0 * math.log10(1.0 / max(mse, 1e-30))
+
+
+def learned_leg(ctx, args, raw, rgb, den, out, w, h, plain_ms):
+    """Learned pipeline: RCD -> CNN luminance with the per-region NLM spot
+    check and fallback -> guided chroma filters -> RL 1.0 x 20, timed like the
+    plain one, plus its PSNR against the noise-free frame next to the plain
+    path's (both after denoise)."""
+    import ctypes
+    import numpy as np
+    import darkslide_amd as D
+    L = D.lib()
+    if not os.path.exists(args.checkpoint):
+        return {"error": f"no trained checkpoint at {os.path.relpath(args.checkpoint, ROOT)}"}
+    net = D.CNN(ctx, seed=1)
+    net.load(args.checkpoint)
+    pat = D._pat(D.RGGB)
+    white = D._f3((16383.0,) * 3)
+    ones = D._f3((1.0, 1.0, 1.0))
+    res = {"checkpoint": os.path.relpath(args.checkpoint, ROOT), "train_steps": net.step, "threshold_db": args.threshold_db,
+           "sizes": {}}
+    for spec in args.learned_sizes.split(","):
+        fw, fh = (int(v) for v in spec.split("x"))
+        n = fw * fh
+        if (fw, fh) == (w, h):
+            b_raw, b_rgb, b_den, b_out = raw, rgb, den, out
+            own = []
+        else:
+            b_raw, b_rgb, b_den, b_out = ctx.alloc(n * 2), ctx.alloc(n * 12), ctx.alloc(n * 12), ctx.alloc(n * 12)
+            own = [b_raw, b_rgb, b_den, b_out]
+            ctx._check(L.ds_synth_bayer_dev(ctx.handle, 0x5EED0002, fw, fh, 14, 1.0, pat, b_raw.ptr))
+        st = D.LearnedStats()
+
+        def lstep(stats=None):
+            ctx._check(L.ds_demosaic_u16_dev(ctx.handle, b_raw.ptr, fw, fh, fw, pat, 512.0, white, ones, D.ALGO_RCD, b_rgb.ptr, None))
+            ctx._check(L.ds_denoise_learned_dev(ctx.handle, net.handle, b_rgb.ptr, fw, fh, 0.0, 10, 3, float(args.threshold_db), b_den.ptr,
+                                                stats))
+            ctx._check(L.ds_raw_details_dev(ctx.handle, b_den.ptr, fw, fh, 1.0, 20, b_out.ptr))
+
+        lstep(ctypes.byref(st))
+        ctx.sync()
+        for _ in range(max(args.warmup - 1, 0)):
+            lstep()
+        ctx.prof_enable(True)
+        ctx.prof_reset()
+        ctx.sync()
+        steps = args.steps
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lstep()
+        ctx.sync()
+        t1 = time.perf_counter()
+        prof = ctx.prof()
+        ctx.prof_enable(False)
+        ms = (t1 - t0) * 1e3 / steps
+        entry = {
+            "value": round(n / 1e6 / (ms * 1e-3), 2), "unit": "MP/s", "ms_per_step": round(ms, 3), "steps": steps,
+            "fallback_regions": int(st.fallback), "regions": int(st.regions),
+            "mean_spot_deviation_db": round(float(st.mean_gap_db), 4), "max_spot_deviation_db": round(float(st.max_gap_db), 4),
+            "stages_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
+        }
+        log(f"learned {fw}x{fh}: {ms:.3f} ms/step, {entry['value']} MP/s, fallback {st.fallback}/{st.regions}")
+        if (fw, fh) == (w, h):
+            entry["vs_plain_pipeline"] = round(plain_ms / ms, 3)
+            # quality against the noise-free frame, after denoise
+            clean_raw = ctx.alloc(n * 2)
+            ctx._check(L.ds_synth_bayer_dev(ctx.handle, 0x5EED0001, fw, fh, 14, 0.0, pat, clean_raw.ptr))
+            ctx._check(L.ds_demosaic_u16_dev(ctx.handle, clean_raw.ptr, fw, fh, fw, pat, 512.0, white, ones, D.ALGO_RCD, b_out.ptr, None))
+            clean = b_out.download((fh, fw, 3), np.float32)
+            clean_raw.free()
+            ctx._check(L.ds_demosaic_u16_dev(ctx.handle, b_raw.ptr, fw, fh, fw, pat, 512.0, white, ones, D.ALGO_RCD, b_rgb.ptr, None))
+            noisy = b_rgb.download((fh, fw, 3), np.float32)
+            ctx._check(L.ds_denoise_ex_dev(ctx.handle, b_rgb.ptr, fw, fh, 0.0, 10, 3, b_den.ptr))
+            plain = b_den.download((fh, fw, 3), np.float32)
+            lstep()
+            learned_rgb = b_den.download((fh, fw, 3), np.float32)
+            p_noisy, p_plain, p_learned = _psnr(noisy, clean), _psnr(plain, clean), _psnr(learned_rgb, clean)
+            entry.update({"psnr_noisy_db": round(p_noisy, 4), "psnr_plain_db": round(p_plain, 4), "psnr_learned_db": round(p_learned, 4),
+                          "psnr_gap_db": round(p_plain - p_learned, 4)})
+            log(f"PSNR vs noise-free: noisy {p_noisy:.3f}  plain {p_plain:.3f}  learned {p_learned:.3f} dB")
+            del clean, noisy, plain, learned_rgb
+        for b in own:
+            b.free()
+        res["sizes"][f"{fw}x{fh}"] = entry
+    net.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +173,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device copy of this size (known-size reference for HBM counters)")
+    ap.add_argument("--checkpoint", default=os.path.join(ROOT, "darkslide_amd", "weights", "cnn_l32x8.dsck"))
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--threshold-db", type=float, default=0.3)
+    ap.add_argument("--learned-sizes", default="6000x4000,8256x5504")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -166,6 +263,9 @@ def main():
         "algorithmic": f"{NLM_FLOP_PER_PX} flop/px x {n} px per launch",
         "dominant_stage": dominant,
     }
+    learned = None
+    if not args.no_learned:
+        learned = learned_leg(ctx, args, raw, rgb, den, out, w, h, ms_per_step)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -194,6 +294,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages_ms": stages,
+        "learned": learned,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

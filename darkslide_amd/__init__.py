@@ -19,7 +19,7 @@ import numpy as np
 
 __all__ = [
     "DarkslideError", "Context", "lib", "default_context", "RGGB", "ALGO_RCD", "ALGO_MHC",
-    "demosaic", "demosaic_u16", "estimate_noise", "nlm", "denoise", "raw_details", "synth_bayer", "CNN", "LearnedStats",
+    "demosaic", "demosaic_u16", "estimate_noise", "nlm", "denoise", "raw_details", "synth_bayer", "CNN", "LearnedStats", "Frame", "PipelineOpts", "DENOISE_NONE", "DENOISE_NLM", "DENOISE_LEARNED",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -41,6 +41,44 @@ class DarkslideError(RuntimeError):
 
 class ProfEntry(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("total_ms", ctypes.c_double), ("launches", ctypes.c_long)]
+
+
+DENOISE_NONE, DENOISE_NLM, DENOISE_LEARNED = 0, 1, 2
+
+
+class Frame(ctypes.Structure):
+    """ds_frame: one raw frame of a batch (device pointers)."""
+    _fields_ = [("raw", ctypes.c_void_p), ("w", ctypes.c_size_t), ("h", ctypes.c_size_t), ("stride", ctypes.c_size_t),
+                ("pattern", ctypes.c_uint8 * 4), ("black", ctypes.c_float), ("white", ctypes.c_float * 3),
+                ("mul", ctypes.c_float * 3), ("rgb_out", ctypes.c_void_p)]
+
+
+class PipelineOpts(ctypes.Structure):
+    _fields_ = [("algo", ctypes.c_int), ("denoise", ctypes.c_int), ("amount", ctypes.c_float), ("search", ctypes.c_long),
+                ("patch", ctypes.c_size_t), ("raw_details", ctypes.c_int), ("rl_sigma", ctypes.c_float), ("rl_iters", ctypes.c_size_t),
+                ("reference_gating", ctypes.c_int), ("net", ctypes.c_void_p), ("threshold_db", ctypes.c_float),
+                ("streams", ctypes.c_int), ("autotune", ctypes.c_int)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("frames", ctypes.c_size_t), ("branches", ctypes.c_size_t), ("graph_reused", ctypes.c_int),
+                ("capture_ms", ctypes.c_double), ("fallback_regions", ctypes.c_size_t), ("regions", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("rcd_variant", ctypes.c_int), ("mhc_variant", ctypes.c_int), ("rl_variant", ctypes.c_int),
+                ("rcd_ms", ctypes.c_float * 4), ("mhc_ms", ctypes.c_float * 4), ("rl_ms", ctypes.c_float * 4)]
+
+    def as_dict(self):
+        return {"rcd_variant": self.rcd_variant, "mhc_variant": self.mhc_variant, "rl_variant": self.rl_variant,
+                "rcd_ms": list(self.rcd_ms)[:3], "mhc_ms": list(self.mhc_ms)[:3], "rl_ms": list(self.rl_ms)[:3]}
+
+
+class WbStats(ctypes.Structure):
+    _fields_ = [("acc", ctypes.c_double * 3), ("n", ctypes.c_double), ("neutral", ctypes.c_float * 3), ("valid", ctypes.c_int)]
 
 
 class LearnedStats(ctypes.Structure):
@@ -108,6 +146,12 @@ _SIGS = {
     "ds_cnn_denoise_plane": [_P, _P, _SZ, _SZ, _P],
     "ds_denoise_learned": [_P, _P, _P, _SZ, _SZ, _F, _L, _SZ, _F, _P, _P],
     "ds_denoise_learned_dev": [_P, _P, _P, _SZ, _SZ, _F, _L, _SZ, _F, _P, _P],
+    "ds_pipeline_batch": [_P, _P, _SZ, _P, _P],
+    "ds_autotune": [_P, _SZ, _SZ, _P],
+    "ds_set_tuning": [_P, _SZ, _SZ, _P],
+    "ds_get_tuning": [_P, _SZ, _SZ, _P],
+    "ds_auto_wb_stats": [_P, _P, _SZ, _SZ, _P, _P],
+    "ds_auto_wb_stats_dev": [_P, _P, _SZ, _SZ, _P, _P],
 }
 _RESTYPES = {"ds_ctx_destroy": None, "ds_last_error": ctypes.c_char_p, "ds_version": ctypes.c_char_p,
              "ds_cnn_param_count": ctypes.c_size_t, "ds_cnn_destroy": None, "ds_cnn_step": ctypes.c_long}
@@ -313,6 +357,39 @@ class Context:
         out = np.empty_like(rgb)
         self._check(lib().ds_raw_details(self.handle, _ptr(rgb), w, h, float(sigma), int(iterations), _ptr(out)))
         return out
+
+    def pipeline_batch(self, frames, algo=ALGO_RCD, denoise=DENOISE_NLM, amount=0.0, search=10, patch=3, raw_details=True,
+                       rl_sigma=1.0, rl_iters=20, reference_gating=False, net=None, threshold_db=0.3, streams=0, autotune=False,
+                       info=True):
+        """Runs the batch graph over `frames` (a list of Frame); returns the batch info (or None with info=False)."""
+        arr = (Frame * len(frames))(*frames)
+        o = PipelineOpts(algo, denoise, amount, search, patch, int(raw_details), rl_sigma, rl_iters, int(reference_gating),
+                         net.handle.value if net is not None else None, threshold_db, streams, int(autotune))
+        bi = BatchInfo()
+        self._check(lib().ds_pipeline_batch(self.handle, arr, len(frames), ctypes.byref(o), ctypes.byref(bi) if info else None))
+        return bi.as_dict() if info else None
+
+    def autotune(self, w, h):
+        t = Tuning()
+        self._check(lib().ds_autotune(self.handle, w, h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, w, h, rcd=0, mhc=0, rl=0):
+        t = Tuning(rcd, mhc, rl)
+        self._check(lib().ds_set_tuning(self.handle, w, h, ctypes.byref(t)))
+
+    def tuning(self, w, h):
+        t = Tuning()
+        self._check(lib().ds_get_tuning(self.handle, w, h, ctypes.byref(t)))
+        return (t.rcd_variant, t.mhc_variant, t.rl_variant)
+
+    def auto_wb_stats(self, rgb, as_shot_neutral):
+        """pipeline/mod.rs:421-441 statistics in FP64 -> dict(acc, n, neutral, valid)."""
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        h, w, _ = rgb.shape
+        st = WbStats()
+        self._check(lib().ds_auto_wb_stats(self.handle, _ptr(rgb), w, h, _f3(as_shot_neutral), ctypes.byref(st)))
+        return {"acc": list(st.acc), "n": st.n, "neutral": list(st.neutral), "valid": st.valid}
 
     def denoise_learned(self, net, rgb, amount, search=10, patch=3, threshold_db=0.3):
         """denoise with the learned luminance and the per-region NLM fallback -> (rgb, stats)."""

@@ -2,6 +2,8 @@
 // device workspace, host<->device staging, stage timing and the entry
 // points that stand in for the reference's demosaic / enhance functions.
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -128,6 +130,62 @@ __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
     if (i < n) p[i] = v;
 }
 
+// number of flagged regions (one block)
+__global__ __launch_bounds__(256) void k_count_flags(const uint8_t* __restrict__ flags, int n, int* __restrict__ out) {
+    __shared__ int red[256];
+    int c = 0;
+    for (int i = threadIdx.x; i < n; i += 256) c += flags[i] ? 1 : 0;
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// Auto white-balance statistics (pipeline/mod.rs:421-437) in FP64: over
+// every third pixel, raw = p * n_as (f32), skip max > 0.9 or min < 0.002,
+// w = sqrt(max), acc[c] += ln(raw[c]) w, n += w.  Per-block partial sums
+// (acc0, acc1, acc2, n); k_wb_final adds them in block order.
+constexpr int WB_NT = 256;
+__global__ __launch_bounds__(WB_NT) void k_wb_partial(const float* __restrict__ rgb, size_t npx, float n0, float n1, float n2,
+                                                      double* __restrict__ part) {
+    __shared__ double red[4][WB_NT];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, wn = 0.0;
+    const size_t cnt = (npx + 2) / 3;
+    for (size_t k = (size_t)blockIdx.x * WB_NT + threadIdx.x; k < cnt; k += (size_t)gridDim.x * WB_NT) {
+        const float* p = rgb + 3 * (3 * k);
+        const float r0 = p[0] * n0, r1 = p[1] * n1, r2 = p[2] * n2;
+        const float m = fmaxf(fmaxf(r0, r1), r2);
+        const float lo = fminf(fminf(r0, r1), r2);
+        if (m > 0.9f || lo < 0.002f) continue;
+        const double wt = sqrt((double)m);
+        a0 += log((double)r0) * wt;
+        a1 += log((double)r1) * wt;
+        a2 += log((double)r2) * wt;
+        wn += wt;
+    }
+    red[0][threadIdx.x] = a0;
+    red[1][threadIdx.x] = a1;
+    red[2][threadIdx.x] = a2;
+    red[3][threadIdx.x] = wn;
+    __syncthreads();
+    for (int s = WB_NT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[(size_t)blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_wb_final(const double* __restrict__ part, int nblk, double* __restrict__ out) {
+    if (threadIdx.x >= 4) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[(size_t)b * 4 + threadIdx.x];
+    out[threadIdx.x] = s;
+}
+
 }  // namespace
 }  // namespace ds
 
@@ -143,7 +201,26 @@ struct Buf {
     size_t bytes = 0;
 };
 
-enum Slot { S_IN = 0, S_IN2, S_OUT, S_CLIP, S_WS, S_L, S_A, S_B, S_LF, S_SIGMA, S_NOISE, S_SCENE, S_X, S_T, S_REGIONS, S_COUNT };
+enum Slot {
+    S_IN = 0, S_IN2, S_OUT, S_CLIP, S_WS, S_L, S_A, S_B, S_LF, S_SIGMA, S_NOISE, S_SCENE, S_X, S_T, S_REGIONS, S_ACT,
+    S_PIPE_RGB, S_PIPE_DEN, S_COUNTS, S_STATS, S_COUNT
+};
+
+// Tile-shape choice for one frame shape (variant indices of the tunable kernels).
+struct Tuned {
+    int rcd = 0, mhc = 0, rl = 0;
+};
+
+// A captured batch: the key is the byte image of its frames, options and
+// tile choices; the executable graph is replayed while the key matches.
+struct GraphEntry {
+    std::vector<uint8_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int branches = 1;
+    size_t n_frames = 0;
+    unsigned long gen = 0;  // workspace generation the graph was captured against
+};
 
 struct Pending {
     std::string name;
@@ -165,6 +242,14 @@ struct ds_ctx {
     bool prof = false;
     std::vector<Pending> pending;
     std::map<std::string, std::pair<double, long>> totals;
+    // batched pipeline: parallel branches (each with its own stream and
+    // workspace), tile choices per frame shape, captured graphs
+    ds_ctx* parent = nullptr;
+    bool capturing = false;
+    std::vector<ds_ctx*> branches;
+    std::map<std::pair<size_t, size_t>, Tuned> tuning;
+    std::vector<GraphEntry> graphs;
+    unsigned long slot_gen = 0;  // bumped when a workspace buffer is replaced
 };
 
 namespace {
@@ -196,7 +281,16 @@ void* slot(ds_ctx* c, Slot s, size_t bytes, hipError_t* err) {
     *err = hipSuccess;
     Buf& b = c->slots[s];
     if (b.bytes >= bytes && b.p) return b.p;
+    if (c->capturing) {  // workspaces are reserved before a graph capture; nothing allocates inside one
+        *err = hipErrorStreamCaptureUnsupported;
+        return nullptr;
+    }
     if (b.p) {
+        // captured batches run on the root stream and may hold this buffer:
+        // wait for them and retire every graph captured before the change
+        ds_ctx* root = c->parent ? c->parent : c;
+        hipStreamSynchronize(root->stream);
+        root->slot_gen++;
         hipStreamSynchronize(c->stream);
         hipFree(b.p);
         b.p = nullptr;
@@ -276,6 +370,12 @@ int set_device(ds_ctx* c) {
 
 // ---- enqueue-only implementations (device pointers)
 
+Tuned tuned_for(const ds_ctx* c, size_t w, size_t h) {
+    const ds_ctx* root = c->parent ? c->parent : c;
+    auto it = root->tuning.find({w, h});
+    return it == root->tuning.end() ? Tuned{} : it->second;
+}
+
 int demosaic_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t* pat, float black,
                  const float* white, const float* mul, int algo, float* rgb, float* clip) {
     if (!valid_bayer(pat)) return fail(c, DS_ERR_INVALID, "pattern must be a Bayer 2x2 with a green checkerboard");
@@ -298,7 +398,10 @@ int demosaic_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, siz
     a.algo = algo;
     a.rgb = rgb;
     a.clip = clip;
-    return run(c, algo == DS_ALGO_RCD && w >= 16 && h >= 16 ? "demosaic_rcd" : "demosaic_mhc", [&] { return launch_demosaic(a, c->stream); });
+    const Tuned t = tuned_for(c, w, h);
+    const int variant = algo == DS_ALGO_RCD ? t.rcd : t.mhc;
+    return run(c, algo == DS_ALGO_RCD && w >= 16 && h >= 16 ? "demosaic_rcd" : "demosaic_mhc",
+               [&] { return launch_demosaic(a, c->stream, variant); });
 }
 
 int noise_dev(ds_ctx* c, const float* plane, size_t w, size_t h, float* sigma_dev) {
@@ -350,7 +453,8 @@ int raw_details_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float sigma
     if (n == 0) return DS_OK;
     if (!(sigma >= 0.0f) || sigma > 1000.0f) return fail(c, DS_ERR_INVALID, "sigma %g out of range", (double)sigma);
     SLOT(c, ws, float*, S_WS, rl_workspace_floats(w, h) * 4);
-    return run(c, "raw_details", [&] { return launch_raw_details(rgb, w, h, sigma, iters, ws, out, c->stream); });
+    const int variant = tuned_for(c, w, h).rl;
+    return run(c, "raw_details", [&] { return launch_raw_details(rgb, w, h, sigma, iters, ws, out, c->stream, variant); });
 }
 
 int synth_dev(ds_ctx* c, uint64_t seed, size_t w, size_t h, int bits, float iso, const uint8_t* pattern, uint16_t* raw_out) {
@@ -387,8 +491,6 @@ struct ds_cnn {
     uint16_t* tr_act = nullptr;  // 7 activations + 2 activation gradients, 32 bf16 per pixel each
     float* tr_f = nullptr;       // x | tgt | out | backward scratch
     size_t tr_fcap = 0;          // floats in tr_f
-    size_t inf_px = 0;         // inference buffers
-    uint16_t* inf_act = nullptr;  // 2 ping-pong activations
     float* params() { return state; }
     float* grad() { return state + CNN_NPARAM; }
     float* adam_m() { return state + 2 * CNN_NPARAM; }
@@ -427,18 +529,6 @@ int cnn_ensure_train(ds_cnn* net, int n, int H, int W) {
     HIPCHK(c, hipMalloc(&net->tr_f, cap * sizeof(float)), "hipMalloc (CNN training buffers)");
     net->tr_px = px;
     net->tr_fcap = cap;
-    return DS_OK;
-}
-
-int cnn_ensure_infer(ds_cnn* net, size_t npx) {
-    ds_ctx* c = net->ctx;
-    if (net->inf_px >= npx && net->inf_act) return DS_OK;
-    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    if (net->inf_act) hipFree(net->inf_act);
-    net->inf_act = nullptr;
-    net->inf_px = 0;
-    HIPCHK(c, hipMalloc(&net->inf_act, 2 * npx * CNN_ACT_BYTES), "hipMalloc (CNN activations)");
-    net->inf_px = npx;
     return DS_OK;
 }
 
@@ -510,18 +600,23 @@ int cnn_check_batch(ds_cnn* net, size_t n, size_t h, size_t w, size_t margin) {
 }
 
 // Learned luminance: LF = L + s (net(x) - x), x = cnn_input(L, s), s from sig.
-int cnn_plane_dev(ds_cnn* net, const float* L, size_t w, size_t h, const float* sig, float* lf) {
-    ds_ctx* c = net->ctx;
+// Runs on c's stream with c's workspace (activations included), so
+// contexts (and the branches of a batch) can share one network.
+int cnn_plane_dev(ds_ctx* c, ds_cnn* net, const float* L, size_t w, size_t h, const float* sig, float* lf) {
     const size_t n = w * h;
     if (int rc = cnn_check_shape(c, 1, h, w)) return rc;
-    if (int rc = cnn_ensure_infer(net, n)) return rc;
-    if (int rc = cnn_pack_now(net)) return rc;
+    if (!net->packed) {
+        int rc = run(c, "cnn_pack", [&] { return cnn_pack(net->params(), net->wpk, net->wpk + CNN_WBF16, c->stream); });
+        if (rc) return rc;
+        net->packed = true;
+    }
     SLOT(c, x, float*, S_X, n * 4);
     SLOT(c, o, float*, S_T, n * 4);
+    SLOT(c, act, uint16_t*, S_ACT, 2 * n * CNN_ACT_BYTES);
     int rc = run(c, "cnn_input", [&] { return cnn_input(L, (int)w, (int)h, sig, x, c->stream); });
     if (rc) return rc;
-    uint16_t* a = net->inf_act;
-    uint16_t* b = net->inf_act + n * 32;
+    uint16_t* a = act;
+    uint16_t* b = act + n * 32;
     uint16_t* acts[7] = {a, b, a, b, a, b, a};
     rc = run(c, "cnn_forward", [&] { return cnn_forward(net->params(), net->wpk, x, 1, (int)h, (int)w, acts, o, c->stream); });
     if (rc) return rc;
@@ -531,7 +626,7 @@ int cnn_plane_dev(ds_cnn* net, const float* L, size_t w, size_t h, const float* 
 // denoise with the learned luminance stand-in for NLM, the per-region NLM
 // spot check and fallback, then the reference's composite (enhance.rs:74-92).
 int denoise_learned_dev(ds_ctx* c, ds_cnn* net, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float thr_db,
-                        float* out, ds_learned_stats* stats) {
+                        float* out, ds_learned_stats* stats, int* count_dev = nullptr) {
     const size_t n = w * h;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (n == 0) return DS_OK;
@@ -543,19 +638,27 @@ int denoise_learned_dev(ds_ctx* c, ds_cnn* net, const float* rgb, size_t w, size
     SLOT(c, sig, float*, S_SIGMA, 64);
     SLOT(c, ws, float*, S_WS, 12 * n * 4);
     const size_t nreg = nlm_regions(w, h);
-    SLOT(c, regbuf, float*, S_REGIONS, nreg * 5 + 64);
+    SLOT(c, regbuf, float*, S_REGIONS, (2 * nreg + 64 + nlm_spot_scratch_floats(w, h)) * 4);
     float* gap = regbuf;
     uint8_t* flags = reinterpret_cast<uint8_t*>(regbuf + nreg);
+    float* spot = regbuf + 2 * nreg + 64;
     int rc = run(c, "oklab_fwd", [&] { return launch_oklab_fwd(rgb, n, c->ok, L, A, B, c->stream); });
     if (rc) return rc;
     if ((rc = noise_dev(c, L, w, h, sig))) return rc;
-    if ((rc = cnn_plane_dev(net, L, w, h, sig, LF))) return rc;
+    if ((rc = cnn_plane_dev(c, net, L, w, h, sig, LF))) return rc;
     const float k = amount / 100.0f;
     NlmH hp{0.0f, sig, 0.6f + 1.6f * k};
-    rc = run(c, "nlm_spot_check", [&] { return launch_nlm_spot_check(L, LF, w, h, hp, search, patch, thr_db, gap, flags, c->stream); });
+    rc = run(c, "nlm_spot_check", [&] { return launch_nlm_spot_check(L, LF, w, h, hp, search, patch, thr_db, gap, flags, spot, c->stream); });
     if (rc) return rc;
     rc = run(c, "nlm_fallback", [&] { return launch_nlm_regions(L, LF, w, h, hp, search, patch, flags, c->stream); });
     if (rc) return rc;
+    if (count_dev) {
+        rc = run(c, "count_fallback", [&] {
+            hipLaunchKernelGGL(k_count_flags, dim3(1), dim3(256), 0, c->stream, flags, (int)nreg, count_dev);
+            return hipGetLastError();
+        });
+        if (rc) return rc;
+    }
     const float m = 0.5f + 0.5f * k;
     float rf = std::fmax(((float)(w > h ? w : h) / 600.0f) * (1.0f + 2.0f * k), 2.0f);
     size_t r = (size_t)rf;
@@ -626,6 +729,528 @@ void cnn_init_params(uint64_t seed, std::vector<float>& p) {
     for (size_t i = 0; i < 9 * 32; i++) p[CNN_OFF_OUT + i] = (float)(normal() * 0.1 * std::sqrt(2.0 / 288.0));
 }
 
+// ---------------------------------------------------------------------------
+// tile autotuner
+
+// Median of `reps` timed runs of f (after one untimed run), in ms.
+template <class F>
+float time_ms(ds_ctx* c, int reps, F&& f) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return -1.0f;
+    if (hipEventCreate(&b) != hipSuccess) {
+        hipEventDestroy(a);
+        return -1.0f;
+    }
+    std::vector<float> t;
+    bool ok = f() == hipSuccess;
+    for (int i = 0; ok && i < reps; i++) {
+        ok = hipEventRecord(a, c->stream) == hipSuccess && f() == hipSuccess && hipEventRecord(b, c->stream) == hipSuccess &&
+             hipEventSynchronize(b) == hipSuccess;
+        float ms = 0.0f;
+        if (ok) ok = hipEventElapsedTime(&ms, a, b) == hipSuccess;
+        t.push_back(ms);
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (!ok) return -1.0f;
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+}
+
+// Times every tile variant of RCD, MHC and the RL iteration on a synthetic
+// frame of this shape and keeps the fastest for the shape.
+int autotune_shape(ds_ctx* c, size_t w, size_t h, ds_tuning* out) {
+    const size_t n = w * h;
+    if (n == 0 || w < 16 || h < 16) return fail(c, DS_ERR_INVALID, "autotune needs a frame of at least 16 x 16");
+    if (c->capturing) return fail(c, DS_ERR_INVALID, "autotune inside a capture");
+    static const uint8_t rggb[4] = {0, 1, 1, 2};
+    const float white[3] = {16383.0f, 16383.0f, 16383.0f};
+    const float ones[3] = {1.0f, 1.0f, 1.0f};
+    uint16_t* raw = nullptr;
+    float* rgb = nullptr;
+    float* ws = nullptr;
+    float* out_rgb = nullptr;
+    hipError_t e = hipMalloc(&raw, n * 2);
+    if (e == hipSuccess) e = hipMalloc(&rgb, n * 12);
+    if (e == hipSuccess) e = hipMalloc(&out_rgb, n * 12);
+    if (e == hipSuccess) e = hipMalloc(&ws, rl_workspace_floats(w, h) * 4);
+    auto release = [&] {
+        hipStreamSynchronize(c->stream);
+        if (raw) hipFree(raw);
+        if (rgb) hipFree(rgb);
+        if (out_rgb) hipFree(out_rgb);
+        if (ws) hipFree(ws);
+    };
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(c, e, "hipMalloc (autotune)");
+    }
+    int rc = synth_dev(c, 0x70E0ull, w, h, 14, 4.0f, rggb, raw);
+    if (rc) {
+        release();
+        return rc;
+    }
+    ds_tuning t{};
+    DemosaicArgs a{};
+    a.raw_u16 = raw;
+    a.stride = w;
+    a.w = w;
+    a.h = h;
+    std::memcpy(a.pattern, rggb, 4);
+    a.black = 512.0f;
+    for (int k = 0; k < 3; k++) {
+        a.white[k] = white[k];
+        a.mul[k] = ones[k];
+    }
+    a.rgb = rgb;
+    a.clip = nullptr;
+    float best[3] = {1e30f, 1e30f, 1e30f};
+    for (int algo = 0; algo < 2; algo++) {
+        a.algo = algo == 0 ? DS_ALGO_RCD : DS_ALGO_MHC;
+        for (int v = 0; v < demosaic_variants(a.algo) && v < 4; v++) {
+            const float ms = time_ms(c, 3, [&] { return launch_demosaic(a, c->stream, v); });
+            if (ms < 0.0f) {
+                release();
+                return fail(c, DS_ERR_LAUNCH, "autotune: demosaic variant %d failed", v);
+            }
+            (algo == 0 ? t.rcd_ms : t.mhc_ms)[v] = ms;
+            if (ms < best[algo]) {
+                best[algo] = ms;
+                (algo == 0 ? t.rcd_variant : t.mhc_variant) = v;
+            }
+        }
+    }
+    for (int v = 0; v < RL_VARIANTS && v < 4; v++) {
+        const float ms = time_ms(c, 3, [&] { return launch_raw_details(rgb, w, h, 1.0f, 4, ws, out_rgb, c->stream, v); });
+        if (ms < 0.0f) {
+            release();
+            return fail(c, DS_ERR_LAUNCH, "autotune: RL variant %d failed", v);
+        }
+        t.rl_ms[v] = ms;
+        if (ms < best[2]) {
+            best[2] = ms;
+            t.rl_variant = v;
+        }
+    }
+    release();
+    c->tuning[{w, h}] = Tuned{t.rcd_variant, t.mhc_variant, t.rl_variant};
+    if (out) *out = t;
+    return DS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched pipeline
+
+struct FramePlan {
+    size_t idx;
+    bool denoise, learned, rl;
+    float amount;
+    float rl_sigma;
+    size_t rl_iters;
+};
+
+FramePlan plan_frame(const ds_frame& f, size_t idx, const ds_pipeline_opts& o) {
+    FramePlan p{};
+    p.idx = idx;
+    p.denoise = o.denoise != DS_DENOISE_NONE;
+    p.learned = o.denoise == DS_DENOISE_LEARNED;
+    p.amount = o.amount;
+    p.rl = o.raw_details != 0;
+    p.rl_sigma = o.rl_sigma;
+    p.rl_iters = o.rl_iters;
+    if (o.reference_gating) {
+        // Engine::base_for (pipeline/mod.rs:229-252): denoise(img, amount.max(1.0));
+        // raw_details(img, 0.75, 8) only at full quality with a long edge > 3200
+        p.amount = std::fmax(o.amount, 1.0f);
+        p.rl = p.rl && (f.w > f.h ? f.w : f.h) > 3200;
+        p.rl_sigma = 0.75f;
+        p.rl_iters = 8;
+    }
+    return p;
+}
+
+// Workspace bytes per slot that one frame's stages ask for (mirrors their SLOT calls).
+void frame_needs(const ds_frame& f, const FramePlan& p, size_t need[S_COUNT]) {
+    const size_t n = f.w * f.h;
+    auto up = [&](Slot s, size_t b) { need[s] = std::max(need[s], b); };
+    if (p.denoise || p.rl) up(S_PIPE_RGB, n * 12);
+    if (p.denoise && p.rl) up(S_PIPE_DEN, n * 12);
+    if (p.denoise) {
+        up(S_L, n * 4);
+        up(S_A, n * 4);
+        up(S_B, n * 4);
+        up(S_LF, n * 4);
+        up(S_SIGMA, 64);
+        up(S_WS, 12 * n * 4);
+        const size_t cnt = noise_sample_count(f.w, f.h);
+        up(S_NOISE, (cnt ? cnt : 1) * sizeof(uint32_t));
+        if (p.learned) {
+            const size_t nreg = nlm_regions(f.w, f.h);
+            up(S_REGIONS, (2 * nreg + 64 + nlm_spot_scratch_floats(f.w, f.h)) * 4);
+            up(S_X, n * 4);
+            up(S_T, n * 4);
+            up(S_ACT, 2 * n * CNN_ACT_BYTES);
+        }
+    }
+    if (p.rl) up(S_WS, rl_workspace_floats(f.w, f.h) * 4);
+}
+
+int ensure_branches(ds_ctx* c, int k) {
+    while ((int)c->branches.size() < k) {
+        ds_ctx* b = new ds_ctx();
+        b->device = c->device;
+        b->ok = c->ok;
+        b->parent = c;
+        hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete b;
+            return hip_fail(c, e, "hipStreamCreate (batch branch)");
+        }
+        c->branches.push_back(b);
+    }
+    return DS_OK;
+}
+
+void destroy_graph(GraphEntry& g) {
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.graph) hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+}
+
+// Graphs captured against workspace buffers that have since been replaced.
+void retire_stale_graphs(ds_ctx* c) {
+    std::vector<GraphEntry> keep;
+    for (auto& g : c->graphs) {
+        if (g.gen == c->slot_gen)
+            keep.push_back(g);
+        else
+            destroy_graph(g);
+    }
+    c->graphs.swap(keep);
+}
+
+// One frame's stages on branch context b.
+int enqueue_frame(ds_ctx* b, const ds_frame& f, const FramePlan& p, const ds_pipeline_opts& o, int* count_dev) {
+    const bool direct = !p.denoise && !p.rl;
+    float* rgb = f.rgb_out;
+    if (!direct) {
+        SLOT(b, t, float*, S_PIPE_RGB, f.w * f.h * 12);
+        rgb = t;
+    }
+    int rc = demosaic_dev(b, nullptr, f.raw, f.w, f.h, f.stride, f.pattern, f.black, f.white, f.mul, o.algo, rgb, nullptr);
+    if (rc) return rc;
+    const float* cur = rgb;
+    if (p.denoise) {
+        float* dst = f.rgb_out;
+        if (p.rl) {
+            SLOT(b, t, float*, S_PIPE_DEN, f.w * f.h * 12);
+            dst = t;
+        }
+        rc = p.learned ? denoise_learned_dev(b, o.net, rgb, f.w, f.h, p.amount, o.search, o.patch, o.threshold_db, dst, nullptr, count_dev)
+                       : denoise_dev(b, rgb, f.w, f.h, p.amount, o.search, o.patch, dst);
+        if (rc) return rc;
+        cur = dst;
+    }
+    if (p.rl) rc = raw_details_dev(b, cur, f.w, f.h, p.rl_sigma, p.rl_iters, f.rgb_out);
+    return rc;
+}
+
+int validate_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_opts& o) {
+    if (!frames && n) return fail(c, DS_ERR_INVALID, "frames is null");
+    if (o.algo != DS_ALGO_RCD && o.algo != DS_ALGO_MHC) return fail(c, DS_ERR_INVALID, "unknown demosaic algorithm %d", o.algo);
+    if (o.denoise < DS_DENOISE_NONE || o.denoise > DS_DENOISE_LEARNED) return fail(c, DS_ERR_INVALID, "unknown denoiser %d", o.denoise);
+    if (o.denoise == DS_DENOISE_LEARNED && (!o.net || o.net->ctx != c))
+        return fail(c, DS_ERR_INVALID, "learned denoise needs a network of this context");
+    if (o.denoise != DS_DENOISE_NONE && (o.search < 0 || o.search > 1024 || o.patch > 64))
+        return fail(c, DS_ERR_INVALID, "search %ld / patch %zu out of range", o.search, o.patch);
+    if (o.raw_details && (!(o.rl_sigma >= 0.0f) || o.rl_sigma > 1000.0f)) return fail(c, DS_ERR_INVALID, "RL sigma out of range");
+    if (o.streams < 0 || o.streams > DS_MAX_BRANCHES) return fail(c, DS_ERR_INVALID, "streams %d out of range", o.streams);
+    for (size_t i = 0; i < n; i++) {
+        const ds_frame& f = frames[i];
+        if (int rc = check_dims(c, f.w, f.h)) return rc;
+        if (!f.raw || !f.rgb_out || f.w == 0 || f.h == 0) return fail(c, DS_ERR_INVALID, "frame %zu: null buffer or empty", i);
+        if (f.stride < f.w) return fail(c, DS_ERR_INVALID, "frame %zu: stride < width", i);
+        if (!valid_bayer(f.pattern)) return fail(c, DS_ERR_INVALID, "frame %zu: not a Bayer pattern", i);
+    }
+    return DS_OK;
+}
+
+// Byte image of everything the captured graph depends on.
+std::vector<uint8_t> batch_key(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_opts& o, int k) {
+    std::vector<uint8_t> key;
+    auto put = [&](const void* p, size_t b) {
+        const uint8_t* q = static_cast<const uint8_t*>(p);
+        key.insert(key.end(), q, q + b);
+    };
+    put(&k, sizeof k);
+    put(&n, sizeof n);
+    put(&o.algo, sizeof o.algo);
+    put(&o.denoise, sizeof o.denoise);
+    put(&o.amount, sizeof o.amount);
+    put(&o.search, sizeof o.search);
+    put(&o.patch, sizeof o.patch);
+    put(&o.raw_details, sizeof o.raw_details);
+    put(&o.rl_sigma, sizeof o.rl_sigma);
+    put(&o.rl_iters, sizeof o.rl_iters);
+    put(&o.reference_gating, sizeof o.reference_gating);
+    put(&o.net, sizeof o.net);
+    put(&o.threshold_db, sizeof o.threshold_db);
+    for (size_t i = 0; i < n; i++) {
+        const ds_frame& f = frames[i];
+        put(&f.raw, sizeof f.raw);
+        put(&f.w, sizeof f.w);
+        put(&f.h, sizeof f.h);
+        put(&f.stride, sizeof f.stride);
+        put(f.pattern, 4);
+        put(&f.black, sizeof f.black);
+        put(f.white, sizeof f.white);
+        put(f.mul, sizeof f.mul);
+        put(&f.rgb_out, sizeof f.rgb_out);
+        const Tuned t = tuned_for(c, f.w, f.h);
+        put(&t, sizeof t);
+    }
+    return key;
+}
+
+// Captures the whole batch into one graph: frames are spread over k branches
+// (longest-processing-time order by pixel count), each branch with its own
+// stream and workspace, forked from and joined back into the context stream.
+int capture_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_opts& o, int k, int* counts, GraphEntry& g) {
+    if (int rc = ensure_branches(c, k)) return rc;
+    // assignment
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return frames[a].w * frames[a].h > frames[b].w * frames[b].h; });
+    std::vector<std::vector<size_t>> assign(k);
+    std::vector<double> load(k, 0.0);
+    for (size_t i : order) {
+        int best = 0;
+        for (int j = 1; j < k; j++)
+            if (load[j] < load[best]) best = j;
+        assign[best].push_back(i);
+        load[best] += (double)(frames[i].w * frames[i].h);
+    }
+    // reserve every workspace the captured stages will ask for
+    for (int j = 0; j < k; j++) {
+        size_t need[S_COUNT] = {};
+        for (size_t i : assign[j]) frame_needs(frames[i], plan_frame(frames[i], i, o), need);
+        for (int s = 0; s < S_COUNT; s++) {
+            if (!need[s]) continue;
+            hipError_t e;
+            if (!slot(c->branches[j], (Slot)s, need[s], &e)) return hip_fail(c, e, "batch workspace allocation");
+        }
+    }
+    retire_stale_graphs(c);
+    if (o.denoise == DS_DENOISE_LEARNED && !o.net->packed) {
+        int rc = run(c, "cnn_pack", [&] { return cnn_pack(o.net->params(), o.net->wpk, o.net->wpk + CNN_WBF16, c->stream); });
+        if (rc) return rc;
+        o.net->packed = true;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    std::vector<hipEvent_t> ev(k + 1, nullptr);
+    for (auto& e : ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    auto cleanup = [&] {
+        for (auto& e : ev)
+            if (e) hipEventDestroy(e);
+        c->capturing = false;
+        for (ds_ctx* b : c->branches) b->capturing = false;
+    };
+    const bool prof = c->prof;
+    c->prof = false;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+        c->prof = prof;
+        cleanup();
+        return hip_fail(c, e, "hipStreamBeginCapture");
+    }
+    c->capturing = true;
+    int rc = DS_OK;
+    if (hipEventRecord(ev[k], c->stream) != hipSuccess) rc = fail(c, DS_ERR_DEVICE, "fork event");
+    for (int j = 0; j < k && !rc; j++) {
+        ds_ctx* b = c->branches[j];
+        b->capturing = true;
+        if (hipStreamWaitEvent(b->stream, ev[k], 0) != hipSuccess) {
+            rc = fail(c, DS_ERR_DEVICE, "fork wait");
+            break;
+        }
+        for (size_t i : assign[j]) {
+            const FramePlan p = plan_frame(frames[i], i, o);
+            rc = enqueue_frame(b, frames[i], p, o, p.learned ? counts + i : nullptr);
+            if (rc) {
+                c->err = "frame " + std::to_string(i) + ": " + b->err;
+                break;
+            }
+        }
+        if (!rc && hipEventRecord(ev[j], b->stream) != hipSuccess) rc = fail(c, DS_ERR_DEVICE, "join event");
+        if (!rc && hipStreamWaitEvent(c->stream, ev[j], 0) != hipSuccess) rc = fail(c, DS_ERR_DEVICE, "join wait");
+    }
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(c->stream, &graph);
+    c->prof = prof;
+    cleanup();
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(graph);
+        return hip_fail(c, e, "hipGraphInstantiate");
+    }
+    g.graph = graph;
+    g.exec = exec;
+    g.branches = k;
+    g.n_frames = n;
+    g.gen = c->slot_gen;
+    return DS_OK;
+}
+
+constexpr size_t kMaxGraphs = 4;
+
+int pipeline_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_opts& o, ds_batch_info* info) {
+    if (info) std::memset(info, 0, sizeof(*info));
+    if (int rc = validate_batch(c, frames, n, o)) return rc;
+    if (n == 0) return DS_OK;
+    if (o.autotune)
+        for (size_t i = 0; i < n; i++)
+            if (!c->tuning.count({frames[i].w, frames[i].h}) && frames[i].w >= 16 && frames[i].h >= 16)
+                if (int rc = autotune_shape(c, frames[i].w, frames[i].h, nullptr)) return rc;
+    SLOT(c, counts, int*, S_COUNTS, std::max<size_t>(n, 1024) * sizeof(int));
+    retire_stale_graphs(c);
+    std::vector<int> candidates;
+    if (o.streams > 0)
+        candidates.push_back(o.streams);
+    else if (o.autotune)
+        candidates = {1, 2, 4};
+    else
+        candidates.push_back(2);
+    for (int& k : candidates) k = (int)std::min<size_t>((size_t)k, n);
+    std::sort(candidates.begin(), candidates.end());
+    candidates.erase(std::unique(candidates.begin(), candidates.end()), candidates.end());
+    // a cached graph for this exact batch?
+    GraphEntry* hit = nullptr;
+    for (int k : candidates) {
+        const auto key = batch_key(c, frames, n, o, k);
+        for (auto& g : c->graphs)
+            if (g.key == key) hit = &g;
+        if (hit) break;
+    }
+    double capture_ms = 0.0;
+    if (!hit && candidates.size() > 1) {
+        // branch-count autotune: capture each candidate, keep the fastest
+        GraphEntry best;
+        float best_ms = 1e30f;
+        for (int k : candidates) {
+            GraphEntry g;
+            auto t0 = std::chrono::steady_clock::now();
+            if (int rc = capture_batch(c, frames, n, o, k, counts, g)) {
+                destroy_graph(best);
+                return rc;
+            }
+            capture_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            const float ms = time_ms(c, 1, [&] { return hipGraphLaunch(g.exec, c->stream); });
+            if (ms >= 0.0f && ms < best_ms) {
+                destroy_graph(best);
+                best = g;
+                best.key = batch_key(c, frames, n, o, k);
+                best_ms = ms;
+            } else {
+                destroy_graph(g);
+            }
+        }
+        if (!best.exec) return fail(c, DS_ERR_LAUNCH, "batch graph failed to run");
+        if (best.gen != c->slot_gen) {  // a later candidate's reservation replaced a buffer: capture the winner again
+            const int k = best.branches;
+            destroy_graph(best);
+            if (int rc = capture_batch(c, frames, n, o, k, counts, best)) return rc;
+            best.key = batch_key(c, frames, n, o, k);
+        }
+        c->graphs.push_back(best);
+        hit = &c->graphs.back();
+    } else if (!hit) {
+        GraphEntry g;
+        auto t0 = std::chrono::steady_clock::now();
+        if (int rc = capture_batch(c, frames, n, o, candidates[0], counts, g)) return rc;
+        capture_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        g.key = batch_key(c, frames, n, o, candidates[0]);
+        c->graphs.push_back(g);
+        hit = &c->graphs.back();
+    } else if (info) {
+        info->graph_reused = 1;
+    }
+    if (c->graphs.size() > kMaxGraphs) {
+        // keep the newest; `hit` is the last element or was found among the newest
+        const size_t drop = c->graphs.size() - kMaxGraphs;
+        const GraphEntry keep = *hit;
+        for (size_t i = 0; i < drop; i++)
+            if (c->graphs[i].exec != keep.exec) destroy_graph(c->graphs[i]);
+        c->graphs.erase(c->graphs.begin(), c->graphs.begin() + (long)drop);
+        hit = nullptr;
+        for (auto& g : c->graphs)
+            if (g.exec == keep.exec) hit = &g;
+        if (!hit) {
+            c->graphs.push_back(keep);
+            hit = &c->graphs.back();
+        }
+    }
+    if (o.denoise == DS_DENOISE_LEARNED && !o.net->packed) {
+        int rc = run(c, "cnn_pack", [&] { return cnn_pack(o.net->params(), o.net->wpk, o.net->wpk + CNN_WBF16, c->stream); });
+        if (rc) return rc;
+        o.net->packed = true;
+    }
+    hipGraphExec_t exec = hit->exec;
+    int rc = run(c, "pipeline_batch", [&] { return hipGraphLaunch(exec, c->stream); });
+    if (rc) return rc;
+    if (!info) return DS_OK;
+    info->frames = n;
+    info->branches = (size_t)hit->branches;
+    info->capture_ms = capture_ms;
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (o.denoise == DS_DENOISE_LEARNED) {
+        std::vector<int> cnt(n);
+        HIPCHK(c, hipMemcpy(cnt.data(), counts, n * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        for (size_t i = 0; i < n; i++) {
+            info->fallback_regions += (size_t)cnt[i];
+            info->regions += nlm_regions(frames[i].w, frames[i].h);
+        }
+    }
+    return DS_OK;
+}
+
+int wb_stats_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, const float* n_as, ds_wb_stats* out) {
+    const size_t npx = w * h;
+    const size_t cnt = (npx + 2) / 3;
+    const int nblk = (int)std::min<size_t>(1024, (cnt + WB_NT - 1) / WB_NT);
+    SLOT(c, part, double*, S_STATS, ((size_t)(nblk > 0 ? nblk : 1) * 4 + 4) * sizeof(double));
+    double* res = part + (size_t)(nblk > 0 ? nblk : 1) * 4;
+    double host[4] = {0.0, 0.0, 0.0, 0.0};
+    if (npx) {
+        int rc = run(c, "wb_stats", [&] {
+            hipLaunchKernelGGL(k_wb_partial, dim3(nblk), dim3(WB_NT), 0, c->stream, rgb, npx, n_as[0], n_as[1], n_as[2], part);
+            hipLaunchKernelGGL(k_wb_final, dim3(1), dim3(64), 0, c->stream, part, nblk, res);
+            return hipGetLastError();
+        });
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpyAsync(host, res, sizeof host, hipMemcpyDeviceToHost, c->stream), "D2H");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    }
+    for (int k = 0; k < 3; k++) out->acc[k] = host[k];
+    out->n = host[3];
+    out->valid = host[3] > 0.0 ? 1 : 0;
+    if (out->valid) {  // pipeline/mod.rs:439-441
+        const double g = std::exp(host[1] / host[3]);
+        out->neutral[0] = (float)(std::exp(host[0] / host[3]) / g);
+        out->neutral[1] = 1.0f;
+        out->neutral[2] = (float)(std::exp(host[2] / host[3]) / g);
+    } else {
+        out->neutral[0] = out->neutral[1] = out->neutral[2] = 1.0f;
+    }
+    return DS_OK;
+}
+
 
 }  // namespace
 
@@ -689,6 +1314,16 @@ void ds_ctx_destroy(ds_ctx* c) {
             hipEventDestroy(p.a);
             hipEventDestroy(p.b);
         }
+        for (auto& g : c->graphs) destroy_graph(g);
+        c->graphs.clear();
+        for (ds_ctx* b : c->branches) {
+            hipStreamSynchronize(b->stream);
+            for (auto& sb : b->slots)
+                if (sb.p) hipFree(sb.p);
+            hipStreamDestroy(b->stream);
+            delete b;
+        }
+        c->branches.clear();
         for (auto& b : c->slots)
             if (b.p) hipFree(b.p);
         for (void* p : c->user_allocs) hipFree(p);
@@ -1055,7 +1690,6 @@ void ds_cnn_destroy(ds_cnn* net) {
         hipFree(net->wpk);
         if (net->tr_act) hipFree(net->tr_act);
         if (net->tr_f) hipFree(net->tr_f);
-        if (net->inf_act) hipFree(net->inf_act);
     }
     delete net;
 }
@@ -1208,7 +1842,7 @@ int ds_cnn_denoise_plane(ds_cnn* net, const float* L, size_t w, size_t h, float*
     SLOT(c, sig, float*, S_SIGMA, 64);
     HIPCHK(c, hipMemcpyAsync(dL, L, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
     if (int rc = noise_dev(c, dL, w, h, sig)) return rc;
-    if (int rc = cnn_plane_dev(net, dL, w, h, sig, dLF)) return rc;
+    if (int rc = cnn_plane_dev(c, net, dL, w, h, sig, dLF)) return rc;
     HIPCHK(c, hipMemcpyAsync(lf_out, dLF, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;
@@ -1239,6 +1873,63 @@ int ds_denoise_learned(ds_ctx* c, ds_cnn* net, const float* rgb, size_t w, size_
     HIPCHK(c, hipMemcpyAsync(rgb_out, d_out, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;
+}
+
+
+// ---- batched export, autotuner, statistics
+
+int ds_pipeline_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_opts* opts, ds_batch_info* info) {
+    LOCK(c);
+    if (!opts) return fail(c, DS_ERR_INVALID, "opts is null");
+    return pipeline_batch(c, frames, n, *opts, info);
+}
+
+int ds_autotune(ds_ctx* c, size_t w, size_t h, ds_tuning* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    return autotune_shape(c, w, h, out);
+}
+
+int ds_set_tuning(ds_ctx* c, size_t w, size_t h, const ds_tuning* t) {
+    LOCK(c);
+    if (!t) return fail(c, DS_ERR_INVALID, "tuning is null");
+    if (t->rcd_variant < 0 || t->rcd_variant >= demosaic_variants(DS_ALGO_RCD) || t->mhc_variant < 0 ||
+        t->mhc_variant >= demosaic_variants(DS_ALGO_MHC) || t->rl_variant < 0 || t->rl_variant >= RL_VARIANTS)
+        return fail(c, DS_ERR_INVALID, "tile variant out of range");
+    c->tuning[{w, h}] = Tuned{t->rcd_variant, t->mhc_variant, t->rl_variant};
+    return DS_OK;
+}
+
+int ds_get_tuning(ds_ctx* c, size_t w, size_t h, ds_tuning* out) {
+    LOCK(c);
+    if (!out) return fail(c, DS_ERR_INVALID, "out is null");
+    std::memset(out, 0, sizeof(*out));
+    const Tuned t = tuned_for(c, w, h);
+    out->rcd_variant = t.rcd;
+    out->mhc_variant = t.mhc;
+    out->rl_variant = t.rl;
+    return DS_OK;
+}
+
+int ds_auto_wb_stats_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, const float as_shot_neutral[3], ds_wb_stats* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!out || !as_shot_neutral || (!rgb && w * h)) return fail(c, DS_ERR_INVALID, "null pointer");
+    return wb_stats_dev(c, rgb, w, h, as_shot_neutral, out);
+}
+
+int ds_auto_wb_stats(ds_ctx* c, const float* rgb, size_t w, size_t h, const float as_shot_neutral[3], ds_wb_stats* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!out || !as_shot_neutral || (!rgb && w * h)) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t n = w * h;
+    float* d = nullptr;
+    if (n) {
+        SLOT(c, dd, float*, S_IN, n * 12);
+        d = dd;
+        HIPCHK(c, hipMemcpyAsync(d, rgb, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    }
+    return wb_stats_dev(c, d, w, h, as_shot_neutral, out);
 }
 
 }  // extern "C"

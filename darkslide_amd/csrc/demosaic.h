@@ -22,6 +22,8 @@ struct DemosaicArgs {
     float* clip;  // clip plane out (device) or null
 };
 
-hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream);
+// variant: tile shape, 0 .. demosaic_variants(algo) - 1 (same results)
+int demosaic_variants(int algo);
+hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream, int variant = 0);
 
 }  // namespace ds

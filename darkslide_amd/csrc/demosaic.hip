@@ -427,12 +427,29 @@ __global__ __launch_bounds__(NT) void k_mhc(CfaSrc src, float* __restrict__ rgb,
     }
 }
 
-constexpr int RCD_TX = 64, RCD_TY = 32, RCD_NT = 512;
-constexpr int MHC_TX = 64, MHC_TY = 16, MHC_NT = 256;
+template <int TX, int TY, int NT>
+hipError_t launch_rcd(const CfaSrc& src, float* rgb, float* clip, hipStream_t stream) {
+    dim3 grid((unsigned)((src.w + TX - 1) / TX), (unsigned)((src.h + TY - 1) / TY));
+    hipLaunchKernelGGL((k_rcd<TX, TY, NT>), grid, dim3(NT), 0, stream, src, rgb, clip);
+    return hipGetLastError();
+}
+
+template <int TX, int TY, int NT>
+hipError_t launch_mhc(const CfaSrc& src, float* rgb, float* clip, hipStream_t stream) {
+    dim3 grid((unsigned)((src.w + TX - 1) / TX), (unsigned)((src.h + TY - 1) / TY));
+    hipLaunchKernelGGL((k_mhc<TX, TY, NT>), grid, dim3(NT), 0, stream, src, rgb, clip);
+    return hipGetLastError();
+}
 
 }  // namespace
 
-hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream) {
+// Tile-shape variants (TX x TY, threads), all with the same per-pixel
+// arithmetic, so every variant gives the same result bit for bit.
+//   RCD: 0 = 64x32/512, 1 = 32x32/256, 2 = 64x16/256
+//   MHC: 0 = 64x16/256, 1 = 128x8/256, 2 = 32x32/256
+int demosaic_variants(int algo) { return algo == DS_ALGO_RCD ? 3 : 3; }
+
+hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream, int variant) {
     CfaSrc src{};
     src.f32 = a.cfa_f32;
     src.u16 = a.raw_u16;
@@ -448,13 +465,17 @@ hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream) {
     // rcd_bayer falls back to MHC below 16 pixels (demosaic.rs:149-151)
     bool rcd = a.algo == DS_ALGO_RCD && a.w >= 16 && a.h >= 16;
     if (rcd) {
-        dim3 grid((unsigned)((a.w + RCD_TX - 1) / RCD_TX), (unsigned)((a.h + RCD_TY - 1) / RCD_TY));
-        hipLaunchKernelGGL((k_rcd<RCD_TX, RCD_TY, RCD_NT>), grid, dim3(RCD_NT), 0, stream, src, a.rgb, a.clip);
-    } else {
-        dim3 grid((unsigned)((a.w + MHC_TX - 1) / MHC_TX), (unsigned)((a.h + MHC_TY - 1) / MHC_TY));
-        hipLaunchKernelGGL((k_mhc<MHC_TX, MHC_TY, MHC_NT>), grid, dim3(MHC_NT), 0, stream, src, a.rgb, a.clip);
+        switch (variant) {
+            case 1: return launch_rcd<32, 32, 256>(src, a.rgb, a.clip, stream);
+            case 2: return launch_rcd<64, 16, 256>(src, a.rgb, a.clip, stream);
+            default: return launch_rcd<64, 32, 512>(src, a.rgb, a.clip, stream);
+        }
     }
-    return hipGetLastError();
+    switch (a.algo == DS_ALGO_MHC ? variant : 0) {
+        case 1: return launch_mhc<128, 8, 256>(src, a.rgb, a.clip, stream);
+        case 2: return launch_mhc<32, 32, 256>(src, a.rgb, a.clip, stream);
+        default: return launch_mhc<64, 16, 256>(src, a.rgb, a.clip, stream);
+    }
 }
 
 }  // namespace ds

@@ -35,6 +35,7 @@ constexpr int NLM_SPOT = 16;  // spot-check lattice per region side
 
 size_t nlm_tile_lds_bytes(int S, int P);
 size_t nlm_regions(size_t w, size_t h);
+size_t nlm_spot_scratch_floats(size_t w, size_t h);
 
 // nlm(p, h, search, patch); search >= 0.
 hipError_t launch_nlm(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, hipStream_t stream);
@@ -46,8 +47,8 @@ hipError_t launch_nlm_regions(const float* in, float* out, size_t w, size_t h, N
 
 // Per-region deviation of `cand` from NLM in dB (gap_db) and the fallback
 // flags (gap_db > thr_db, or every region when thr_db <= 0); hp.sigma_dev
-// must be set.
+// must be set; scratch: nlm_spot_scratch_floats(w, h).
 hipError_t launch_nlm_spot_check(const float* in, const float* cand, size_t w, size_t h, NlmH hp, long search, size_t patch, float thr_db,
-                                 float* gap_db, uint8_t* flags, hipStream_t stream);
+                                 float* gap_db, uint8_t* flags, float* scratch, hipStream_t stream);
 
 }  // namespace ds

@@ -274,6 +274,140 @@ __global__ __launch_bounds__(NLM_SPOT* NLM_SPOT) void k_nlm_spot(const float* __
     }
 }
 
+// Spot check with the neighbourhood in LDS: one wave per lattice pixel.  A
+// block of SPOT_WAVES waves takes SPOT_PER_WAVE pixels per wave; SPOT_BLOCKS
+// blocks cover a region's NLM_SPOT x NLM_SPOT lattice.  The wave stages the
+// (2(S+P)+1)^2 clamped neighbourhood of its pixel, its lanes split the search
+// offsets, and a wave reduction gives NLM and the weight sums.  Each wave
+// writes its (sum e^2, sum var) pair; k_nlm_spot_final turns a region's
+// pairs into the deviation and the flag.
+constexpr int SPOT_WAVES = 4;
+constexpr int SPOT_PER_WAVE = 4;
+constexpr int SPOT_BLOCKS = NLM_SPOT * NLM_SPOT / (SPOT_WAVES * SPOT_PER_WAVE);
+constexpr int SPOT_PARTS = SPOT_BLOCKS * SPOT_WAVES;  // (e^2, var) pairs per region
+constexpr int SPOT_MAX_SP = 24;
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <int P>
+__global__ __launch_bounds__(SPOT_WAVES * 64) void k_nlm_spot_lds(const float* __restrict__ in, const float* __restrict__ cand, int w, int h,
+                                                                  int S, NlmWeight wp, float norm, float* __restrict__ part) {
+    extern __shared__ float lds[];
+    const int SP = S + P, D = 2 * SP + 1;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* nb = lds + wave * D * D;
+    const int rx = blockIdx.y, ry = blockIdx.z;
+    const int x0 = rx * NLM_REGION, y0 = ry * NLM_REGION;
+    const int rw = min(NLM_REGION, w - x0), rh = min(NLM_REGION, h - y0);
+    const float inv_h2 = 1.0f / wp.h2();
+    const float sg = wp.sigma_dev ? fmaxf(*wp.sigma_dev, 1e-4f) : 1e-4f;
+    const int side = 2 * S + 1, noff = side * side;
+    float se = 0.0f, sv = 0.0f;
+    for (int k = 0; k < SPOT_PER_WAVE; k++) {
+        const int sidx = (blockIdx.x * SPOT_WAVES + wave) * SPOT_PER_WAVE + k;
+        const int i = sidx % NLM_SPOT, j = sidx / NLM_SPOT;
+        const int x = x0 + ((2 * i + 1) * rw) / (2 * NLM_SPOT), y = y0 + ((2 * j + 1) * rh) / (2 * NLM_SPOT);
+        // nb[(v + SP) * D + (u + SP)] = p[clamp(x + u), clamp(y + v)]
+        for (int base = 0; base < D * D; base += 64 * 8) {
+            float r[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int t = base + lane + 64 * q;
+                r[q] = t < D * D ? in[(size_t)clampi(y + t / D - SP, 0, h - 1) * w + clampi(x + t % D - SP, 0, w - 1)] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int t = base + lane + 64 * q;
+                if (t < D * D) nb[t] = r[q];
+            }
+        }
+        __syncthreads();
+        const bool interior = x - SP >= 0 && y - SP >= 0 && x + SP < w && y + SP < h;
+        // centre patch, as the two-level clamp sees it: window pixel (cx, cy) = clamp(x + i, y + j)
+        float cpat[2 * P + 1][2 * P + 1];
+        int cxs[2 * P + 1], cys[2 * P + 1];
+#pragma unroll
+        for (int q = -P; q <= P; q++) {
+            cxs[q + P] = clampi(x + q, 0, w - 1) - x;
+            cys[q + P] = clampi(y + q, 0, h - 1) - y;
+        }
+#pragma unroll
+        for (int jj = 0; jj < 2 * P + 1; jj++)
+#pragma unroll
+            for (int ii = 0; ii < 2 * P + 1; ii++) cpat[jj][ii] = nb[(cys[jj] + SP) * D + cxs[ii] + SP];
+        float acc = 0.0f, ws = 0.0f, w2 = 0.0f;
+        for (int o = lane; o < noff; o += 64) {
+            const int dy = o / side - S, dx = o % side - S;
+            float dist = 0.0f;
+            if (interior) {
+                const float* q = nb + (SP - P + dy) * D + (SP - P + dx);
+#pragma unroll
+                for (int jj = 0; jj < 2 * P + 1; jj++) {
+                    float row = 0.0f;
+#pragma unroll
+                    for (int ii = 0; ii < 2 * P + 1; ii++) {
+                        const float e = cpat[jj][ii] - q[jj * D + ii];
+                        row += e * e;
+                    }
+                    dist += row * norm;
+                }
+            } else {
+#pragma unroll
+                for (int jj = 0; jj < 2 * P + 1; jj++) {
+                    const int sy = clampi(y + cys[jj] + dy, 0, h - 1) - y + SP;
+                    float row = 0.0f;
+#pragma unroll
+                    for (int ii = 0; ii < 2 * P + 1; ii++) {
+                        const int sx = clampi(x + cxs[ii] + dx, 0, w - 1) - x + SP;
+                        const float e = cpat[jj][ii] - nb[sy * D + sx];
+                        row += e * e;
+                    }
+                    dist += row * norm;
+                }
+            }
+            dist *= norm;
+            const float wt = expf(-(dist * inv_h2));
+            acc += wt * nb[(dy + SP) * D + dx + SP];
+            ws += wt;
+            w2 += wt * wt;
+        }
+        acc = wave_sum(acc);
+        ws = wave_sum(ws);
+        w2 = wave_sum(w2);
+        const float v = acc / fmaxf(ws, 1e-12f);
+        const float e = cand[(size_t)y * w + x] - v;
+        se += e * e;
+        sv += sg * sg * w2 / fmaxf(ws * ws, 1e-30f);
+        __syncthreads();  // nb is restaged next
+    }
+    if (lane == 0) {
+        float* pp = part + ((size_t)(ry * gridDim.y + rx) * SPOT_PARTS + blockIdx.x * SPOT_WAVES + wave) * 2;
+        pp[0] = se;
+        pp[1] = sv;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_nlm_spot_final(const float* __restrict__ part, float thr_db, float* __restrict__ gap_db,
+                                                       uint8_t* __restrict__ flags) {
+    const int r = blockIdx.x;
+    float a = 0.0f, b = 0.0f;
+    for (int q = threadIdx.x; q < SPOT_PARTS; q += 64) {
+        a += part[((size_t)r * SPOT_PARTS + q) * 2];
+        b += part[((size_t)r * SPOT_PARTS + q) * 2 + 1];
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (threadIdx.x == 0) {
+        const float g = 10.0f * log10f(1.0f + a / fmaxf(b, 1e-30f));
+        gap_db[r] = g;
+        flags[r] = (thr_db <= 0.0f || !(g <= thr_db)) ? 1 : 0;  // NaN falls back too
+    }
+}
+
 template <int P>
 hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeight wp, const uint8_t* regions, hipStream_t stream) {
     size_t lds = nlm_tile_lds_bytes(S, P);
@@ -337,14 +471,51 @@ hipError_t launch_nlm(const float* in, float* out, size_t w, size_t h, NlmH hp, 
 
 size_t nlm_regions(size_t w, size_t h) { return ((w + NLM_REGION - 1) / NLM_REGION) * ((h + NLM_REGION - 1) / NLM_REGION); }
 
+size_t nlm_spot_scratch_floats(size_t w, size_t h) { return nlm_regions(w, h) * SPOT_PARTS * 2; }
+
+namespace {
+template <int P>
+hipError_t launch_spot_lds(const float* in, const float* cand, int w, int h, int S, NlmWeight wp, float norm, float* part, hipStream_t stream) {
+    const int D = 2 * (S + P) + 1;
+    const size_t lds = (size_t)SPOT_WAVES * D * D * sizeof(float);
+    static bool attr_set = false;
+    if (!attr_set) {
+        const int DM = 2 * SPOT_MAX_SP + 1;
+        hipError_t e = hipFuncSetAttribute((const void*)k_nlm_spot_lds<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(SPOT_WAVES * DM * DM * sizeof(float)));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    dim3 grid(SPOT_BLOCKS, (unsigned)((w + NLM_REGION - 1) / NLM_REGION), (unsigned)((h + NLM_REGION - 1) / NLM_REGION));
+    hipLaunchKernelGGL(k_nlm_spot_lds<P>, grid, dim3(SPOT_WAVES * 64), lds, stream, in, cand, w, h, S, wp, norm, part);
+    return hipGetLastError();
+}
+}  // namespace
+
 hipError_t launch_nlm_spot_check(const float* in, const float* cand, size_t w, size_t h, NlmH hp, long search, size_t patch, float thr_db,
-                                 float* gap_db, uint8_t* flags, hipStream_t stream) {
+                                 float* gap_db, uint8_t* flags, float* scratch, hipStream_t stream) {
     if (w == 0 || h == 0) return hipSuccess;
     double norm;
-    const NlmWeight wp = make_weight(hp, (int)patch, norm);
-    dim3 grid((unsigned)((w + NLM_REGION - 1) / NLM_REGION), (unsigned)((h + NLM_REGION - 1) / NLM_REGION));
-    hipLaunchKernelGGL(k_nlm_spot, grid, dim3(NLM_SPOT * NLM_SPOT), 0, stream, in, cand, (int)w, (int)h, (int)search, (int)patch, wp, (float)norm,
-                       thr_db, gap_db, flags);
+    const int S = (int)search, P = (int)patch;
+    const NlmWeight wp = make_weight(hp, P, norm);
+    const unsigned nrx = (unsigned)((w + NLM_REGION - 1) / NLM_REGION), nry = (unsigned)((h + NLM_REGION - 1) / NLM_REGION);
+    if (S + P <= SPOT_MAX_SP && P <= 5) {
+        hipError_t e = hipSuccess;
+        switch (P) {
+            case 0: e = launch_spot_lds<0>(in, cand, (int)w, (int)h, S, wp, (float)norm, scratch, stream); break;
+            case 1: e = launch_spot_lds<1>(in, cand, (int)w, (int)h, S, wp, (float)norm, scratch, stream); break;
+            case 2: e = launch_spot_lds<2>(in, cand, (int)w, (int)h, S, wp, (float)norm, scratch, stream); break;
+            case 3: e = launch_spot_lds<3>(in, cand, (int)w, (int)h, S, wp, (float)norm, scratch, stream); break;
+            case 4: e = launch_spot_lds<4>(in, cand, (int)w, (int)h, S, wp, (float)norm, scratch, stream); break;
+            default: e = launch_spot_lds<5>(in, cand, (int)w, (int)h, S, wp, (float)norm, scratch, stream); break;
+        }
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_nlm_spot_final, dim3(nrx * nry), dim3(64), 0, stream, scratch, thr_db, gap_db, flags);
+        return hipGetLastError();
+    }
+    dim3 grid(nrx, nry);
+    hipLaunchKernelGGL(k_nlm_spot, grid, dim3(NLM_SPOT * NLM_SPOT), 0, stream, in, cand, (int)w, (int)h, S, P, wp, (float)norm, thr_db, gap_db,
+                       flags);
     return hipGetLastError();
 }
 

@@ -12,6 +12,9 @@ int gaussian_taps(float sigma, float* k);
 size_t rl_workspace_floats(size_t w, size_t h);
 
 // raw_details(rgb, sigma, iters) -> rgb_out (enhance.rs:96-119).
-hipError_t launch_raw_details(const float* rgb, size_t w, size_t h, float sigma, size_t iters, float* ws, float* rgb_out, hipStream_t st);
+// variant: tile shape of the iteration kernel, 0 .. RL_VARIANTS - 1 (same results)
+constexpr int RL_VARIANTS = 3;
+hipError_t launch_raw_details(const float* rgb, size_t w, size_t h, float sigma, size_t iters, float* ws, float* rgb_out, hipStream_t st,
+                              int variant = 0);
 
 }  // namespace ds

@@ -25,7 +25,7 @@ namespace ds {
 
 namespace {
 
-constexpr int RL_TX = 64, RL_TY = 32, RL_NT = 256;
+constexpr int RL_TX = 64, RL_NT = 256;
 
 struct Taps {
     float k[13];
@@ -52,10 +52,11 @@ __global__ void k_rl_apply(const float* __restrict__ rgb, const float* __restric
     out[3 * i + 2] = rgb[3 * i + 2] * r;
 }
 
-template <int R>
+template <int R, int TY>
 __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est, const float* __restrict__ yp, float* __restrict__ est_out,
                                                    int w, int h, Taps taps) {
-    constexpr int TX = RL_TX, TY = RL_TY, NT = RL_NT, K = 2 * R + 1, B = 8;
+    static_assert(TY % 8 == 0, "row blocks of 8");
+    constexpr int TX = RL_TX, NT = RL_NT, K = 2 * R + 1, B = 8;
     // est: [-2R, T+2R) both ways; y, ratio: [-R, T+R); tmp: cols [-R, TX+R), rows [-2R, TY+2R)
     constexpr int EW = TX + 4 * R, EH = TY + 4 * R;
     constexpr int YW = TX + 2 * R, YH = TY + 2 * R;
@@ -219,11 +220,22 @@ __global__ void k_rl_update(float* __restrict__ est, const float* __restrict__ c
     if (i < n) est[i] = clampr(est[i] * corr[i], 0.0f, 64.0f);
 }
 
-template <int R>
-hipError_t launch_iter(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, hipStream_t st) {
-    dim3 grid((unsigned)((w + RL_TX - 1) / RL_TX), (unsigned)((h + RL_TY - 1) / RL_TY));
-    hipLaunchKernelGGL(k_rl_iter<R>, grid, dim3(RL_NT), 0, st, est, y, est_out, (int)w, (int)h, t);
+template <int R, int TY>
+hipError_t launch_iter_ty(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, hipStream_t st) {
+    dim3 grid((unsigned)((w + RL_TX - 1) / RL_TX), (unsigned)((h + TY - 1) / TY));
+    hipLaunchKernelGGL((k_rl_iter<R, TY>), grid, dim3(RL_NT), 0, st, est, y, est_out, (int)w, (int)h, t);
     return hipGetLastError();
+}
+
+// Tile-shape variants of the iteration kernel (64 x TY, same results):
+// 0 = 64x32, 1 = 64x16, 2 = 64x64.
+template <int R>
+hipError_t launch_iter(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, int variant, hipStream_t st) {
+    switch (variant) {
+        case 1: return launch_iter_ty<R, 16>(est, y, est_out, w, h, t, st);
+        case 2: return launch_iter_ty<R, 64>(est, y, est_out, w, h, t, st);
+        default: return launch_iter_ty<R, 32>(est, y, est_out, w, h, t, st);
+    }
 }
 
 }  // namespace
@@ -243,7 +255,8 @@ int gaussian_taps(float sigma, float* k) {
 
 size_t rl_workspace_floats(size_t w, size_t h) { return 5 * w * h; }
 
-hipError_t launch_raw_details(const float* rgb, size_t w, size_t h, float sigma, size_t iters, float* ws, float* rgb_out, hipStream_t st) {
+hipError_t launch_raw_details(const float* rgb, size_t w, size_t h, float sigma, size_t iters, float* ws, float* rgb_out, hipStream_t st,
+                              int variant) {
     const size_t n = w * h;
     if (n == 0) return hipSuccess;
     float* y = ws;
@@ -262,12 +275,12 @@ hipError_t launch_raw_details(const float* rgb, size_t w, size_t h, float sigma,
     for (size_t it = 0; it < iters; it++) {
         if (exact) {
             switch (r) {
-                case 1: e = launch_iter<1>(est, y, est2, w, h, taps, st); break;
-                case 2: e = launch_iter<2>(est, y, est2, w, h, taps, st); break;
-                case 3: e = launch_iter<3>(est, y, est2, w, h, taps, st); break;
-                case 4: e = launch_iter<4>(est, y, est2, w, h, taps, st); break;
-                case 5: e = launch_iter<5>(est, y, est2, w, h, taps, st); break;
-                case 6: e = launch_iter<6>(est, y, est2, w, h, taps, st); break;
+                case 1: e = launch_iter<1>(est, y, est2, w, h, taps, variant, st); break;
+                case 2: e = launch_iter<2>(est, y, est2, w, h, taps, variant, st); break;
+                case 3: e = launch_iter<3>(est, y, est2, w, h, taps, variant, st); break;
+                case 4: e = launch_iter<4>(est, y, est2, w, h, taps, variant, st); break;
+                case 5: e = launch_iter<5>(est, y, est2, w, h, taps, variant, st); break;
+                case 6: e = launch_iter<6>(est, y, est2, w, h, taps, variant, st); break;
                 default: return hipErrorInvalidValue;
             }
             if (e != hipSuccess) return e;

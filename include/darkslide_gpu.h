@@ -216,6 +216,91 @@ int ds_denoise_learned(ds_ctx* ctx, ds_cnn* net, const float* rgb, size_t w, siz
 int ds_denoise_learned_dev(ds_ctx* ctx, ds_cnn* net, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch,
                            float threshold_db, float* rgb_out, ds_learned_stats* stats);
 
+/* ---- batched export ----------------------------------------------------- */
+
+/* Stands in for the sequential per-file loops of the CLI and GUI batch
+ * export (cli.rs:142-187, dialogs.rs:310-365), each of which runs
+ * Engine::base_for (pipeline/mod.rs:229-252) per image: demosaic ->
+ * [denoise] -> [raw_details].  Every frame of the batch goes through one
+ * hipGraph, captured on the first call and replayed while the frames
+ * (pointers, shapes, formats) and options stay the same.  Frames are spread
+ * over `streams` parallel graph branches (longest first), each with its own
+ * workspace.  With autotune set, tile shapes are chosen per frame shape by
+ * timing every variant once (ds_autotune), and with streams == 0 the branch
+ * count too.  All pointers are device pointers; the call only enqueues,
+ * unless info is set (then it waits and fills it). */
+#define DS_DENOISE_NONE 0
+#define DS_DENOISE_NLM 1
+#define DS_DENOISE_LEARNED 2
+#define DS_MAX_BRANCHES 8
+
+typedef struct ds_frame {
+    const uint16_t* raw; /* h rows of `stride` u16 */
+    size_t w, h, stride;
+    uint8_t pattern[4];
+    float black;
+    float white[3];
+    float mul[3];
+    float* rgb_out; /* w*h interleaved RGB */
+} ds_frame;
+
+typedef struct ds_pipeline_opts {
+    int algo;             /* DS_ALGO_RCD / DS_ALGO_MHC */
+    int denoise;          /* DS_DENOISE_* */
+    float amount;         /* denoise amount (enhance.rs:75) */
+    long search;          /* NLM search radius */
+    size_t patch;         /* NLM patch radius */
+    int raw_details;      /* run RL */
+    float rl_sigma;
+    size_t rl_iters;
+    int reference_gating; /* 1: base_for's rules: amount.max(1), RL (0.75, 8) only for a long edge > 3200 */
+    ds_cnn* net;          /* DS_DENOISE_LEARNED */
+    float threshold_db;   /* learned: spot-check threshold */
+    int streams;          /* graph branches, 1..DS_MAX_BRANCHES; 0 = 2, or tuned when autotune is set */
+    int autotune;
+} ds_pipeline_opts;
+
+typedef struct ds_batch_info {
+    size_t frames;
+    size_t branches;
+    int graph_reused;        /* 1: replayed a graph captured by an earlier call */
+    double capture_ms;       /* host time spent capturing (0 when reused) */
+    size_t fallback_regions; /* learned: regions recomputed with NLM */
+    size_t regions;
+} ds_batch_info;
+
+int ds_pipeline_batch(ds_ctx* ctx, const ds_frame* frames, size_t n, const ds_pipeline_opts* opts, ds_batch_info* info);
+
+/* Tile autotuner.  Variants of the RCD and MHC demosaic tiles and of the RL
+ * iteration tile compute the same results; ds_autotune times each on a
+ * synthetic frame of the shape and keeps the fastest for that shape (used
+ * by every later call on the context).  *_ms: median time per variant. */
+typedef struct ds_tuning {
+    int rcd_variant, mhc_variant, rl_variant;
+    float rcd_ms[4], mhc_ms[4], rl_ms[4];
+} ds_tuning;
+
+int ds_autotune(ds_ctx* ctx, size_t w, size_t h, ds_tuning* out);
+int ds_set_tuning(ds_ctx* ctx, size_t w, size_t h, const ds_tuning* tuning);
+int ds_get_tuning(ds_ctx* ctx, size_t w, size_t h, ds_tuning* out);
+
+/* ---- per-image statistics (FP64 on the GPU) ------------------------------ */
+
+/* The statistics of Engine::auto_wb_xy (pipeline/mod.rs:421-441) over an
+ * RGB image: every third pixel, raw = p * as_shot_neutral, skipped when
+ * max > 0.9 or min < 0.002; w = sqrt(max); acc[c] = sum ln(raw[c]) w,
+ * n = sum w, accumulated in FP64; neutral = [exp(acc0/n)/g, 1, exp(acc2/n)/g]
+ * with g = exp(acc1/n) (valid = 0 and neutral 1 when n == 0). */
+typedef struct ds_wb_stats {
+    double acc[3];
+    double n;
+    float neutral[3];
+    int valid;
+} ds_wb_stats;
+
+int ds_auto_wb_stats(ds_ctx* ctx, const float* rgb, size_t w, size_t h, const float as_shot_neutral[3], ds_wb_stats* out);
+int ds_auto_wb_stats_dev(ds_ctx* ctx, const float* rgb, size_t w, size_t h, const float as_shot_neutral[3], ds_wb_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -270,3 +270,17 @@ def cnn_input(L, sigma):
     f.restype = None
     f(_p(L), w, h, ctypes.c_double(sigma), _p(x))
     return x
+
+
+def auto_wb_stats(rgb, as_shot_neutral):
+    """pipeline/mod.rs:421-441 statistics (sequential FP64) -> (acc[3], n, neutral or None)."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    npx = rgb.size // 3
+    acc = np.zeros(3, np.float64)
+    n = ctypes.c_double()
+    neutral = np.zeros(3, np.float32)
+    nas = np.ascontiguousarray(as_shot_neutral, dtype=np.float32)
+    f = lib().or_auto_wb_stats
+    f.restype = ctypes.c_int
+    ok = f(_p(rgb), ctypes.c_size_t(npx), _p(nas), _p(acc), ctypes.byref(n), _p(neutral))
+    return acc, n.value, (neutral if ok else None)

@@ -783,4 +783,29 @@ void or_box_radii_for_gauss(float sigma, size_t* radii) { box_radii_for_gauss(si
 
 int or_max_threads(void) { return omp_get_max_threads(); }
 
+// Engine::auto_wb_xy statistics (pipeline/mod.rs:421-441): sequential FP64
+// accumulation over every third pixel of the image, as the reference loops.
+// out: acc[3], n, neutral[3] (neutral only when n > 0; returns 1 then).
+int or_auto_wb_stats(const float* rgb, size_t npx, const float* n_as, double* acc, double* n_out, float* neutral) {
+    acc[0] = acc[1] = acc[2] = 0.0;
+    double n = 0.0;
+    for (size_t i = 0; i < npx; i += 3) {
+        const float* p = rgb + 3 * i;
+        const float raw[3] = {p[0] * n_as[0], p[1] * n_as[1], p[2] * n_as[2]};
+        const float m = std::max(std::max(raw[0], raw[1]), raw[2]);
+        const float lo = std::min(std::min(raw[0], raw[1]), raw[2]);
+        if (m > 0.9f || lo < 0.002f) continue;
+        const double w = std::sqrt((double)m);
+        for (int c = 0; c < 3; c++) acc[c] += std::log((double)raw[c]) * w;
+        n += w;
+    }
+    *n_out = n;
+    if (!(n > 0.0)) return 0;
+    const double g = std::exp(acc[1] / n);
+    neutral[0] = (float)(std::exp(acc[0] / n) / g);
+    neutral[1] = 1.0f;
+    neutral[2] = (float)(std::exp(acc[2] / n) / g);
+    return 1;
+}
+
 }  // extern "C"

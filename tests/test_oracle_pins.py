@@ -538,3 +538,46 @@ def test_reference_denoise_reproducibility_envelope(oracle_mod):
     assert maxabs(O.nlm(L, 0.6 * s, 10, 3, nthreads=1), O.nlm(L, 0.6 * s, 10, 3, nthreads=8)) < 5e-6
     A = np.ascontiguousarray(O.oklab_from_prophoto(rgb)[..., 1])
     assert np.array_equal(O.guided_filter(L, A, 2, 1e-4), O.guided_filter(L, A, 2, 1e-4))
+
+
+# ------------------------------------------------------------------ auto_wb_xy statistics (pipeline/mod.rs:421-441)
+
+def test_auto_wb_stats_uniform_closed_form(oracle_mod):
+    """A uniform image: every sampled pixel adds ln(raw_c) sqrt(max); the neutral is raw_c / raw_g."""
+    O = oracle_mod
+    h, w = 7, 11
+    v = np.array([0.3, 0.5, 0.2], np.float32)
+    nas = np.array([0.8, 1.0, 1.25], np.float32)
+    rgb = np.broadcast_to(v, (h, w, 3)).copy()
+    acc, n, neutral = O.auto_wb_stats(rgb, nas)
+    cnt = len(range(0, h * w, 3))
+    raw = (v * nas).astype(np.float32).astype(np.float64)
+    wt = np.sqrt(raw.max())
+    assert abs(n - cnt * wt) <= 1e-12 * n
+    assert np.allclose(acc, cnt * np.log(raw) * wt, rtol=1e-12, atol=0)
+    assert np.allclose(neutral, [raw[0] / raw[1], 1.0, raw[2] / raw[1]], rtol=2e-7)
+
+
+def test_auto_wb_stats_sampling_and_thresholds(oracle_mod):
+    """Only every third pixel counts, and pixels with max > 0.9 or min < 0.002 are skipped."""
+    O = oracle_mod
+    rng = np.random.default_rng(3)
+    rgb = rng.uniform(0.05, 0.6, (9, 10, 3)).astype(np.float32)
+    nas = np.ones(3, np.float32)
+    base = O.auto_wb_stats(rgb, nas)
+    flat = rgb.reshape(-1, 3)
+    other = rgb.copy().reshape(-1, 3)
+    other[1::3] = rng.uniform(0.05, 0.6, other[1::3].shape)  # unsampled pixels do not matter
+    other[2::3] = 0.95
+    o2 = O.auto_wb_stats(other.reshape(rgb.shape), nas)
+    assert np.array_equal(base[0], o2[0]) and base[1] == o2[1]
+    # a sampled pixel pushed over 0.9 (or under 0.002) drops out: the sums equal those without it
+    cut = flat.copy()
+    cut[0] = [0.95, 0.5, 0.5]
+    cut[3] = [0.5, 0.001, 0.5]
+    acc_c, n_c, _ = O.auto_wb_stats(cut.reshape(rgb.shape), nas)
+    keep = flat[6::3]
+    m = keep.max(axis=1).astype(np.float64)
+    assert abs(n_c - np.sum(np.sqrt(m))) <= 1e-12 * n_c
+    # nothing valid -> no neutral
+    assert O.auto_wb_stats(np.full((4, 4, 3), 0.95, np.float32), nas)[2] is None
