@@ -140,6 +140,7 @@ _SIGS = {
     "ds_cnn_save": [_P, ctypes.c_char_p],
     "ds_cnn_load": [_P, ctypes.c_char_p],
     "ds_cnn_forward": [_P, _P, _SZ, _SZ, _SZ, _P],
+    "ds_cnn_infer": [_P, _P, _SZ, _SZ, _SZ, _P],
     "ds_cnn_loss_grad": [_P, _P, _P, _SZ, _SZ, _SZ, _SZ, ctypes.POINTER(ctypes.c_double), _P],
     "ds_cnn_train_step_dev": [_P, _P, _P, _SZ, _SZ, _SZ, _SZ, _F, ctypes.POINTER(ctypes.c_double)],
     "ds_cnn_make_pairs_dev": [_P, ctypes.c_uint64, _SZ, _SZ, _F, _L, _SZ, _SZ, _SZ, _P, _P],
@@ -466,6 +467,14 @@ class CNN:
         n, h, w = x.shape
         y = np.empty_like(x)
         self.ctx._check(lib().ds_cnn_forward(self.handle, _ptr(x), n, h, w, _ptr(y)))
+        return y
+
+    def infer(self, x):
+        """Fused inference forward (same result as forward)."""
+        x = np.ascontiguousarray(x, np.float32)
+        n, h, w = x.shape
+        y = np.empty_like(x)
+        self.ctx._check(lib().ds_cnn_infer(self.handle, _ptr(x), n, h, w, _ptr(y)))
         return y
 
     def loss_grad(self, x, t, margin):

@@ -612,13 +612,10 @@ int cnn_plane_dev(ds_ctx* c, ds_cnn* net, const float* L, size_t w, size_t h, co
     }
     SLOT(c, x, float*, S_X, n * 4);
     SLOT(c, o, float*, S_T, n * 4);
-    SLOT(c, act, uint16_t*, S_ACT, 2 * n * CNN_ACT_BYTES);
     int rc = run(c, "cnn_input", [&] { return cnn_input(L, (int)w, (int)h, sig, x, c->stream); });
     if (rc) return rc;
-    uint16_t* a = act;
-    uint16_t* b = act + n * 32;
-    uint16_t* acts[7] = {a, b, a, b, a, b, a};
-    rc = run(c, "cnn_forward", [&] { return cnn_forward(net->params(), net->wpk, x, 1, (int)h, (int)w, acts, o, c->stream); });
+    SLOT(c, act, uint16_t*, S_ACT, cnn_fused_act_bytes(1, (int)h, (int)w));
+    rc = run(c, "cnn_forward", [&] { return cnn_forward_fused(net->params(), net->wpk, x, 1, (int)h, (int)w, act, o, c->stream); });
     if (rc) return rc;
     return run(c, "cnn_output", [&] { return cnn_output(o, x, L, n, sig, lf, c->stream); });
 }
@@ -889,7 +886,7 @@ void frame_needs(const ds_frame& f, const FramePlan& p, size_t need[S_COUNT]) {
             up(S_REGIONS, (2 * nreg + 64 + nlm_spot_scratch_floats(f.w, f.h)) * 4);
             up(S_X, n * 4);
             up(S_T, n * 4);
-            up(S_ACT, 2 * n * CNN_ACT_BYTES);
+            up(S_ACT, cnn_fused_act_bytes(1, (int)f.h, (int)f.w));
         }
     }
     if (p.rl) up(S_WS, rl_workspace_floats(f.w, f.h) * 4);
@@ -1781,6 +1778,26 @@ int ds_cnn_forward(ds_cnn* net, const float* x, size_t n_img, size_t h, size_t w
     HIPCHK(c, hipMemcpyAsync(dx, x, npx * 4, hipMemcpyHostToDevice, c->stream), "H2D");
     if (int rc = cnn_forward_train(net, dx, (int)n_img, (int)h, (int)w, dout)) return rc;
     HIPCHK(c, hipMemcpyAsync(out, dout, npx * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_cnn_infer(ds_cnn* net, const float* x, size_t n_img, size_t h, size_t w, float* out) {
+    if (!net) return DS_ERR_INVALID;
+    ds_ctx* c = net->ctx;
+    LOCK(c);
+    if (!x || !out) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (int rc = cnn_check_shape(c, n_img, h, w)) return rc;
+    const size_t npx = n_img * h * w;
+    SLOT(c, dx, float*, S_X, npx * 4);
+    SLOT(c, dy, float*, S_T, npx * 4);
+    if (int rc = cnn_pack_now(net)) return rc;
+    HIPCHK(c, hipMemcpyAsync(dx, x, npx * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    SLOT(c, act, uint16_t*, S_ACT, cnn_fused_act_bytes((int)n_img, (int)h, (int)w));
+    if (int rc = run(c, "cnn_forward",
+                     [&] { return cnn_forward_fused(net->params(), net->wpk, dx, (int)n_img, (int)h, (int)w, act, dy, c->stream); }))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(out, dy, npx * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;
 }

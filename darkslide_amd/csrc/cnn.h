@@ -21,6 +21,11 @@ size_t cnn_param_count();
 hipError_t cnn_pack(const float* params, uint16_t* wfwd, uint16_t* wbwd, hipStream_t st);
 hipError_t cnn_forward(const float* params, const uint16_t* wfwd, const float* x, int n_img, int H, int W, uint16_t* const* acts, float* out,
                        hipStream_t st);
+// Inference forward in two fused streaming passes (activation a4 in HBM,
+// cnn_fused_act_bytes); equals cnn_forward bit for bit.
+size_t cnn_fused_act_bytes(int n_img, int H, int W);
+hipError_t cnn_forward_fused(const float* params, const uint16_t* wfwd, const float* x, int n_img, int H, int W, uint16_t* act, float* out,
+                             hipStream_t st);
 size_t cnn_wgrad_blocks(int n_img, int H, int W);
 size_t cnn_scratch_floats(int n_img, int H, int W);
 hipError_t cnn_backward(const float* params, const uint16_t* wbwd, const float* x, const float* tgt, int n_img, int H, int W, int margin,

@@ -109,14 +109,19 @@ __global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ a
     if (p >= npx * n_img) return;
     const size_t img = p / npx, q = p % npx;
     const int y = (int)(q / Wd), xx = (int)(q % Wd);
+    // per-tap partial sums over ci, then bias + taps in order (the fused
+    // forward adds in the same order)
     float acc = sw[9 * C];
     for (int t = 0; t < 9; t++) {
         const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
-        if (yy < 0 || yy >= H || xc < 0 || xc >= Wd) continue;
-        float v[C];
-        load_px32(act + (img * npx + (size_t)yy * Wd + xc) * C, v);
+        float part = 0.0f;
+        if (yy >= 0 && yy < H && xc >= 0 && xc < Wd) {
+            float v[C];
+            load_px32(act + (img * npx + (size_t)yy * Wd + xc) * C, v);
 #pragma unroll
-        for (int ci = 0; ci < C; ci++) acc += sw[t * C + ci] * v[ci];
+            for (int ci = 0; ci < C; ci++) part += sw[t * C + ci] * v[ci];
+        }
+        acc += part;
     }
     out[p] = x[p] + acc;
 }
@@ -486,6 +491,236 @@ __global__ void k_crops(const float* __restrict__ a, const float* __restrict__ b
     cb[i] = b[src];
 }
 
+// ---------------------------------------------------------------- fused inference forward
+
+// The forward in two streaming passes of three MFMA layers each:
+//   pass A: x -> a1 (FP32) -> a2 -> a3 -> a4, a4 written to HBM (bf16 NHWC);
+//   pass B: a4 -> a5 -> a6 -> a7 -> y (FP32 output layer and residual).
+// A block owns a strip of FS_OUT output columns and FS_SH output rows and
+// walks down it one row per step, keeping the last three rows of each
+// activation it needs in LDS rings.  Rows are FS_W = FS_OUT + 2 FS_HALO
+// pixels wide (6 MFMA column tiles); a pass reaches 4 pixels, so the halo
+// columns' values go stale one column per layer from the strip edge inwards
+// and never reach the stored columns.  Activations outside the image are
+// stored as 0 (the zero padding of every layer).  The next input row is
+// fetched into registers one step ahead.  The arithmetic is that of
+// k_conv_in / k_conv_mid<0> / k_conv_out operation by operation, so the
+// result equals the layer-by-layer forward bit for bit.
+constexpr int FS_HALO = 4;
+constexpr int FS_W = 96;                    // 6 MFMA column tiles
+constexpr int FS_OUT = FS_W - 2 * FS_HALO;  // 88
+constexpr int FS_RW = FS_W + 2;             // ring row: 1 zero pixel each side
+constexpr int FS_SH = 192;
+constexpr int FS_NT = 256;
+constexpr size_t FS_ROW_BYTES = (size_t)FS_RW * C * sizeof(uint16_t);
+// pass A: rings a1..a3, x ring, W_in/b_in; pass B: rings a4..a7, output partials, W_out
+constexpr size_t FS_LDS_A = 3 * 3 * FS_ROW_BYTES + (3 * FS_RW + C * 9 + C) * sizeof(float);
+constexpr size_t FS_LDS_B = 4 * 3 * FS_ROW_BYTES + (9 * FS_OUT + 9 * C) * sizeof(float);
+
+__device__ __forceinline__ int mod3(int r) { return ((r % 3) + 3) % 3; }
+
+// One MFMA layer row: dst row (columns 0..FS_W-1) from three source rows.
+// GLOBAL_OUT: store the stored columns to HBM instead of an LDS ring.
+template <bool GLOBAL_OUT>
+__device__ __forceinline__ void fs_layer(const uint16_t* const src[3], uint16_t* dst, const bf16x8 (&wa)[9], const float (&bias)[4], int wave,
+                                         int lane, bool row_in, int gx0, int Wd, int store_lo, int store_hi) {
+    const int g = lane >> 4, ln = lane & 15, mt = wave & 1;
+    f32x4 acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+    bf16x8 bcur[3], bnxt[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) bcur[q] = *reinterpret_cast<const bf16x8*>(src[0] + (size_t)(((wave >> 1) + 2 * q) * 16 + ln) * C + 8 * g);
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        if (t < 8) {
+            const int ky = (t + 1) / 3, kx = (t + 1) % 3;
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                bnxt[q] = *reinterpret_cast<const bf16x8*>(src[ky] + (size_t)(((wave >> 1) + 2 * q) * 16 + ln + kx) * C + 8 * g);
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t], bcur[q], acc[q], 0, 0, 0);
+        if (t < 8) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) bcur[q] = bnxt[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int j = ((wave >> 1) + 2 * q) * 16 + ln;
+        const int gx = gx0 + j;
+        const bool in = row_in && gx >= 0 && gx < Wd;
+        const uint32_t keep = in ? 0xffffffffu : 0u;
+        uint2 o;
+        o.x = (f2bf(fmaxf(acc[q][0] + bias[0], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[q][1] + bias[1], 0.0f)) << 16)) & keep;
+        o.y = (f2bf(fmaxf(acc[q][2] + bias[2], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[q][3] + bias[3], 0.0f)) << 16)) & keep;
+        if (GLOBAL_OUT) {
+            if (in && j >= store_lo && j < store_hi) *reinterpret_cast<uint2*>(dst + (size_t)gx * C + mt * 16 + 4 * g) = o;
+        } else {
+            *reinterpret_cast<uint2*>(dst + (size_t)(j + 1) * C + mt * 16 + 4 * g) = o;
+        }
+    }
+}
+
+// PASS 0 (A): x -> a4 (act_out, n*H*W*32 bf16).  PASS 1 (B): a4 (act_in) -> y.
+template <int PASS>
+__global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict__ x, const uint16_t* __restrict__ act_in,
+                                                        const float* __restrict__ params, const uint16_t* __restrict__ wfwd,
+                                                        uint16_t* __restrict__ act_out, float* __restrict__ out, int H, int Wd) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t fs_lds[];
+    constexpr int NR = PASS == 0 ? 3 : 4;  // LDS rings
+    uint16_t* ring = reinterpret_cast<uint16_t*>(fs_lds);
+    float* fl = reinterpret_cast<float*>(fs_lds + NR * 3 * FS_ROW_BYTES);
+    float* xr = fl;                                        // A: [3][FS_RW]
+    float* w_in = fl + 3 * FS_RW;                          // A: W_in[32][9], b_in[32]
+    float* opart = fl;                                     // B: [9][FS_OUT]
+    float* w_out = fl + 9 * FS_OUT;                        // B: W_out[9][32]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4;
+    const int x0 = blockIdx.x * FS_OUT, y0 = blockIdx.y * FS_SH;
+    const size_t ioff = (size_t)blockIdx.z * H * Wd;
+    const int gx0 = x0 - FS_HALO;  // global column of computed column 0
+    auto RING = [&](int k, int row) { return ring + (size_t)(k * 3 + mod3(row)) * FS_RW * C; };
+
+    for (int i = tid; i < (int)(NR * 3 * FS_ROW_BYTES / 16); i += FS_NT) reinterpret_cast<uint4*>(fs_lds)[i] = make_uint4(0, 0, 0, 0);
+    if (PASS == 0) {
+        for (int i = tid; i < C * 9 + C; i += FS_NT) w_in[i] = params[CNN_OFF_IN + i];
+    } else {
+        for (int i = tid; i < 9 * C; i += FS_NT) w_out[i] = params[CNN_OFF_OUT + i];
+    }
+    const float b_out = params[CNN_OFF_OUT + 9 * C];
+    const int l0 = PASS == 0 ? 0 : 3;  // first middle layer of the pass
+    const int mt = wave & 1;
+    bf16x8 wa[3][9];
+    float bm[3][4];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+            wa[l][t] = *reinterpret_cast<const bf16x8*>(wfwd + (size_t)(l0 + l) * C * 9 * C + ((mt * 16 + (lane & 15)) * 9 + t) * C + 8 * g);
+#pragma unroll
+        for (int i = 0; i < 4; i++) bm[l][i] = params[CNN_OFF_MID + (size_t)(l0 + l) * CNN_MID_STRIDE + C * 9 * C + mt * 16 + 4 * g + i];
+    }
+
+    // input rows: A reads x rows (FS_RW floats), B reads a4 rows (FS_RW pixels of 64 bytes)
+    // row i of the input ring holds global column gx0 - 1 + i
+    constexpr int IN_UNITS = PASS == 0 ? FS_RW : FS_RW * 4;  // floats / 16-byte chunks per row
+    constexpr int IN_K = (IN_UNITS + FS_NT - 1) / FS_NT;
+    float xv[PASS == 0 ? IN_K : 1];
+    uint4 av[PASS == 0 ? 1 : IN_K];
+    auto fetch = [&](int row) {  // global -> registers
+#pragma unroll
+        for (int k = 0; k < IN_K; k++) {
+            const int u = tid + k * FS_NT;
+            if (PASS == 0) {
+                const int gx = gx0 - 1 + u;
+                xv[k] = (u < IN_UNITS && row >= 0 && row < H && gx >= 0 && gx < Wd) ? x[ioff + (size_t)row * Wd + gx] : 0.0f;
+            } else {
+                const int gx = gx0 - 1 + (u >> 2);
+                av[k] = (u < IN_UNITS && row >= 0 && row < H && gx >= 0 && gx < Wd)
+                            ? reinterpret_cast<const uint4*>(act_in + (ioff + (size_t)row * Wd + gx) * C)[u & 3]
+                            : make_uint4(0, 0, 0, 0);
+            }
+        }
+    };
+    auto commit = [&](int row) {  // registers -> input ring
+#pragma unroll
+        for (int k = 0; k < IN_K; k++) {
+            const int u = tid + k * FS_NT;
+            if (u < IN_UNITS) {
+                if (PASS == 0)
+                    xr[mod3(row) * FS_RW + u] = xv[k];
+                else
+                    reinterpret_cast<uint4*>(RING(0, row))[u] = av[k];
+            }
+        }
+    };
+    // step r consumes input rows r-1..r+1 and fetches r+2; the first step is r = y0 - 3
+    const int r_first = y0 - 3;
+    fetch(r_first - 1);
+    __syncthreads();  // ring zeroing done before any commit
+    commit(r_first - 1);
+    fetch(r_first);
+    commit(r_first);
+    fetch(r_first + 1);
+
+    for (int r = r_first; r < y0 + FS_SH + 4; r++) {
+        commit(r + 1);
+        fetch(r + 2);
+        __syncthreads();
+        if (PASS == 0) {
+            // ---- a1 row r (FP32, as k_conv_in) into ring 0
+            uint16_t* dst = RING(0, r);
+            const bool row_in = r >= 0 && r < H;
+            const float* b_in = w_in + C * 9;
+            for (int task = tid; task < FS_W * 4; task += FS_NT) {
+                const int j = task >> 2, cg = task & 3;
+                const int gx = gx0 + j;
+                const uint16_t keep = (row_in && gx >= 0 && gx < Wd) ? (uint16_t)0xffff : (uint16_t)0;
+                float v[9];
+#pragma unroll
+                for (int t = 0; t < 9; t++) v[t] = xr[mod3(r + t / 3 - 1) * FS_RW + j + t % 3];
+                uint16_t o[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int co = cg * 8 + q;
+                    float a = b_in[co];
+#pragma unroll
+                    for (int t = 0; t < 9; t++) a += w_in[co * 9 + t] * v[t];
+                    o[q] = f2bf(fmaxf(a, 0.0f)) & keep;
+                }
+                uint4 u;
+                u.x = o[0] | ((uint32_t)o[1] << 16);
+                u.y = o[2] | ((uint32_t)o[3] << 16);
+                u.z = o[4] | ((uint32_t)o[5] << 16);
+                u.w = o[6] | ((uint32_t)o[7] << 16);
+                reinterpret_cast<uint4*>(dst + (size_t)(j + 1) * C)[cg] = u;
+            }
+            __syncthreads();
+        }
+        // ---- three MFMA layers: ring k -> ring k+1 (A: rows r-1, r-2, r-3; B: rows r, r-1, r-2)
+        const int base = PASS == 0 ? r - 1 : r;
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+            const int orow = base - l;
+            const uint16_t* src[3] = {RING(l, orow - 1), RING(l, orow), RING(l, orow + 1)};
+            const bool row_in = orow >= 0 && orow < H;
+            if (PASS == 0 && l == 2) {
+                // a4: stored columns and rows of this block go to HBM
+                if (orow >= y0 && orow < y0 + FS_SH)
+                    fs_layer<true>(src, act_out + ioff * C + (size_t)orow * Wd * C, wa[l], bm[l], wave, lane, row_in, gx0, Wd, FS_HALO,
+                                   FS_HALO + FS_OUT);
+            } else {
+                fs_layer<false>(src, RING(l + 1, orow), wa[l], bm[l], wave, lane, row_in, gx0, Wd, 0, 0);
+                __syncthreads();
+            }
+        }
+        if (PASS == 1) {
+            // ---- output row r - 3 (as k_conv_out: per-tap partials, then bias + taps)
+            const int orow = r - 3;
+            for (int task = tid; task < 9 * FS_OUT; task += FS_NT) {
+                const int t = task / FS_OUT, jo = task % FS_OUT;
+                const int j = jo + FS_HALO + t % 3 - 1;
+                float v[C];
+                load_px32(RING(3, orow + t / 3 - 1) + (size_t)(j + 1) * C, v);
+                float part = 0.0f;
+#pragma unroll
+                for (int ci = 0; ci < C; ci++) part += w_out[t * C + ci] * v[ci];
+                opart[t * FS_OUT + jo] = part;
+            }
+            __syncthreads();
+            if (tid < FS_OUT && orow >= y0 && orow < y0 + FS_SH && orow < H) {
+                const int gx = x0 + tid;
+                if (gx < Wd) {
+                    float acc = b_out;
+#pragma unroll
+                    for (int t = 0; t < 9; t++) acc += opart[t * FS_OUT + tid];
+                    const size_t p = ioff + (size_t)orow * Wd + gx;
+                    out[p] = x[p] + acc;
+                }
+            }
+        }
+    }
+}
+
 unsigned nb(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -513,6 +748,23 @@ hipError_t cnn_forward(const float* params, const uint16_t* wfwd, const float* x
     }
     hipLaunchKernelGGL(k_conv_out, dim3(nb(npx)), dim3(256), 0, st, acts[6], x, params + CNN_OFF_OUT, params + CNN_OFF_OUT + 9 * C, out, H, W,
                        n_img);
+    return hipGetLastError();
+}
+
+size_t cnn_fused_act_bytes(int n_img, int H, int W) { return (size_t)n_img * H * W * C * sizeof(uint16_t); }
+
+hipError_t cnn_forward_fused(const float* params, const uint16_t* wfwd, const float* x, int n_img, int H, int W, uint16_t* act, float* out,
+                             hipStream_t st) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_cnn_fused<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)FS_LDS_A);
+        if (e == hipSuccess) e = hipFuncSetAttribute((const void*)k_cnn_fused<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)FS_LDS_B);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    dim3 grid((unsigned)((W + FS_OUT - 1) / FS_OUT), (unsigned)((H + FS_SH - 1) / FS_SH), (unsigned)n_img);
+    hipLaunchKernelGGL(k_cnn_fused<0>, grid, dim3(FS_NT), FS_LDS_A, st, x, (const uint16_t*)nullptr, params, wfwd, act, (float*)nullptr, H, W);
+    hipLaunchKernelGGL(k_cnn_fused<1>, grid, dim3(FS_NT), FS_LDS_B, st, x, (const uint16_t*)act, params, wfwd, (uint16_t*)nullptr, out, H, W);
     return hipGetLastError();
 }
 

@@ -182,6 +182,9 @@ int ds_cnn_load(ds_cnn* net, const char* path);
  * ds_cnn_loss_grad: loss = mean over pixels at least `margin` from the
  * image border of (y - t)^2, and its gradient (ds_cnn_param_count floats). */
 int ds_cnn_forward(ds_cnn* net, const float* x, size_t n_img, size_t h, size_t w, float* y_out);
+/* Inference forward with all layers fused in one kernel (activations stay
+ * in LDS); the same result as ds_cnn_forward, bit for bit. */
+int ds_cnn_infer(ds_cnn* net, const float* x, size_t n_img, size_t h, size_t w, float* y_out);
 int ds_cnn_loss_grad(ds_cnn* net, const float* x, const float* t, size_t n_img, size_t h, size_t w, size_t margin, double* loss,
                      float* grad);
 /* One Adam step (lr, beta 0.9 / 0.999, eps 1e-8) on a device batch; with

@@ -74,6 +74,19 @@ def test_forward_matches_oracle(net, oracle_mod, shape):
     assert np.sqrt((err ** 2).mean()) <= 1e-3 * scale
 
 
+@pytest.mark.parametrize("shape", [(1, 64, 64), (2, 19, 70), (1, 300, 170), (3, 5, 7), (1, 1, 1), (1, 517, 81), (2, 260, 161)])
+def test_fused_forward_equals_layer_by_layer(net, shape):
+    """The fused inference kernel (one block per 80-column strip, rings of
+    rows in LDS) reproduces the layer-by-layer forward bit for bit, across
+    strip and row-block boundaries and image borders."""
+    say("cnn fused forward", shape)
+    net.set_params(_params(21))
+    x = np.random.default_rng(22).normal(0, 2, shape).astype(np.float32)
+    y = net.forward(x)
+    yf = net.infer(x)
+    assert np.array_equal(yf, y), np.abs(yf - y).max()
+
+
 @pytest.mark.parametrize("shape,margin", [((2, 19, 70), 2), ((1, 40, 64), 0)])
 def test_gradient_matches_oracle(net, oracle_mod, shape, margin):
     say("cnn gradient", shape, margin)
@@ -169,6 +182,7 @@ def test_denoise_plane_is_forward_plus_output_map(ctx, net):
     y = net.forward(x)[0]
     ref = L + s * (y - x[0])
     assert np.abs(lf - ref).max() <= 1e-6
+    assert np.array_equal(net.infer(x)[0], y)
 
 
 def _noisy_rgb(ctx, seed, w, h, iso):
