@@ -29,6 +29,9 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 NLM_FLOP_PER_PX = 441 * 12      # per (pixel, offset): diff 2, two window updates 4, scale 1, /h^2 1, exp 1, fma 2, add 1
 DEMOSAIC_BYTES_PER_PX = 18      # u16 in + f32 RGB out + f32 clip out
+# HBM bytes per pixel that k_nlm_tile moves per launch, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE runs calibrated on a 1 GiB copy (profiles/hbm_traffic_counters.md)
+NLM_TRAFFIC_BYTES_PER_PX = 14.09
 MHC_BYTES_PER_PX = 14
 
 
@@ -161,6 +164,128 @@ def learned_leg(ctx, args, raw, rgb, den, out, w, h, plain_ms):
     return res
 
 
+BATCH_SIZES = [(4256, 2848), (6000, 4000), (7008, 4672), (8256, 5504), (9504, 6336)]  # 12, 24, 33, 45, 61 MP
+
+
+def batched_leg(ctx, args):
+    """Batched export: frames cycling 12/24/33/45/61 MP, alternating 12/14-bit,
+    noise level seeded in [1, 16], all resident in HBM, through one captured
+    graph (tile shapes and branch count autotuned on the first call); plain
+    and learned denoise.  Timed over whole graph launches."""
+    import numpy as np
+    import darkslide_amd as D
+    L = D.lib()
+    rng = np.random.default_rng(0x5EED)
+    frames, bufs = [], []
+    total = 0
+    for i in range(args.batch_frames):
+        fw, fh = BATCH_SIZES[i % len(BATCH_SIZES)]
+        bits = 14 if i % 2 == 0 else 12
+        iso = float(np.exp(rng.uniform(0.0, np.log(16.0))))
+        raw, out = ctx.alloc(fw * fh * 2), ctx.alloc(fw * fh * 12)
+        ctx._check(L.ds_synth_bayer_dev(ctx.handle, 0x5EED0000 + i, fw, fh, bits, iso, D._pat(D.RGGB), raw.ptr))
+        black, white = (512.0, 16383.0) if bits == 14 else (128.0, 4095.0)
+        frames.append(D.Frame(raw.ptr.value, fw, fh, fw, (0, 1, 1, 2), black, (white,) * 3, (1.0, 1.0, 1.0), out.ptr.value))
+        bufs += [raw, out]
+        total += fw * fh
+    log(f"batched: {len(frames)} frames, {total / 1e6:.0f} MP")
+    res = {"frames": len(frames), "megapixels": round(total / 1e6, 1),
+           "sizes": "12/24/33/45/61 MP cycling, 12/14-bit alternating, noise 1-16x", "legs": {}}
+    net = None
+    if os.path.exists(args.checkpoint):
+        net = D.CNN(ctx, seed=1)
+        net.load(args.checkpoint)
+    legs = [("plain", D.DENOISE_NLM, None)] + ([("learned", D.DENOISE_LEARNED, net)] if net else [])
+    for name, mode, nn in legs:
+        kw = dict(denoise=mode, amount=0.0, search=10, patch=3, raw_details=True, rl_sigma=1.0, rl_iters=20, net=nn,
+                  threshold_db=args.threshold_db)
+        t0 = time.perf_counter()
+        info = ctx.pipeline_batch(frames, streams=0, autotune=True, **kw)  # capture (+ tuning) and one run
+        setup_s = time.perf_counter() - t0
+        ctx.sync()
+        times = []
+        for _ in range(args.batch_reps):
+            t0 = time.perf_counter()
+            ctx.pipeline_batch(frames, streams=0, autotune=True, info=False, **kw)
+            ctx.sync()
+            times.append(time.perf_counter() - t0)
+        t = statistics.median(times)
+        last = ctx.pipeline_batch(frames, streams=0, autotune=True, **kw)
+        entry = {"value": round(total / 1e6 / t, 2), "unit": "MP/s", "s_per_batch": round(t, 4), "reps": args.batch_reps,
+                 "branches": info["branches"], "graph_reused_on_rerun": bool(last["graph_reused"]),
+                 "first_call_s_incl_capture_and_tuning": round(setup_s, 3)}
+        if mode == D.DENOISE_LEARNED:
+            entry["fallback_regions"] = last["fallback_regions"]
+            entry["regions"] = last["regions"]
+        res["legs"][name] = entry
+        log(f"batched {name}: {entry['value']} MP/s ({t:.3f} s per batch, {info['branches']} branches)")
+    tuning = {}
+    for fw, fh in BATCH_SIZES:
+        tuning[f"{fw}x{fh}"] = ctx.tuning(fw, fh)
+    res["tile_variants_rcd_mhc_rl"] = tuning
+    if net:
+        net.close()
+    for b in bufs:
+        b.free()
+    return res
+
+
+def stage_errors(ctx, args):
+    """Max-abs and RMS error of every stage against the FP32 restatement of
+    the reference and against its FP64 twin, on a size x size crop of the
+    bench's synthetic scene: each stage's GPU input is also the oracle's
+    input.  The learned CNN's forward is compared with the oracle's
+    bf16-emulating FP64 forward."""
+    import numpy as np
+    import darkslide_amd as D
+    import oracle as O
+    O.build()
+    n = args.error_size
+    raw = D.synth_bayer(0x5EED0003, n, n, 14, 4.0)
+    out = {}
+
+    def err(name, gpu, ref32, ref64):
+        g = np.asarray(gpu, np.float64)
+        d32 = g - np.asarray(ref32, np.float64)
+        d64 = g - np.asarray(ref64, np.float64)
+        out[name] = {"max_abs_vs_fp32": float(np.abs(d32).max()), "rms_vs_fp32": float(np.sqrt(np.mean(d32 * d32))),
+                     "max_abs_vs_fp64": float(np.abs(d64).max()), "rms_vs_fp64": float(np.sqrt(np.mean(d64 * d64)))}
+
+    cfa32 = O.cfa_normalize_u16(raw, D.RGGB, 512.0, [16383.0] * 3)
+    cfa64 = O.cfa_normalize_u16(raw, D.RGGB, 512.0, [16383.0] * 3, dtype=np.float64)
+    rgb, clip = ctx.demosaic_u16(raw)
+    r32, c32 = O.demosaic(cfa32)
+    r64, c64 = O.demosaic(cfa64)
+    err("demosaic_rcd", rgb, r32, r64)
+    err("clip_plane", clip, c32, c64)
+    mhc, _ = ctx.demosaic_u16(raw, algo=D.ALGO_MHC)
+    err("demosaic_mhc", mhc, O.mhc_bayer(cfa32), O.mhc_bayer(cfa64))
+    lab = ctx.oklab_from_prophoto(rgb)
+    err("oklab", lab, O.oklab_from_prophoto(rgb), O.oklab_from_prophoto(rgb.astype(np.float64)))
+    Lp = np.ascontiguousarray(lab[..., 0])
+    sg = ctx.estimate_noise(Lp)
+    err("estimate_noise", [sg], [O.estimate_noise(Lp)], [O.estimate_noise(Lp.astype(np.float64))])
+    hp = np.float32(max(np.float32(sg), np.float32(1e-4))) * np.float32(0.6)
+    err("nlm", ctx.nlm(Lp, hp, 10, 3), O.nlm(Lp, hp, 10, 3), O.nlm(Lp.astype(np.float64), float(hp), 10, 3))
+    dn = ctx.denoise(rgb, 0.0, 10, 3)
+    err("denoise", dn, O.denoise(rgb, 0.0, 10, 3), O.denoise(rgb.astype(np.float64), 0.0, 10, 3))
+    rl = ctx.raw_details(dn, 1.0, 20)
+    err("raw_details", rl, O.raw_details(dn, 1.0, 20), O.raw_details(dn.astype(np.float64), 1.0, 20))
+    if os.path.exists(args.checkpoint):
+        net = D.CNN(ctx, seed=1)
+        net.load(args.checkpoint)
+        s = max(np.float32(sg), np.float32(1e-4))
+        x = O.cnn_input(Lp, float(s)).astype(np.float32)[None, :128, :128].copy()
+        y = net.infer(x)
+        p = net.params().astype(np.float64)
+        ref = O.cnn_forward(p, x, bf16=True)
+        err("cnn_forward_bf16_vs_fp64_emulation", y, ref, ref)
+        out["cnn_forward_bf16_vs_fp64_emulation"]["note"] = "crop 128x128; both columns against the FP64 forward with bf16 roundings"
+        net.close()
+    out["_input"] = f"{n}x{n} RGGB 14-bit synthetic frame, noise 4x; NLM search 10 patch 3 h=0.6 sigma; RL 1.0 x 20"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +302,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--threshold-db", type=float, default=0.3)
     ap.add_argument("--learned-sizes", default="6000x4000,8256x5504")
+    ap.add_argument("--batch-frames", type=int, default=48, help="frames of the batched-export leg (0: skip)")
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--no-errors", action="store_true", help="skip the per-stage error report against the oracle")
+    ap.add_argument("--error-size", type=int, default=384)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -259,13 +388,22 @@ def main():
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(nlm_tflops / PEAK_FP32_TFLOPS, 4),
-        "traffic": None,
+        "traffic": round(NLM_TRAFFIC_BYTES_PER_PX * n),
+        "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (calibrated on a 1 GiB copy); "
+                        "algorithmic minimum 8 B/px",
         "algorithmic": f"{NLM_FLOP_PER_PX} flop/px x {n} px per launch",
         "dominant_stage": dominant,
     }
     learned = None
     if not args.no_learned:
         learned = learned_leg(ctx, args, raw, rgb, den, out, w, h, ms_per_step)
+    batched = None
+    if args.batch_frames > 0:
+        batched = batched_leg(ctx, args)
+    errors = None
+    if not args.no_errors:
+        log("per-stage errors against the oracle ...")
+        errors = stage_errors(ctx, args)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -295,6 +433,8 @@ def main():
         "cpu_baseline": cpu,
         "stages_ms": stages,
         "learned": learned,
+        "batched": batched,
+        "stage_errors": errors,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
