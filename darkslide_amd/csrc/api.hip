@@ -483,7 +483,7 @@ int synth_dev(ds_ctx* c, uint64_t seed, size_t w, size_t h, int bits, float iso,
 struct ds_cnn {
     ds_ctx* ctx = nullptr;
     float* state = nullptr;    // params | grad | adam m | adam v, CNN_NPARAM floats each
-    uint16_t* wpk = nullptr;   // bf16 middle weights: forward layout | data-gradient layout
+    uint16_t* wpk = nullptr;   // bf16 weights of the MFMA convolutions (CNN_WPK values)
     long step = 0;
     uint64_t seed = 0;
     bool packed = false;
@@ -504,7 +504,7 @@ constexpr size_t CNN_ACT_BYTES = 32 * sizeof(uint16_t);  // per pixel
 int cnn_pack_now(ds_cnn* net) {
     ds_ctx* c = net->ctx;
     if (net->packed) return DS_OK;
-    int rc = run(c, "cnn_pack", [&] { return cnn_pack(net->params(), net->wpk, net->wpk + CNN_WBF16, c->stream); });
+    int rc = run(c, "cnn_pack", [&] { return cnn_pack(net->params(), net->wpk, c->stream); });
     if (rc == DS_OK) net->packed = true;
     return rc;
 }
@@ -578,7 +578,7 @@ int cnn_grad_dev(ds_cnn* net, int n, int H, int W, int margin) {
     for (int k = 0; k < 7; k++) acts[k] = net->tr_act + (size_t)k * npx * 32;
     uint16_t* gacts[2] = {net->tr_act + 7 * npx * 32, net->tr_act + 8 * npx * 32};
     return run(c, "cnn_backward", [&] {
-        return cnn_backward(net->params(), net->wpk + CNN_WBF16, x, tgt, n, H, W, margin, acts, out, gacts, scratch, net->grad(), nullptr,
+        return cnn_backward(net->params(), net->wpk, x, tgt, n, H, W, margin, acts, out, gacts, scratch, net->grad(), nullptr,
                             c->stream);
     });
 }
@@ -606,7 +606,7 @@ int cnn_plane_dev(ds_ctx* c, ds_cnn* net, const float* L, size_t w, size_t h, co
     const size_t n = w * h;
     if (int rc = cnn_check_shape(c, 1, h, w)) return rc;
     if (!net->packed) {
-        int rc = run(c, "cnn_pack", [&] { return cnn_pack(net->params(), net->wpk, net->wpk + CNN_WBF16, c->stream); });
+        int rc = run(c, "cnn_pack", [&] { return cnn_pack(net->params(), net->wpk, c->stream); });
         if (rc) return rc;
         net->packed = true;
     }
@@ -1040,7 +1040,7 @@ int capture_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline
     }
     retire_stale_graphs(c);
     if (o.denoise == DS_DENOISE_LEARNED && !o.net->packed) {
-        int rc = run(c, "cnn_pack", [&] { return cnn_pack(o.net->params(), o.net->wpk, o.net->wpk + CNN_WBF16, c->stream); });
+        int rc = run(c, "cnn_pack", [&] { return cnn_pack(o.net->params(), o.net->wpk, c->stream); });
         if (rc) return rc;
         o.net->packed = true;
     }
@@ -1194,7 +1194,7 @@ int pipeline_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipelin
         }
     }
     if (o.denoise == DS_DENOISE_LEARNED && !o.net->packed) {
-        int rc = run(c, "cnn_pack", [&] { return cnn_pack(o.net->params(), o.net->wpk, o.net->wpk + CNN_WBF16, c->stream); });
+        int rc = run(c, "cnn_pack", [&] { return cnn_pack(o.net->params(), o.net->wpk, c->stream); });
         if (rc) return rc;
         o.net->packed = true;
     }
@@ -1654,7 +1654,7 @@ int ds_cnn_create(ds_ctx* c, uint64_t seed, ds_cnn** out) {
     net->ctx = c;
     net->seed = seed;
     hipError_t e = hipMalloc(&net->state, 4 * CNN_NPARAM * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&net->wpk, 2 * CNN_WBF16 * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(&net->wpk, CNN_WPK * sizeof(uint16_t));
     if (e != hipSuccess) {
         if (net->state) hipFree(net->state);
         delete net;

@@ -44,43 +44,49 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
 
-// ---------------------------------------------------------------- input / output layers (VALU, FP32)
+// ---------------------------------------------------------------- input / output layers (MFMA)
 
-// a1[p][co] = relu(b[co] + sum_tap W[co][tap] * x[p + tap])
-__global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, const float* __restrict__ W, const float* __restrict__ b,
-                                                 uint16_t* __restrict__ act, int H, int Wd, int n_img) {
-    __shared__ float sw[C * 9 + C];
-    for (int i = threadIdx.x; i < C * 9 + C; i += blockDim.x) sw[i] = i < C * 9 ? W[i] : b[i - C * 9];
+// Input layer: a1[co][px] = relu(b_in[co] + sum_tap W_in[co][tap] x[px + tap]),
+// one v_mfma_f32_16x16x32_bf16 per 16 x 16 tile with K = 32 (taps 0..8, the
+// rest zero); x is rounded to bf16 as the MFMA's B operand.  Tile 64 x 4, one
+// row per wave; x tile with a 1-pixel halo in LDS (zero outside the image).
+__global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, const uint16_t* __restrict__ win, const float* __restrict__ b_in,
+                                                 uint16_t* __restrict__ act, int H, int Wd) {
+    constexpr int TW = 64, TH = 4;
+    __shared__ float s_x[(TH + 2) * (TW + 2)];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, ln = lane & 15;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const size_t ioff = (size_t)blockIdx.z * H * Wd;
+    for (int i = tid; i < (TH + 2) * (TW + 2); i += 256) {
+        const int gy = y0 + i / (TW + 2) - 1, gx = x0 + i % (TW + 2) - 1;
+        s_x[i] = (gy >= 0 && gy < H && gx >= 0 && gx < Wd) ? x[ioff + (size_t)gy * Wd + gx] : 0.0f;
+    }
+    bf16x8 wa[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) wa[mt] = *reinterpret_cast<const bf16x8*>(win + (mt * 16 + ln) * 32 + 8 * g);
     __syncthreads();
-    const size_t npx = (size_t)H * Wd;
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npx * n_img) return;
-    const size_t img = p / npx, q = p % npx;
-    const int y = (int)(q / Wd), xx = (int)(q % Wd);
-    const float* xi = x + img * npx;
-    float v[9];
+    const int oy = y0 + wave;
 #pragma unroll
-    for (int t = 0; t < 9; t++) {
-        const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
-        v[t] = (yy >= 0 && yy < H && xc >= 0 && xc < Wd) ? xi[(size_t)yy * Wd + xc] : 0.0f;
-    }
-    uint16_t o[C];
+    for (int nt = 0; nt < TW / 16; nt++) {
+        bf16x8 bv;
 #pragma unroll
-    for (int co = 0; co < C; co++) {
-        float a = sw[C * 9 + co];
+        for (int e = 0; e < 8; e++) {
+            const int t = 8 * g + e;
+            const float v = t < 9 ? s_x[(wave + t / 3) * (TW + 2) + nt * 16 + ln + t % 3] : 0.0f;
+            bv[e] = (__bf16)v;
+        }
+        const int ox = x0 + nt * 16 + ln;
 #pragma unroll
-        for (int t = 0; t < 9; t++) a += sw[co * 9 + t] * v[t];
-        o[co] = f2bf(fmaxf(a, 0.0f));
-    }
-    uint4* dst = reinterpret_cast<uint4*>(act + p * C);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        uint4 u;
-        u.x = o[8 * k] | ((uint32_t)o[8 * k + 1] << 16);
-        u.y = o[8 * k + 2] | ((uint32_t)o[8 * k + 3] << 16);
-        u.z = o[8 * k + 4] | ((uint32_t)o[8 * k + 5] << 16);
-        u.w = o[8 * k + 6] | ((uint32_t)o[8 * k + 7] << 16);
-        dst[k] = u;
+        for (int mt = 0; mt < 2; mt++) {
+            const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[mt], bv, f32x4{0, 0, 0, 0}, 0, 0, 0);
+            if (oy < H && ox < Wd) {
+                const int co = mt * 16 + 4 * g;
+                uint2 o;
+                o.x = f2bf(fmaxf(acc[0] + b_in[co], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[1] + b_in[co + 1], 0.0f)) << 16);
+                o.y = f2bf(fmaxf(acc[2] + b_in[co + 2], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[3] + b_in[co + 3], 0.0f)) << 16);
+                *reinterpret_cast<uint2*>(act + (ioff + (size_t)oy * Wd + ox) * C + co) = o;
+            }
+        }
     }
 }
 
@@ -98,32 +104,61 @@ __device__ __forceinline__ void load_px32(const uint16_t* p, float* v) {
     }
 }
 
-// out[p] = x[p] + b + sum_{tap, ci} W[tap][ci] * a7[p + tap][ci]
-__global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ act, const float* __restrict__ x, const float* __restrict__ W,
-                                                  const float* __restrict__ b, float* __restrict__ out, int H, int Wd, int n_img) {
-    __shared__ float sw[9 * C + 1];
-    for (int i = threadIdx.x; i < 9 * C + 1; i += blockDim.x) sw[i] = i < 9 * C ? W[i] : b[0];
-    __syncthreads();
-    const size_t npx = (size_t)H * Wd;
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npx * n_img) return;
-    const size_t img = p / npx, q = p % npx;
-    const int y = (int)(q / Wd), xx = (int)(q % Wd);
-    // per-tap partial sums over ci, then bias + taps in order (the fused
-    // forward adds in the same order)
-    float acc = sw[9 * C];
-    for (int t = 0; t < 9; t++) {
-        const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
-        float part = 0.0f;
-        if (yy >= 0 && yy < H && xc >= 0 && xc < Wd) {
-            float v[C];
-            load_px32(act + (img * npx + (size_t)yy * Wd + xc) * C, v);
+// Output layer: y[px] = x[px] + (b_out + sum_{tap, ci} W_out[tap][ci] a7[px + tap][ci]),
+// nine MFMAs (one per tap, K = 32 channels) with W_out in row 0 of the A
+// operand; tile and a7 staging as k_conv_mid.
+constexpr int CP_ = 40;
+__global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ act, const float* __restrict__ x, const uint16_t* __restrict__ wout,
+                                                  const float* __restrict__ b_out, float* __restrict__ out, int H, int Wd) {
+    constexpr int TW = 64, TH = 4;
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[(TH + 2) * (TW + 2) * CP_];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, ln = lane & 15;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const size_t ioff = (size_t)blockIdx.z * H * Wd;
+    {
+        constexpr int N16 = (TH + 2) * (TW + 2) * (C / 8);
+        constexpr int K = (N16 + 255) / 256;
+        uint4 r[K];
 #pragma unroll
-            for (int ci = 0; ci < C; ci++) part += sw[t * C + ci] * v[ci];
+        for (int k = 0; k < K; k++) {
+            const int i = tid + k * 256;
+            const int pix = i >> 2, part = i & 3;
+            const int gy = y0 + pix / (TW + 2) - 1, gx = x0 + pix % (TW + 2) - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (i < N16 && gy >= 0 && gy < H && gx >= 0 && gx < Wd) v = reinterpret_cast<const uint4*>(act + (ioff + (size_t)gy * Wd + gx) * C)[part];
+            r[k] = v;
         }
-        acc += part;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = tid + k * 256;
+            if (i < N16) reinterpret_cast<uint4*>(s_in + (i >> 2) * CP_)[i & 3] = r[k];
+        }
     }
-    out[p] = x[p] + acc;
+    bf16x8 wa[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        bf16x8 z = {};
+        if (ln == 0) z = *reinterpret_cast<const bf16x8*>(wout + t * C + 8 * g);
+        wa[t] = z;
+    }
+    const float bo = b_out[0];
+    __syncthreads();
+    const int oy = y0 + wave;
+#pragma unroll
+    for (int nt = 0; nt < TW / 16; nt++) {
+        f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            const int ky = t / 3, kx = t % 3;
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(s_in + ((wave + ky) * (TW + 2) + nt * 16 + ln + kx) * CP_ + 8 * g);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t], bv, acc, 0, 0, 0);
+        }
+        const int ox = x0 + nt * 16 + ln;
+        if (g == 0 && oy < H && ox < Wd) {
+            const size_t p = ioff + (size_t)oy * Wd + ox;
+            out[p] = x[p] + (acc[0] + bo);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- 32 -> 32 conv on MFMA
@@ -335,10 +370,10 @@ __global__ void k_mse(const float* __restrict__ out, const float* __restrict__ t
 }
 
 // g7[p][ci] = (a7[p][ci] > 0) * sum_tap W[tap][ci] * dout[p - tap]
-__global__ __launch_bounds__(256) void k_out_dgrad(const float* __restrict__ dout, const float* __restrict__ W, const uint16_t* __restrict__ a7,
+__global__ __launch_bounds__(256) void k_out_dgrad(const float* __restrict__ dout, const uint16_t* __restrict__ W, const uint16_t* __restrict__ a7,
                                                    uint16_t* __restrict__ g7, int H, int Wd, int n_img) {
     __shared__ float sw[9 * C];
-    for (int i = threadIdx.x; i < 9 * C; i += blockDim.x) sw[i] = W[i];
+    for (int i = threadIdx.x; i < 9 * C; i += blockDim.x) sw[i] = bf2f(W[i]);  // the forward's bf16 weights
     __syncthreads();
     const size_t npx = (size_t)H * Wd;
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -409,7 +444,7 @@ __global__ void k_io_wgrad(const float* __restrict__ dout, const uint16_t* __res
                 const size_t img = p / npx, q = p % npx;
                 const int y = (int)(q / Wd) + dy, xx = (int)(q % Wd) + dx;
                 if (y < 0 || y >= H || xx < 0 || xx >= Wd) continue;
-                s += bf2f(g1[p * C + co]) * x[img * npx + (size_t)y * Wd + xx];
+                s += bf2f(g1[p * C + co]) * bf2f(f2bf(x[img * npx + (size_t)y * Wd + xx]));  // the forward's bf16 x
             }
         }
     }
@@ -428,6 +463,19 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ gr, floa
     m[i] = mi;
     v[i] = vi;
     w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// bf16 copies of the input and output layers: W_in as a [co][k] A operand
+// (k = tap, zero for k >= 9), W_out as [tap][ci].
+__global__ void k_pack_io(const float* __restrict__ params, uint16_t* __restrict__ win, uint16_t* __restrict__ wout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < C * 32) {
+        const int co = i / 32, k = i % 32;
+        win[i] = k < 9 ? f2bf(params[CNN_OFF_IN + co * 9 + k]) : (uint16_t)0;
+    } else if (i < C * 32 + 9 * C) {
+        const int j = i - C * 32;
+        wout[j] = f2bf(params[CNN_OFF_OUT + j]);
+    }
 }
 
 // bf16 copies of the six middle layers: forward layout [co][tap][ci] and the
@@ -514,10 +562,18 @@ constexpr int FS_SH = 192;
 constexpr int FS_NT = 256;
 constexpr size_t FS_ROW_BYTES = (size_t)FS_RW * C * sizeof(uint16_t);
 // pass A: rings a1..a3, x ring, W_in/b_in; pass B: rings a4..a7, output partials, W_out
-constexpr size_t FS_LDS_A = 3 * 3 * FS_ROW_BYTES + (3 * FS_RW + C * 9 + C) * sizeof(float);
-constexpr size_t FS_LDS_B = 4 * 3 * FS_ROW_BYTES + (9 * FS_OUT + 9 * C) * sizeof(float);
+constexpr size_t FS_LDS_A = 3 * 3 * FS_ROW_BYTES + 3 * FS_RW * sizeof(float);  // rings a1..a3, x ring
+constexpr size_t FS_LDS_B = 4 * 3 * FS_ROW_BYTES;                              // rings a4..a7
 
 __device__ __forceinline__ int mod3(int r) { return ((r % 3) + 3) % 3; }
+
+// Ring rows store a pixel's four 16-byte channel chunks XOR-swizzled by
+// (pixel >> 1) & 3, so that the eight lanes of a ds_read_b128 group, which
+// read the same chunk of eight consecutive pixels, hit eight different bank
+// groups (64-byte pixels would otherwise put every other pixel on the same
+// banks).
+__device__ __forceinline__ int fs_sw(int pr) { return (pr >> 1) & 3; }
+__device__ __forceinline__ int fs_chunk(int pr, int c) { return pr * 4 + (c ^ fs_sw(pr)); }  // uint4 index in a ring row
 
 // One MFMA layer row: dst row (columns 0..FS_W-1) from three source rows.
 // GLOBAL_OUT: store the stored columns to HBM instead of an LDS ring.
@@ -528,14 +584,14 @@ __device__ __forceinline__ void fs_layer(const uint16_t* const src[3], uint16_t*
     f32x4 acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
     bf16x8 bcur[3], bnxt[3];
 #pragma unroll
-    for (int q = 0; q < 3; q++) bcur[q] = *reinterpret_cast<const bf16x8*>(src[0] + (size_t)(((wave >> 1) + 2 * q) * 16 + ln) * C + 8 * g);
+    for (int q = 0; q < 3; q++) bcur[q] = reinterpret_cast<const bf16x8*>(src[0])[fs_chunk(((wave >> 1) + 2 * q) * 16 + ln, g)];
 #pragma unroll
     for (int t = 0; t < 9; t++) {
         if (t < 8) {
             const int ky = (t + 1) / 3, kx = (t + 1) % 3;
 #pragma unroll
             for (int q = 0; q < 3; q++)
-                bnxt[q] = *reinterpret_cast<const bf16x8*>(src[ky] + (size_t)(((wave >> 1) + 2 * q) * 16 + ln + kx) * C + 8 * g);
+                bnxt[q] = reinterpret_cast<const bf16x8*>(src[ky])[fs_chunk(((wave >> 1) + 2 * q) * 16 + ln + kx, g)];
         }
 #pragma unroll
         for (int q = 0; q < 3; q++) acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t], bcur[q], acc[q], 0, 0, 0);
@@ -556,7 +612,7 @@ __device__ __forceinline__ void fs_layer(const uint16_t* const src[3], uint16_t*
         if (GLOBAL_OUT) {
             if (in && j >= store_lo && j < store_hi) *reinterpret_cast<uint2*>(dst + (size_t)gx * C + mt * 16 + 4 * g) = o;
         } else {
-            *reinterpret_cast<uint2*>(dst + (size_t)(j + 1) * C + mt * 16 + 4 * g) = o;
+            *reinterpret_cast<uint2*>(dst + (size_t)fs_chunk(j + 1, 2 * mt + (g >> 1)) * 8 + 4 * (g & 1)) = o;
         }
     }
 }
@@ -564,27 +620,32 @@ __device__ __forceinline__ void fs_layer(const uint16_t* const src[3], uint16_t*
 // PASS 0 (A): x -> a4 (act_out, n*H*W*32 bf16).  PASS 1 (B): a4 (act_in) -> y.
 template <int PASS>
 __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict__ x, const uint16_t* __restrict__ act_in,
-                                                        const float* __restrict__ params, const uint16_t* __restrict__ wfwd,
+                                                        const float* __restrict__ params, const uint16_t* __restrict__ wpk,
                                                         uint16_t* __restrict__ act_out, float* __restrict__ out, int H, int Wd) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fs_lds[];
     constexpr int NR = PASS == 0 ? 3 : 4;  // LDS rings
     uint16_t* ring = reinterpret_cast<uint16_t*>(fs_lds);
-    float* fl = reinterpret_cast<float*>(fs_lds + NR * 3 * FS_ROW_BYTES);
-    float* xr = fl;                                        // A: [3][FS_RW]
-    float* w_in = fl + 3 * FS_RW;                          // A: W_in[32][9], b_in[32]
-    float* opart = fl;                                     // B: [9][FS_OUT]
-    float* w_out = fl + 9 * FS_OUT;                        // B: W_out[9][32]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4;
+    float* xr = reinterpret_cast<float*>(fs_lds + NR * 3 * FS_ROW_BYTES);  // A: x ring [3][FS_RW]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, ln = lane & 15;
     const int x0 = blockIdx.x * FS_OUT, y0 = blockIdx.y * FS_SH;
     const size_t ioff = (size_t)blockIdx.z * H * Wd;
     const int gx0 = x0 - FS_HALO;  // global column of computed column 0
     auto RING = [&](int k, int row) { return ring + (size_t)(k * 3 + mod3(row)) * FS_RW * C; };
 
     for (int i = tid; i < (int)(NR * 3 * FS_ROW_BYTES / 16); i += FS_NT) reinterpret_cast<uint4*>(fs_lds)[i] = make_uint4(0, 0, 0, 0);
+    // input / output layer operands (as k_conv_in / k_conv_out)
+    bf16x8 win = {}, wout[9];
+    float bin[4] = {0, 0, 0, 0};
     if (PASS == 0) {
-        for (int i = tid; i < C * 9 + C; i += FS_NT) w_in[i] = params[CNN_OFF_IN + i];
-    } else {
-        for (int i = tid; i < 9 * C; i += FS_NT) w_out[i] = params[CNN_OFF_OUT + i];
+        win = *reinterpret_cast<const bf16x8*>(wpk + CNN_WPK_IN + ((wave & 1) * 16 + ln) * 32 + 8 * g);
+#pragma unroll
+        for (int i = 0; i < 4; i++) bin[i] = params[CNN_OFF_IN + C * 9 + (wave & 1) * 16 + 4 * g + i];
+    }
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        bf16x8 z = {};
+        if (PASS == 1 && ln == 0) z = *reinterpret_cast<const bf16x8*>(wpk + CNN_WPK_OUT + t * C + 8 * g);
+        wout[t] = z;
     }
     const float b_out = params[CNN_OFF_OUT + 9 * C];
     const int l0 = PASS == 0 ? 0 : 3;  // first middle layer of the pass
@@ -595,7 +656,7 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     for (int l = 0; l < 3; l++) {
 #pragma unroll
         for (int t = 0; t < 9; t++)
-            wa[l][t] = *reinterpret_cast<const bf16x8*>(wfwd + (size_t)(l0 + l) * C * 9 * C + ((mt * 16 + (lane & 15)) * 9 + t) * C + 8 * g);
+            wa[l][t] = *reinterpret_cast<const bf16x8*>(wpk + (size_t)(l0 + l) * C * 9 * C + ((mt * 16 + (lane & 15)) * 9 + t) * C + 8 * g);
 #pragma unroll
         for (int i = 0; i < 4; i++) bm[l][i] = params[CNN_OFF_MID + (size_t)(l0 + l) * CNN_MID_STRIDE + C * 9 * C + mt * 16 + 4 * g + i];
     }
@@ -629,7 +690,7 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
                 if (PASS == 0)
                     xr[mod3(row) * FS_RW + u] = xv[k];
                 else
-                    reinterpret_cast<uint4*>(RING(0, row))[u] = av[k];
+                    reinterpret_cast<uint4*>(RING(0, row))[fs_chunk(u >> 2, u & 3)] = av[k];
             }
         }
     };
@@ -647,32 +708,26 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         fetch(r + 2);
         __syncthreads();
         if (PASS == 0) {
-            // ---- a1 row r (FP32, as k_conv_in) into ring 0
+            // ---- a1 row r into ring 0 (one MFMA per tile, K = taps; as k_conv_in)
             uint16_t* dst = RING(0, r);
             const bool row_in = r >= 0 && r < H;
-            const float* b_in = w_in + C * 9;
-            for (int task = tid; task < FS_W * 4; task += FS_NT) {
-                const int j = task >> 2, cg = task & 3;
-                const int gx = gx0 + j;
-                const uint16_t keep = (row_in && gx >= 0 && gx < Wd) ? (uint16_t)0xffff : (uint16_t)0;
-                float v[9];
 #pragma unroll
-                for (int t = 0; t < 9; t++) v[t] = xr[mod3(r + t / 3 - 1) * FS_RW + j + t % 3];
-                uint16_t o[8];
+            for (int q = 0; q < 3; q++) {
+                const int j = ((wave >> 1) + 2 * q) * 16 + ln;
+                bf16x8 bv;
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    const int co = cg * 8 + q;
-                    float a = b_in[co];
-#pragma unroll
-                    for (int t = 0; t < 9; t++) a += w_in[co * 9 + t] * v[t];
-                    o[q] = f2bf(fmaxf(a, 0.0f)) & keep;
+                for (int e = 0; e < 8; e++) {
+                    const int t = 8 * g + e;
+                    const float v = t < 9 ? xr[mod3(r + t / 3 - 1) * FS_RW + j + t % 3] : 0.0f;
+                    bv[e] = (__bf16)v;
                 }
-                uint4 u;
-                u.x = o[0] | ((uint32_t)o[1] << 16);
-                u.y = o[2] | ((uint32_t)o[3] << 16);
-                u.z = o[4] | ((uint32_t)o[5] << 16);
-                u.w = o[6] | ((uint32_t)o[7] << 16);
-                reinterpret_cast<uint4*>(dst + (size_t)(j + 1) * C)[cg] = u;
+                const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(win, bv, f32x4{0, 0, 0, 0}, 0, 0, 0);
+                const int gx = gx0 + j;
+                const uint32_t keep = (row_in && gx >= 0 && gx < Wd) ? 0xffffffffu : 0u;
+                uint2 o;
+                o.x = (f2bf(fmaxf(acc[0] + bin[0], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[1] + bin[1], 0.0f)) << 16)) & keep;
+                o.y = (f2bf(fmaxf(acc[2] + bin[2], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[3] + bin[3], 0.0f)) << 16)) & keep;
+                *reinterpret_cast<uint2*>(dst + (size_t)fs_chunk(j + 1, 2 * (wave & 1) + (g >> 1)) * 8 + 4 * (g & 1)) = o;
             }
             __syncthreads();
         }
@@ -694,27 +749,25 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
             }
         }
         if (PASS == 1) {
-            // ---- output row r - 3 (as k_conv_out: per-tap partials, then bias + taps)
+            // ---- output row r - 3: W_out in row 0 of the A operand, one MFMA per tap (as k_conv_out)
             const int orow = r - 3;
-            for (int task = tid; task < 9 * FS_OUT; task += FS_NT) {
-                const int t = task / FS_OUT, jo = task % FS_OUT;
-                const int j = jo + FS_HALO + t % 3 - 1;
-                float v[C];
-                load_px32(RING(3, orow + t / 3 - 1) + (size_t)(j + 1) * C, v);
-                float part = 0.0f;
+            const uint16_t* src[3] = {RING(3, orow - 1), RING(3, orow), RING(3, orow + 1)};
+            const bool store_row = orow >= y0 && orow < y0 + FS_SH && orow < H;
 #pragma unroll
-                for (int ci = 0; ci < C; ci++) part += w_out[t * C + ci] * v[ci];
-                opart[t * FS_OUT + jo] = part;
-            }
-            __syncthreads();
-            if (tid < FS_OUT && orow >= y0 && orow < y0 + FS_SH && orow < H) {
-                const int gx = x0 + tid;
-                if (gx < Wd) {
-                    float acc = b_out;
+            for (int q = 0; q < 2; q++) {
+                const int nt = wave + 4 * q;
+                if (nt < FS_W / 16) {
+                    f32x4 acc = {0, 0, 0, 0};
 #pragma unroll
-                    for (int t = 0; t < 9; t++) acc += opart[t * FS_OUT + tid];
-                    const size_t p = ioff + (size_t)orow * Wd + gx;
-                    out[p] = x[p] + acc;
+                    for (int t = 0; t < 9; t++) {
+                        const bf16x8 bv = reinterpret_cast<const bf16x8*>(src[t / 3])[fs_chunk(nt * 16 + ln + t % 3, g)];
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wout[t], bv, acc, 0, 0, 0);
+                    }
+                    const int j = nt * 16 + ln, gx = gx0 + j;
+                    if (g == 0 && store_row && j >= FS_HALO && j < FS_HALO + FS_OUT && gx < Wd) {
+                        const size_t p = ioff + (size_t)orow * Wd + gx;
+                        out[p] = x[p] + (acc[0] + b_out);
+                    }
                 }
             }
         }
@@ -729,31 +782,30 @@ unsigned nb(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 size_t cnn_param_count() { return CNN_NPARAM; }
 
-hipError_t cnn_pack(const float* params, uint16_t* wfwd, uint16_t* wbwd, hipStream_t st) {
-    hipLaunchKernelGGL(k_pack_mid, dim3(nb(6 * C * 9 * C)), dim3(256), 0, st, params + CNN_OFF_MID, wfwd, wbwd);
+hipError_t cnn_pack(const float* params, uint16_t* wpk, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack_mid, dim3(nb(6 * C * 9 * C)), dim3(256), 0, st, params + CNN_OFF_MID, wpk, wpk + CNN_WPK_BWD);
+    hipLaunchKernelGGL(k_pack_io, dim3(nb(C * 32 + 9 * C)), dim3(256), 0, st, params, wpk + CNN_WPK_IN, wpk + CNN_WPK_OUT);
     return hipGetLastError();
 }
 
 // Forward of n_img images H x W: x (fp32) -> out (fp32); acts: 7 activation
 // buffers of n_img*H*W*32 bf16 (kept for the backward pass).
-hipError_t cnn_forward(const float* params, const uint16_t* wfwd, const float* x, int n_img, int H, int W, uint16_t* const* acts, float* out,
+hipError_t cnn_forward(const float* params, const uint16_t* wpk, const float* x, int n_img, int H, int W, uint16_t* const* acts, float* out,
                        hipStream_t st) {
-    const size_t npx = (size_t)n_img * H * W;
-    hipLaunchKernelGGL(k_conv_in, dim3(nb(npx)), dim3(256), 0, st, x, params + CNN_OFF_IN, params + CNN_OFF_IN + C * 9, acts[0], H, W, n_img);
     dim3 grid((unsigned)((W + TW - 1) / TW), (unsigned)((H + TH - 1) / TH), (unsigned)n_img);
+    hipLaunchKernelGGL(k_conv_in, grid, dim3(256), 0, st, x, wpk + CNN_WPK_IN, params + CNN_OFF_IN + C * 9, acts[0], H, W);
     for (int l = 0; l < 6; l++) {
         const float* bias = params + CNN_OFF_MID + (size_t)l * (C * 9 * C + C) + C * 9 * C;
-        hipLaunchKernelGGL(k_conv_mid<0>, grid, dim3(NT), 0, st, acts[l], wfwd + (size_t)l * C * 9 * C, bias, (const uint16_t*)nullptr, acts[l + 1],
+        hipLaunchKernelGGL(k_conv_mid<0>, grid, dim3(NT), 0, st, acts[l], wpk + (size_t)l * C * 9 * C, bias, (const uint16_t*)nullptr, acts[l + 1],
                            H, W);
     }
-    hipLaunchKernelGGL(k_conv_out, dim3(nb(npx)), dim3(256), 0, st, acts[6], x, params + CNN_OFF_OUT, params + CNN_OFF_OUT + 9 * C, out, H, W,
-                       n_img);
+    hipLaunchKernelGGL(k_conv_out, grid, dim3(256), 0, st, acts[6], x, wpk + CNN_WPK_OUT, params + CNN_OFF_OUT + 9 * C, out, H, W);
     return hipGetLastError();
 }
 
 size_t cnn_fused_act_bytes(int n_img, int H, int W) { return (size_t)n_img * H * W * C * sizeof(uint16_t); }
 
-hipError_t cnn_forward_fused(const float* params, const uint16_t* wfwd, const float* x, int n_img, int H, int W, uint16_t* act, float* out,
+hipError_t cnn_forward_fused(const float* params, const uint16_t* wpk, const float* x, int n_img, int H, int W, uint16_t* act, float* out,
                              hipStream_t st) {
     static bool attr_set = false;
     if (!attr_set) {
@@ -763,8 +815,8 @@ hipError_t cnn_forward_fused(const float* params, const uint16_t* wfwd, const fl
         attr_set = true;
     }
     dim3 grid((unsigned)((W + FS_OUT - 1) / FS_OUT), (unsigned)((H + FS_SH - 1) / FS_SH), (unsigned)n_img);
-    hipLaunchKernelGGL(k_cnn_fused<0>, grid, dim3(FS_NT), FS_LDS_A, st, x, (const uint16_t*)nullptr, params, wfwd, act, (float*)nullptr, H, W);
-    hipLaunchKernelGGL(k_cnn_fused<1>, grid, dim3(FS_NT), FS_LDS_B, st, x, (const uint16_t*)act, params, wfwd, (uint16_t*)nullptr, out, H, W);
+    hipLaunchKernelGGL(k_cnn_fused<0>, grid, dim3(FS_NT), FS_LDS_A, st, x, (const uint16_t*)nullptr, params, wpk, act, (float*)nullptr, H, W);
+    hipLaunchKernelGGL(k_cnn_fused<1>, grid, dim3(FS_NT), FS_LDS_B, st, x, (const uint16_t*)act, params, wpk, (uint16_t*)nullptr, out, H, W);
     return hipGetLastError();
 }
 
@@ -785,7 +837,7 @@ size_t cnn_scratch_floats(int n_img, int H, int W) {
 
 // Loss (MSE over the crop interior) and gradients of all parameters into grad.
 // gacts: 2 scratch activation-gradient buffers; scratch: cnn_scratch_floats.
-hipError_t cnn_backward(const float* params, const uint16_t* wbwd, const float* x, const float* tgt, int n_img, int H, int W, int margin,
+hipError_t cnn_backward(const float* params, const uint16_t* wpk, const float* x, const float* tgt, int n_img, int H, int W, int margin,
                         uint16_t* const* acts, const float* out, uint16_t* const* gacts, float* scratch, float* grad, float* loss_part_out,
                         hipStream_t st) {
     const size_t npx = (size_t)n_img * H * W;
@@ -796,7 +848,7 @@ hipError_t cnn_backward(const float* params, const uint16_t* wbwd, const float* 
     hipLaunchKernelGGL(k_mse, dim3(nb(npx)), dim3(256), 0, st, out, tgt, H, W, n_img, margin, 1.0f / (float)(nvalid > 0 ? nvalid : 1), dout,
                        loss_part_out ? loss_part_out : lossp);
     // a7 -> g7
-    hipLaunchKernelGGL(k_out_dgrad, dim3(nb(npx)), dim3(256), 0, st, dout, params + CNN_OFF_OUT, acts[6], gacts[0], H, W, n_img);
+    hipLaunchKernelGGL(k_out_dgrad, dim3(nb(npx)), dim3(256), 0, st, dout, wpk + CNN_WPK_OUT, acts[6], gacts[0], H, W, n_img);
     dim3 grid((unsigned)((W + TW - 1) / TW), (unsigned)((H + TH - 1) / TH), (unsigned)n_img);
     const int wblk = (int)cnn_wgrad_blocks(n_img, H, W);
     const int npar = C * 9 * C + C;
@@ -806,7 +858,7 @@ hipError_t cnn_backward(const float* params, const uint16_t* wbwd, const float* 
         hipLaunchKernelGGL(k_wgrad_mid, dim3(wblk), dim3(NT), 0, st, acts[l], gacts[cur], H, W, n_img, 16, part);
         hipLaunchKernelGGL(k_reduce_parts, dim3(nb(npar)), dim3(256), 0, st, part, wblk, npar, grad + CNN_OFF_MID + (size_t)l * npar, 0,
                            (float*)nullptr);
-        hipLaunchKernelGGL(k_conv_mid<1>, grid, dim3(NT), 0, st, gacts[cur], wbwd + (size_t)l * C * 9 * C, (const float*)nullptr, acts[l],
+        hipLaunchKernelGGL(k_conv_mid<1>, grid, dim3(NT), 0, st, gacts[cur], wpk + CNN_WPK_BWD + (size_t)l * C * 9 * C, (const float*)nullptr, acts[l],
                            gacts[cur ^ 1], H, W);
         cur ^= 1;
     }

@@ -62,10 +62,14 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
     constexpr int YW = TX + 2 * R, YH = TY + 2 * R;
     constexpr int MW = TX + 2 * R, MH = TY + 4 * R;
     constexpr int QW = TX, QH = TY + 2 * R;
-    __shared__ float s_est[EH * EW];
-    __shared__ float s_y[YH * YW];
-    __shared__ float s_tmp[MH * MW];
-    __shared__ float s_rat[YH * YW];
+    // LDS row strides: odd, so that lanes walking down consecutive rows (the
+    // horizontal passes) or along a row (the vertical passes) hit distinct banks
+    constexpr int ES = EW | 1, YS = YW | 1, MS = MW | 1, QS = QW | 1;
+    constexpr int TMP = (MH * MS > QH * QS) ? MH * MS : QH * QS;
+    __shared__ float s_est[EH * ES];
+    __shared__ float s_y[YH * YS];
+    __shared__ float s_tmp[TMP];
+    __shared__ float s_rat[YH * YS];
     const int tid = threadIdx.x;
     const int ox = blockIdx.x * TX, oy = blockIdx.y * TY;
     float kk[K];
@@ -77,30 +81,31 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
             int u = i % EW - 2 * R, v = i / EW - 2 * R;
             return est[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
         },
-        [&](int i, float v) { s_est[i] = v; });
+        [&](int i, float v) { s_est[(i / EW) * ES + i % EW] = v; });
     stage_tile<YH * YW, NT>(
         [&](int i) {
             int u = i % YW - R, v = i / YW - R;
             return yp[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
         },
-        [&](int i, float v) { s_y[i] = v; });
+        [&](int i, float v) { s_y[(i / YW) * YS + i % YW] = v; });
     __syncthreads();
-    auto E = [&](int u, int v) -> float { return s_est[(v + 2 * R) * EW + (u + 2 * R)]; };
+    auto E = [&](int u, int v) -> float { return s_est[(v + 2 * R) * ES + (u + 2 * R)]; };
     const bool interior = ox >= 2 * R && oy >= 2 * R && ox + TX + 2 * R <= w && oy + TY + 2 * R <= h;
 
     if (interior) {
         // Register-blocked passes: each task produces B consecutive outputs
         // from B + 2R staged samples.  Every output still accumulates its
-        // taps in order 0..2R with separate multiply and add.
+        // taps in order 0..2R with separate multiply and add.  Horizontal
+        // passes put consecutive lanes on consecutive rows.
         // horizontal blur of est: rows [-2R, TY+2R), cols [-R, TX+R)
         constexpr int H1R = (MW + B - 1) / B;
         for (int t = tid; t < MH * H1R; t += NT) {
-            const int v = t / H1R - 2 * R, u0 = (t % H1R) * B - R;
-            const float* src = s_est + (v + 2 * R) * EW + (u0 - R + 2 * R);
+            const int v = t % MH - 2 * R, u0 = (t / MH) * B - R;
+            const float* src = s_est + (v + 2 * R) * ES + (u0 - R + 2 * R);
             float x[B + 2 * R];
 #pragma unroll
             for (int i = 0; i < B + 2 * R; i++) x[i] = src[min(i, EW - 1 - (u0 - R + 2 * R))];
-            float* dst = s_tmp + (v + 2 * R) * MW + (u0 + R);
+            float* dst = s_tmp + (v + 2 * R) * MS + (u0 + R);
 #pragma unroll
             for (int j = 0; j < B; j++) {
                 float acc = 0.0f;
@@ -114,17 +119,17 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         constexpr int V1R = (YH + B - 1) / B;
         for (int t = tid; t < YW * V1R; t += NT) {
             const int u = t % YW - R, v0 = (t / YW) * B - R;
-            const float* src = s_tmp + (v0 - R + 2 * R) * MW + (u + R);
+            const float* src = s_tmp + (v0 - R + 2 * R) * MS + (u + R);
             float x[B + 2 * R];
 #pragma unroll
-            for (int i = 0; i < B + 2 * R; i++) x[i] = src[min(i, MH - 1 - (v0 - R + 2 * R)) * MW];
+            for (int i = 0; i < B + 2 * R; i++) x[i] = src[min(i, MH - 1 - (v0 - R + 2 * R)) * MS];
 #pragma unroll
             for (int j = 0; j < B; j++) {
                 float acc = 0.0f;
 #pragma unroll
                 for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
                 if (v0 + j < TY + R) {
-                    const int idx = (v0 + j + R) * YW + (u + R);
+                    const int idx = (v0 + j + R) * YS + (u + R);
                     s_rat[idx] = s_y[idx] / fmaxr(acc, 1e-6f);
                 }
             }
@@ -133,12 +138,12 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         // horizontal blur of ratio: rows [-R, TY+R), cols [0, TX)
         constexpr int H2R = QW / B;
         for (int t = tid; t < QH * H2R; t += NT) {
-            const int v = t / H2R - R, u0 = (t % H2R) * B;
-            const float* src = s_rat + (v + R) * YW + u0;
+            const int v = t % QH - R, u0 = (t / QH) * B;
+            const float* src = s_rat + (v + R) * YS + u0;
             float x[B + 2 * R];
 #pragma unroll
             for (int i = 0; i < B + 2 * R; i++) x[i] = src[i];
-            float* dst = s_tmp + (v + R) * QW + u0;
+            float* dst = s_tmp + (v + R) * QS + u0;
 #pragma unroll
             for (int j = 0; j < B; j++) {
                 float acc = 0.0f;
@@ -152,10 +157,10 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         constexpr int V2R = TY / B;
         for (int t = tid; t < QW * V2R; t += NT) {
             const int u = t % QW, v0 = (t / QW) * B;
-            const float* src = s_tmp + v0 * QW + u;
+            const float* src = s_tmp + v0 * QS + u;
             float x[B + 2 * R];
 #pragma unroll
-            for (int i = 0; i < B + 2 * R; i++) x[i] = src[i * QW];
+            for (int i = 0; i < B + 2 * R; i++) x[i] = src[i * QS];
 #pragma unroll
             for (int j = 0; j < B; j++) {
                 float acc = 0.0f;
@@ -178,7 +183,7 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         float acc = 0.0f;
 #pragma unroll
         for (int j = 0; j < K; j++) acc += kk[j] * E(cu + j - R, v);
-        s_tmp[i] = acc;
+        s_tmp[(v + 2 * R) * MS + (u + R)] = acc;
     }
     __syncthreads();
     for (int i = tid; i < YH * YW; i += NT) {
@@ -186,8 +191,9 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         int cv = cv_of(v);
         float acc = 0.0f;
 #pragma unroll
-        for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(cv + j - R + 2 * R) * MW + (u + R)];
-        s_rat[i] = s_y[i] / fmaxr(acc, 1e-6f);
+        for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(cv + j - R + 2 * R) * MS + (u + R)];
+        const int idx = (v + R) * YS + (u + R);
+        s_rat[idx] = s_y[idx] / fmaxr(acc, 1e-6f);
     }
     __syncthreads();
     for (int i = tid; i < QH * QW; i += NT) {
@@ -195,8 +201,8 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         int cu = cu_of(u);
         float acc = 0.0f;
 #pragma unroll
-        for (int j = 0; j < K; j++) acc += kk[j] * s_rat[(v + R) * YW + (cu + j - R + R)];
-        s_tmp[i] = acc;
+        for (int j = 0; j < K; j++) acc += kk[j] * s_rat[(v + R) * YS + (cu + j - R + R)];
+        s_tmp[(v + R) * QS + u] = acc;
     }
     __syncthreads();
     for (int i = tid; i < TX * TY; i += NT) {
@@ -205,7 +211,7 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         if (gx >= w || gy >= h) continue;
         float acc = 0.0f;
 #pragma unroll
-        for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(v + j - R + R) * QW + u];
+        for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(v + j - R + R) * QS + u];
         est_out[(size_t)gy * w + gx] = clampr(E(u, v) * acc, 0.0f, 64.0f);
     }
 }

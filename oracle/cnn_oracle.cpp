@@ -14,10 +14,11 @@
 // Loss: mean over the pixels at least `margin` from the crop border of
 // (out - tgt)^2.
 //
-// With bf16 != 0 the forward rounds what the GPU stores in bfloat16 (the six
-// middle layers' weights and every activation a1..a7) to the nearest bf16,
-// ties to even; everything else is FP64.  The backward is the exact FP64
-// gradient of that forward, with the roundings taken as identities.
+// With bf16 != 0 the forward rounds what the GPU feeds its BF16 MFMAs to the
+// nearest bf16, ties to even: every layer's weights, the input layer's x, and
+// every activation a1..a7; biases, the residual x and all sums stay FP64.
+// The backward is the exact FP64 gradient of that forward, with the
+// roundings taken as identities.
 
 #include <cmath>
 #include <cstddef>
@@ -46,6 +47,7 @@ struct Net {
     int n, H, W;
     bool bf;
     std::vector<double> wmid;  // (possibly rounded) middle weights, 6 x [co][tap][ci]
+    std::vector<double> win, wout;  // (possibly rounded) input / output layer weights
     const double* p;
     size_t npx() const { return (size_t)n * H * W; }
     size_t at(int img, int y, int x) const { return ((size_t)img * H + y) * W + x; }
@@ -87,7 +89,8 @@ void forward(Net& N, const float* x, std::vector<std::vector<double>>& acts, dou
                     for (int t = 0; t < 9; t++) {
                         const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
                         if (yy < 0 || yy >= N.H || xc < 0 || xc >= N.W) continue;
-                        s += p[OFF_IN + co * 9 + t] * (double)x[N.at(img, yy, xc)];
+                        const double xv = (double)x[N.at(img, yy, xc)];
+                        s += N.win[co * 9 + t] * (N.bf ? bf16_round(xv) : xv);
                     }
                     s = s > 0.0 ? s : 0.0;
                     ap[co] = N.bf ? bf16_round(s) : s;
@@ -108,7 +111,7 @@ void forward(Net& N, const float* x, std::vector<std::vector<double>>& acts, dou
                     const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
                     if (yy < 0 || yy >= N.H || xc < 0 || xc >= N.W) continue;
                     const double* ap = &acts[6][N.at(img, yy, xc) * C];
-                    for (int ci = 0; ci < C; ci++) s += p[OFF_OUT + t * C + ci] * ap[ci];
+                    for (int ci = 0; ci < C; ci++) s += N.wout[t * C + ci] * ap[ci];
                 }
                 const size_t q = N.at(img, y, xx);
                 out[q] = (double)x[q] + s;
@@ -121,6 +124,10 @@ void init_net(Net& N, const double* params, int n, int H, int W, int bf16) {
     N.W = W;
     N.bf = bf16 != 0;
     N.p = params;
+    N.win.resize(C * 9);
+    N.wout.resize(9 * C);
+    for (int i = 0; i < C * 9; i++) N.win[i] = N.bf ? bf16_round(params[OFF_IN + i]) : params[OFF_IN + i];
+    for (int i = 0; i < 9 * C; i++) N.wout[i] = N.bf ? bf16_round(params[OFF_OUT + i]) : params[OFF_OUT + i];
     N.wmid.resize((size_t)6 * C * 9 * C);
     for (int l = 0; l < 6; l++)
         for (size_t i = 0; i < (size_t)C * 9 * C; i++) {
@@ -168,7 +175,6 @@ double or_cnn_loss_grad_f64(const double* params, const float* x, const float* t
     loss *= inv_n;
     if (!grad) return loss;
     std::memset(grad, 0, NPARAM * sizeof(double));
-    const double* p = params;
     // output layer
     std::vector<double> g(np * C, 0.0);  // d loss / d a7
     for (int img = 0; img < n; img++)
@@ -183,7 +189,7 @@ double or_cnn_loss_grad_f64(const double* params, const float* x, const float* t
                     const size_t q = N.at(img, yy, xc) * C;
                     for (int ci = 0; ci < C; ci++) {
                         grad[OFF_OUT + t * C + ci] += d * acts[6][q + ci];
-                        g[q + ci] += d * p[OFF_OUT + t * C + ci];
+                        g[q + ci] += d * N.wout[t * C + ci];
                     }
                 }
             }
@@ -234,7 +240,8 @@ double or_cnn_loss_grad_f64(const double* params, const float* x, const float* t
                     for (int t = 0; t < 9; t++) {
                         const int yy = y + t / 3 - 1, xc = xx + t % 3 - 1;
                         if (yy < 0 || yy >= H || xc < 0 || xc >= W) continue;
-                        grad[OFF_IN + co * 9 + t] += gp[co] * (double)x[N.at(img, yy, xc)];
+                        const double xv = (double)x[N.at(img, yy, xc)];
+                        grad[OFF_IN + co * 9 + t] += gp[co] * (N.bf ? bf16_round(xv) : xv);
                     }
                 }
             }

@@ -6,7 +6,8 @@ independent library routines and by calculus:
   * gradient against torch autograd in FP64, and against central finite
     differences of the oracle's own loss;
   * the bf16 rounding against torch's float32 -> bfloat16 conversion;
-  * the bf16-emulating forward against torch with the same roundings;
+  * the bf16-emulating forward against torch with the same roundings
+    (weights, the input layer's x, activations);
   * Adam against torch.optim.Adam;
   * the network input against a numpy restatement with edge padding.
 """
@@ -40,7 +41,8 @@ def _torch_forward(P, X, bf16=False):
     def rnd(t):
         return t.to(torch.float32).to(torch.bfloat16).to(torch.float64) if bf16 else t
 
-    a = rnd(torch.relu(F.conv2d(X[:, None], P[0:288].view(32, 1, 3, 3), P[288:320], padding=1)))
+    Win = P[0:288].view(32, 1, 3, 3)
+    a = rnd(torch.relu(F.conv2d(rnd(X)[:, None], rnd(Win), P[288:320], padding=1)))
     o = 320
     for _ in range(6):
         W = P[o:o + 9216].view(32, 3, 3, 32).permute(0, 3, 1, 2).contiguous()
@@ -49,7 +51,7 @@ def _torch_forward(P, X, bf16=False):
         a = rnd(torch.relu(F.conv2d(a, W, P[o + 9216:o + 9248], padding=1)))
         o += 9248
     Wout = P[o:o + 288].view(3, 3, 32).permute(2, 0, 1)[None].contiguous()
-    return X + F.conv2d(a, Wout, P[o + 288:o + 289], padding=1)[:, 0]
+    return X + F.conv2d(a, rnd(Wout), P[o + 288:o + 289], padding=1)[:, 0]
 
 
 def test_param_count():
