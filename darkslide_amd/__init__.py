@@ -57,7 +57,7 @@ class PipelineOpts(ctypes.Structure):
     _fields_ = [("algo", ctypes.c_int), ("denoise", ctypes.c_int), ("amount", ctypes.c_float), ("search", ctypes.c_long),
                 ("patch", ctypes.c_size_t), ("raw_details", ctypes.c_int), ("rl_sigma", ctypes.c_float), ("rl_iters", ctypes.c_size_t),
                 ("reference_gating", ctypes.c_int), ("net", ctypes.c_void_p), ("threshold_db", ctypes.c_float),
-                ("streams", ctypes.c_int), ("autotune", ctypes.c_int)]
+                ("streams", ctypes.c_int), ("autotune", ctypes.c_int), ("highlights", ctypes.c_int)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -102,6 +102,12 @@ _SIGS = {
     "ds_last_error": [_P],
     "ds_version": [],
     "ds_demosaic_f32": [_P, _P, _SZ, _SZ, _P, _P, _I, _P, _P],
+    "ds_cfa_clip_levels_f32": [_P, _P, _SZ, _SZ, _P, _P],
+    "ds_cfa_clip_levels_u16": [_P, _P, _SZ, _SZ, _SZ, _P, _F, _P, _P],
+    "ds_reconstruct_highlights": [_P, _P, _P, _SZ, _SZ, _P, _P],
+    "ds_cfa_clip_levels_f32_dev": [_P, _P, _SZ, _SZ, _P, _P],
+    "ds_cfa_clip_levels_u16_dev": [_P, _P, _SZ, _SZ, _SZ, _P, _F, _P, _P],
+    "ds_reconstruct_highlights_dev": [_P, _P, _P, _SZ, _SZ, _P, _P],
     "ds_demosaic_u16": [_P, _P, _SZ, _SZ, _SZ, _P, _F, _P, _P, _I, _P, _P],
     "ds_estimate_noise": [_P, _P, _SZ, _SZ, _P],
     "ds_nlm": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _P],
@@ -291,6 +297,32 @@ class Context:
                                           _ptr(rgb), _ptr(cl)))
         return (rgb, cl) if clip else rgb
 
+    def cfa_clip_levels(self, cfa, pattern=RGGB):
+        """demosaic.rs:419 cfa_clip_levels on a normalised CFA -> [3] float32."""
+        cfa = np.ascontiguousarray(cfa, np.float32)
+        h, w = cfa.shape
+        out = np.zeros(3, np.float32)
+        self._check(lib().ds_cfa_clip_levels_f32(self.handle, _ptr(cfa), w, h, _pat(pattern), _ptr(out)))
+        return out
+
+    def cfa_clip_levels_u16(self, raw, pattern=RGGB, black=512.0, white=(16383.0,) * 3):
+        """cfa_clip_levels of raw DN normalised as decode.rs:291-306 -> [3] float32."""
+        raw = np.ascontiguousarray(raw, np.uint16)
+        h, w = raw.shape
+        out = np.zeros(3, np.float32)
+        self._check(lib().ds_cfa_clip_levels_u16(self.handle, _ptr(raw), w, h, w, _pat(pattern), float(black), _f3(white), _ptr(out)))
+        return out
+
+    def reconstruct_highlights(self, rgb, clip, mul, raw_clip):
+        """demosaic.rs:435 reconstruct_highlights -> new (h, w, 3) array (the input is not changed)."""
+        out = np.array(rgb, np.float32, order="C", copy=True)
+        clip = np.ascontiguousarray(clip, np.float32)
+        h, w, _ = out.shape
+        if clip.shape != (h, w):
+            raise ValueError("clip plane shape must be (h, w)")
+        self._check(lib().ds_reconstruct_highlights(self.handle, _ptr(out), _ptr(clip), w, h, _f3(mul), _f3(raw_clip)))
+        return out
+
     def estimate_noise(self, plane):
         plane = np.ascontiguousarray(plane, np.float32)
         h, w = plane.shape
@@ -361,11 +393,11 @@ class Context:
 
     def pipeline_batch(self, frames, algo=ALGO_RCD, denoise=DENOISE_NLM, amount=0.0, search=10, patch=3, raw_details=True,
                        rl_sigma=1.0, rl_iters=20, reference_gating=False, net=None, threshold_db=0.3, streams=0, autotune=False,
-                       info=True):
+                       info=True, highlights=False):
         """Runs the batch graph over `frames` (a list of Frame); returns the batch info (or None with info=False)."""
         arr = (Frame * len(frames))(*frames)
         o = PipelineOpts(algo, denoise, amount, search, patch, int(raw_details), rl_sigma, rl_iters, int(reference_gating),
-                         net.handle.value if net is not None else None, threshold_db, streams, int(autotune))
+                         net.handle.value if net is not None else None, threshold_db, streams, int(autotune), int(highlights))
         bi = BatchInfo()
         self._check(lib().ds_pipeline_batch(self.handle, arr, len(frames), ctypes.byref(o), ctypes.byref(bi) if info else None))
         return bi.as_dict() if info else None

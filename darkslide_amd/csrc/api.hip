@@ -17,6 +17,7 @@
 #include "darkslide_gpu.h"
 #include "demosaic.h"
 #include "denoise.h"
+#include "highlights.h"
 #include "nlm.h"
 #include "box.h"
 #include "rl.h"
@@ -203,8 +204,12 @@ struct Buf {
 
 enum Slot {
     S_IN = 0, S_IN2, S_OUT, S_CLIP, S_WS, S_L, S_A, S_B, S_LF, S_SIGMA, S_NOISE, S_SCENE, S_X, S_T, S_REGIONS, S_ACT,
-    S_PIPE_RGB, S_PIPE_DEN, S_COUNTS, S_STATS, S_COUNT
+    S_PIPE_RGB, S_PIPE_DEN, S_COUNTS, S_STATS, S_HL, S_PIPE_CLIP, S_COUNT
 };
+
+// S_HL: clip levels (3 floats) at 0, cfa_clip_levels partials from HL_PART_OFF.
+constexpr size_t HL_PART_OFF = 64;
+constexpr size_t HL_FLOATS = HL_PART_OFF + 3 * CLIP_PARTS;
 
 // Tile-shape choice for one frame shape (variant indices of the tunable kernels).
 struct Tuned {
@@ -402,6 +407,37 @@ int demosaic_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, siz
     const int variant = algo == DS_ALGO_RCD ? t.rcd : t.mhc;
     return run(c, algo == DS_ALGO_RCD && w >= 16 && h >= 16 ? "demosaic_rcd" : "demosaic_mhc",
                [&] { return launch_demosaic(a, c->stream, variant); });
+}
+
+// cfa_clip_levels (demosaic.rs:419-430) into levels_dev (3 device floats).
+int clip_levels_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t* pat, float black,
+                    const float* white, float* levels_dev) {
+    if (!valid_bayer(pat)) return fail(c, DS_ERR_INVALID, "pattern must be a Bayer 2x2 with a green checkerboard");
+    if (!levels_dev || (!cfa && !raw)) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (stride < w) return fail(c, DS_ERR_INVALID, "stride %zu < width %zu", stride, w);
+    SLOT(c, hl, float*, S_HL, HL_FLOATS * 4);
+    DemosaicArgs a{};
+    a.cfa_f32 = cfa;
+    a.raw_u16 = raw;
+    a.stride = stride;
+    a.w = w;
+    a.h = h;
+    std::memcpy(a.pattern, pat, 4);
+    a.black = black;
+    for (int k = 0; k < 3; k++) {
+        a.white[k] = white ? white[k] : 1.0f;
+        a.mul[k] = 1.0f;
+    }
+    return run(c, "cfa_clip_levels", [&] { return launch_cfa_clip_levels(a, hl + HL_PART_OFF, levels_dev, c->stream); });
+}
+
+// reconstruct_highlights (demosaic.rs:435-502) in place; raw_clip on the device.
+int highlights_dev(ds_ctx* c, float* rgb, const float* clip, size_t w, size_t h, const float* mul, const float* raw_clip) {
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    if (!rgb || !clip || !mul || !raw_clip) return fail(c, DS_ERR_INVALID, "null pointer");
+    SLOT(c, ws, float*, S_WS, highlights_ws_floats(w, h) * 4);
+    return run(c, "reconstruct_highlights", [&] { return launch_reconstruct_highlights(rgb, clip, w, h, mul, raw_clip, ws, c->stream); });
 }
 
 int noise_dev(ds_ctx* c, const float* plane, size_t w, size_t h, float* sigma_dev) {
@@ -840,7 +876,7 @@ int autotune_shape(ds_ctx* c, size_t w, size_t h, ds_tuning* out) {
 
 struct FramePlan {
     size_t idx;
-    bool denoise, learned, rl;
+    bool highlights, denoise, learned, rl;
     float amount;
     float rl_sigma;
     size_t rl_iters;
@@ -849,6 +885,7 @@ struct FramePlan {
 FramePlan plan_frame(const ds_frame& f, size_t idx, const ds_pipeline_opts& o) {
     FramePlan p{};
     p.idx = idx;
+    p.highlights = o.highlights != 0;
     p.denoise = o.denoise != DS_DENOISE_NONE;
     p.learned = o.denoise == DS_DENOISE_LEARNED;
     p.amount = o.amount;
@@ -871,6 +908,11 @@ void frame_needs(const ds_frame& f, const FramePlan& p, size_t need[S_COUNT]) {
     const size_t n = f.w * f.h;
     auto up = [&](Slot s, size_t b) { need[s] = std::max(need[s], b); };
     if (p.denoise || p.rl) up(S_PIPE_RGB, n * 12);
+    if (p.highlights) {
+        up(S_PIPE_CLIP, n * 4);
+        up(S_HL, HL_FLOATS * 4);
+        up(S_WS, highlights_ws_floats(f.w, f.h) * 4);
+    }
     if (p.denoise && p.rl) up(S_PIPE_DEN, n * 12);
     if (p.denoise) {
         up(S_L, n * 4);
@@ -935,8 +977,19 @@ int enqueue_frame(ds_ctx* b, const ds_frame& f, const FramePlan& p, const ds_pip
         SLOT(b, t, float*, S_PIPE_RGB, f.w * f.h * 12);
         rgb = t;
     }
-    int rc = demosaic_dev(b, nullptr, f.raw, f.w, f.h, f.stride, f.pattern, f.black, f.white, f.mul, o.algo, rgb, nullptr);
+    float* clip = nullptr;
+    if (p.highlights) {
+        SLOT(b, t, float*, S_PIPE_CLIP, f.w * f.h * 4);
+        clip = t;
+    }
+    int rc = demosaic_dev(b, nullptr, f.raw, f.w, f.h, f.stride, f.pattern, f.black, f.white, f.mul, o.algo, rgb, clip);
     if (rc) return rc;
+    if (p.highlights) {
+        // Engine::base (pipeline/mod.rs:186-187): rebuild clipped channels at the clip levels of this frame
+        SLOT(b, hl, float*, S_HL, HL_FLOATS * 4);
+        if ((rc = clip_levels_dev(b, nullptr, f.raw, f.w, f.h, f.stride, f.pattern, f.black, f.white, hl))) return rc;
+        if ((rc = highlights_dev(b, rgb, clip, f.w, f.h, f.mul, hl))) return rc;
+    }
     const float* cur = rgb;
     if (p.denoise) {
         float* dst = f.rgb_out;
@@ -993,6 +1046,7 @@ std::vector<uint8_t> batch_key(ds_ctx* c, const ds_frame* frames, size_t n, cons
     put(&o.reference_gating, sizeof o.reference_gating);
     put(&o.net, sizeof o.net);
     put(&o.threshold_db, sizeof o.threshold_db);
+    put(&o.highlights, sizeof o.highlights);
     for (size_t i = 0; i < n; i++) {
         const ds_frame& f = frames[i];
         put(&f.raw, sizeof f.raw);
@@ -1407,6 +1461,26 @@ int ds_demosaic_f32_dev(ds_ctx* c, const float* cfa, size_t w, size_t h, const u
     return demosaic_dev(c, cfa, nullptr, w, h, w, pattern, 0.0f, nullptr, mul, algo, rgb_out, clip_out);
 }
 
+int ds_cfa_clip_levels_f32_dev(ds_ctx* c, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], float* levels_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    return clip_levels_dev(c, cfa, nullptr, w, h, w, pattern, 0.0f, nullptr, levels_out);
+}
+
+int ds_cfa_clip_levels_u16_dev(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
+                               const float white[3], float* levels_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!white) return fail(c, DS_ERR_INVALID, "white is null");
+    return clip_levels_dev(c, nullptr, raw, w, h, stride, pattern, black, white, levels_out);
+}
+
+int ds_reconstruct_highlights_dev(ds_ctx* c, float* rgb, const float* clip, size_t w, size_t h, const float mul[3], const float* raw_clip) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    return highlights_dev(c, rgb, clip, w, h, mul, raw_clip);
+}
+
 int ds_estimate_noise_dev(ds_ctx* c, const float* plane, size_t w, size_t h, float* sigma_out) {
     LOCK(c);
     if (int rc = check_dims(c, w, h)) return rc;
@@ -1478,6 +1552,54 @@ int ds_demosaic_u16(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, size_t s
     if (int rc = demosaic_dev(c, nullptr, d_raw, w, h, stride, pattern, black, white, mul, algo, d_rgb, d_clip)) return rc;
     HIPCHK(c, hipMemcpyAsync(rgb_out, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
     if (clip_out) HIPCHK(c, hipMemcpyAsync(clip_out, d_clip, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_cfa_clip_levels_f32(ds_ctx* c, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], float levels_out[3]) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!cfa || !levels_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t n = w * h;
+    SLOT(c, d_in, float*, S_IN, (n ? n : 1) * 4);
+    SLOT(c, d_lv, float*, S_SIGMA, 64);
+    if (n) HIPCHK(c, hipMemcpyAsync(d_in, cfa, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = clip_levels_dev(c, d_in, nullptr, w, h, w, pattern, 0.0f, nullptr, d_lv)) return rc;
+    HIPCHK(c, hipMemcpyAsync(levels_out, d_lv, 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_cfa_clip_levels_u16(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
+                           const float white[3], float levels_out[3]) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!raw || !levels_out || !white) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (stride < w) return fail(c, DS_ERR_INVALID, "stride %zu < width %zu", stride, w);
+    const size_t n = h ? (h - 1) * stride + w : 0;
+    SLOT(c, d_raw, uint16_t*, S_IN, (n ? n : 1) * 2);
+    SLOT(c, d_lv, float*, S_SIGMA, 64);
+    if (n) HIPCHK(c, hipMemcpyAsync(d_raw, raw, n * 2, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = clip_levels_dev(c, nullptr, d_raw, w, h, stride, pattern, black, white, d_lv)) return rc;
+    HIPCHK(c, hipMemcpyAsync(levels_out, d_lv, 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_reconstruct_highlights(ds_ctx* c, float* rgb, const float* clip, size_t w, size_t h, const float mul[3], const float raw_clip[3]) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!rgb || !clip || !mul || !raw_clip) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    SLOT(c, d_rgb, float*, S_OUT, n * 12);
+    SLOT(c, d_clip, float*, S_CLIP, n * 4);
+    SLOT(c, d_rc, float*, S_SIGMA, 64);
+    HIPCHK(c, hipMemcpyAsync(d_rgb, rgb, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    HIPCHK(c, hipMemcpyAsync(d_clip, clip, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    HIPCHK(c, hipMemcpyAsync(d_rc, raw_clip, 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = highlights_dev(c, d_rgb, d_clip, w, h, mul, d_rc)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;
 }

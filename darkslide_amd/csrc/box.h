@@ -9,6 +9,7 @@ constexpr int BOX_MAX_PLANES = 6;
 
 struct BoxPlanes {
     float* p[BOX_MAX_PLANES];
+    const int* gate = nullptr;  // when set, the pass does nothing while *gate == 0
 };
 struct CBoxPlanes {
     const float* p[BOX_MAX_PLANES];

@@ -43,6 +43,7 @@ constexpr int BX = 64;  // chunk length and rows / columns per wave
 template <class In>
 __global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_boxx_h(In in, BoxPlanes out, int w, int h, int r,
                                                                                           float norm) {
+    if (out.gate && *out.gate == 0) return;
     __shared__ float tile[2][BX][BX + 1];
     const int t = threadIdx.x;
     const int y0 = blockIdx.x * BX;
@@ -103,6 +104,7 @@ __global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 // the stores of a wave cover one contiguous row piece.
 __global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_boxx_v(CBoxPlanes in, BoxPlanes out, int w, int h, int r,
                                                                                           float norm) {
+    if (out.gate && *out.gate == 0) return;
     const int t = threadIdx.x;
     const int x = min(blockIdx.x * BX + t, w - 1);  // lanes past the edge shadow the last column
     const bool live = blockIdx.x * BX + t < w;
@@ -143,6 +145,7 @@ __global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 // Any radius: one thread per row / column reading global memory directly.
 template <class In>
 __global__ void k_boxx_h_any(In in, BoxPlanes out, int w, int h, int r, float norm, int planes) {
+    if (out.gate && *out.gate == 0) return;
     const int y = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (y >= h || k >= planes) return;
@@ -156,6 +159,7 @@ __global__ void k_boxx_h_any(In in, BoxPlanes out, int w, int h, int r, float no
 }
 
 __global__ void k_boxx_v_any(CBoxPlanes in, BoxPlanes out, int w, int h, int r, float norm) {
+    if (out.gate && *out.gate == 0) return;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (x >= w) return;

@@ -26,4 +26,9 @@ struct DemosaicArgs {
 int demosaic_variants(int algo);
 hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream, int variant = 0);
 
+// cfa_clip_levels (demosaic.rs:419-430) of a's CFA (rgb, clip and algo are
+// not used) into out3 (device, 3 floats); partial: CLIP_PARTS * 3 floats.
+constexpr int CLIP_PARTS = 256;
+hipError_t launch_cfa_clip_levels(const DemosaicArgs& a, float* partial, float* out3, hipStream_t stream);
+
 }  // namespace ds

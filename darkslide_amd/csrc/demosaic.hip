@@ -441,15 +441,7 @@ hipError_t launch_mhc(const CfaSrc& src, float* rgb, float* clip, hipStream_t st
     return hipGetLastError();
 }
 
-}  // namespace
-
-// Tile-shape variants (TX x TY, threads), all with the same per-pixel
-// arithmetic, so every variant gives the same result bit for bit.
-//   RCD: 0 = 64x32/512, 1 = 32x32/256, 2 = 64x16/256
-//   MHC: 0 = 64x16/256, 1 = 128x8/256, 2 = 32x32/256
-int demosaic_variants(int algo) { return algo == DS_ALGO_RCD ? 3 : 3; }
-
-hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream, int variant) {
+CfaSrc make_src(const DemosaicArgs& a) {
     CfaSrc src{};
     src.f32 = a.cfa_f32;
     src.u16 = a.raw_u16;
@@ -462,6 +454,78 @@ hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream, int varian
     for (int i = 0; i < 4; i++) src.pat.c[i] = a.pattern[i];
     src.w = (int)a.w;
     src.h = (int)a.h;
+    return src;
+}
+
+// cfa_clip_levels, first stage (demosaic.rs:419-427): block b takes sampled
+// rows 3*(b + k*gridDim.x) and leaves the per-colour max of the normalised
+// CFA in partial[3*b + c].  Starting from 0, fmaxf skips NaN exactly like
+// f32::max; max is order-free, so the two stages equal the serial loop.
+__global__ __launch_bounds__(256) void k_clip_partial(CfaSrc src, float* __restrict__ partial) {
+    float mx[3] = {0.0f, 0.0f, 0.0f};
+    const int rows = (src.h + 2) / 3;
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int y = 3 * r;
+        for (int x = threadIdx.x; x < src.w; x += 256) {
+            const int c = src.pat.at(x, y);
+            const float v = src.cfa(x, y, c);
+            if (c == 0) mx[0] = fmaxf(mx[0], v);
+            else if (c == 1) mx[1] = fmaxf(mx[1], v);
+            else mx[2] = fmaxf(mx[2], v);
+        }
+    }
+    __shared__ float red[3][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = 0; c < 3; c++) {
+        float v = mx[c];
+        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+        if (lane == 0) red[c][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int c = threadIdx.x;
+        partial[3 * blockIdx.x + c] = fmaxf(fmaxf(red[c][0], red[c][1]), fmaxf(red[c][2], red[c][3]));
+    }
+}
+
+// Second stage: a max is trusted as a clip level only at >= 0.95 of the
+// nominal white, else f32::MAX (demosaic.rs:428-429).
+__global__ __launch_bounds__(256) void k_clip_final(const float* __restrict__ partial, int parts, float* __restrict__ out3) {
+    __shared__ float red[3][256];
+    for (int c = 0; c < 3; c++) {
+        float v = 0.0f;
+        for (int i = threadIdx.x; i < parts; i += 256) v = fmaxf(v, partial[3 * i + c]);
+        red[c][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int c = 0; c < 3; c++) red[c][threadIdx.x] = fmaxf(red[c][threadIdx.x], red[c][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) {
+        const float m = red[threadIdx.x][0];
+        out3[threadIdx.x] = m >= 0.95f ? m : 3.40282347e+38f;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_cfa_clip_levels(const DemosaicArgs& a, float* partial, float* out3, hipStream_t stream) {
+    const CfaSrc src = make_src(a);
+    hipLaunchKernelGGL(k_clip_partial, dim3(CLIP_PARTS), dim3(256), 0, stream, src, partial);
+    hipLaunchKernelGGL(k_clip_final, dim3(1), dim3(256), 0, stream, partial, CLIP_PARTS, out3);
+    return hipGetLastError();
+}
+
+// Tile-shape variants (TX x TY, threads), all with the same per-pixel
+// arithmetic, so every variant gives the same result bit for bit.
+//   RCD: 0 = 64x32/512, 1 = 32x32/256, 2 = 64x16/256
+//   MHC: 0 = 64x16/256, 1 = 128x8/256, 2 = 32x32/256
+int demosaic_variants(int algo) { return algo == DS_ALGO_RCD ? 3 : 3; }
+
+hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream, int variant) {
+    const CfaSrc src = make_src(a);
     // rcd_bayer falls back to MHC below 16 pixels (demosaic.rs:149-151)
     bool rcd = a.algo == DS_ALGO_RCD && a.w >= 16 && a.h >= 16;
     if (rcd) {

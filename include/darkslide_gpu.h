@@ -69,6 +69,22 @@ int ds_demosaic_f32(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uin
 int ds_demosaic_u16(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
                     const float white[3], const float mul[3], int algo, float* rgb_out, float* clip_out);
 
+/* ---- highlight reconstruction ------------------------------------------ */
+
+/* demosaic.rs:419 cfa_clip_levels(cfa): per-colour max of the normalised
+ * CFA over every third row, kept as the clip level when >= 0.95, else
+ * FLT_MAX.  levels_out: 3 floats (host). */
+int ds_cfa_clip_levels_f32(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], float levels_out[3]);
+/* Same on raw DN, normalised as in ds_demosaic_u16. */
+int ds_cfa_clip_levels_u16(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
+                           const float white[3], float levels_out[3]);
+
+/* demosaic.rs:435 reconstruct_highlights(img, clip, mul, raw_clip), in
+ * place on rgb (w*h*3, white-balanced camera RGB from ds_demosaic_*); clip:
+ * the clip plane ds_demosaic_* returns; raw_clip: ds_cfa_clip_levels'
+ * result.  As called by Engine::base (pipeline/mod.rs:186-187). */
+int ds_reconstruct_highlights(ds_ctx* ctx, float* rgb, const float* clip, size_t w, size_t h, const float mul[3], const float raw_clip[3]);
+
 /* ---- enhance ----------------------------------------------------------- */
 
 /* enhance.rs:10 estimate_noise(p) */
@@ -121,6 +137,14 @@ int ds_demosaic_u16_dev(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, si
                         const float white[3], const float mul[3], int algo, float* rgb_out, float* clip_out);
 int ds_demosaic_f32_dev(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], int algo,
                         float* rgb_out, float* clip_out);
+/* levels_out: 3 device floats */
+int ds_cfa_clip_levels_f32_dev(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], float* levels_out);
+int ds_cfa_clip_levels_u16_dev(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4],
+                               float black, const float white[3], float* levels_out);
+/* rgb, clip, raw_clip (3 floats) on the device; mul on the host.  With no
+ * positive clip value nothing changes, decided on the device (no wait). */
+int ds_reconstruct_highlights_dev(ds_ctx* ctx, float* rgb, const float* clip, size_t w, size_t h, const float mul[3],
+                                  const float* raw_clip);
 /* sigma_out is a device pointer to one float */
 int ds_estimate_noise_dev(ds_ctx* ctx, const float* plane, size_t w, size_t h, float* sigma_out);
 int ds_nlm_dev(ds_ctx* ctx, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, float* out);
@@ -224,7 +248,8 @@ int ds_denoise_learned_dev(ds_ctx* ctx, ds_cnn* net, const float* rgb, size_t w,
 /* Stands in for the sequential per-file loops of the CLI and GUI batch
  * export (cli.rs:142-187, dialogs.rs:310-365), each of which runs
  * Engine::base_for (pipeline/mod.rs:229-252) per image: demosaic ->
- * [denoise] -> [raw_details].  Every frame of the batch goes through one
+ * [denoise] -> [raw_details], with Engine::base's highlight reconstruction
+ * (pipeline/mod.rs:186-187) after the demosaic when opts.highlights is set.  Every frame of the batch goes through one
  * hipGraph, captured on the first call and replayed while the frames
  * (pointers, shapes, formats) and options stay the same.  Frames are spread
  * over `streams` parallel graph branches (longest first), each with its own
@@ -261,6 +286,7 @@ typedef struct ds_pipeline_opts {
     float threshold_db;   /* learned: spot-check threshold */
     int streams;          /* graph branches, 1..DS_MAX_BRANCHES; 0 = 2, or tuned when autotune is set */
     int autotune;
+    int highlights;       /* 1: reconstruct_highlights after demosaic, as Engine::base does (pipeline/mod.rs:186-187) */
 } ds_pipeline_opts;
 
 typedef struct ds_batch_info {

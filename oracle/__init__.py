@@ -284,3 +284,25 @@ def auto_wb_stats(rgb, as_shot_neutral):
     f.restype = ctypes.c_int
     ok = f(_p(rgb), ctypes.c_size_t(npx), _p(nas), _p(acc), ctypes.byref(n), _p(neutral))
     return acc, n.value, (neutral if ok else None)
+
+
+def cfa_clip_levels(cfa, pattern=(0, 1, 1, 2)):
+    """demosaic.rs:419-430 -> 3 clip levels (f32::MAX where untrusted)."""
+    cfa = np.ascontiguousarray(cfa)
+    h, w = cfa.shape
+    out = np.empty(3, cfa.dtype)
+    f, _ = _fn("or_cfa_clip_levels", cfa.dtype)
+    f(_p(cfa), ctypes.c_size_t(w), ctypes.c_size_t(h), _pat(pattern), _p(out))
+    return out
+
+
+def reconstruct_highlights(img, clip, mul, raw_clip):
+    """demosaic.rs:435-502 on a copy of img (interleaved RGB)."""
+    img = np.array(img, copy=True, order="C")
+    clip = np.ascontiguousarray(clip, img.dtype)
+    h, w, _ = img.shape
+    f, _ = _fn("or_reconstruct_highlights", img.dtype)
+    m = np.asarray(mul, img.dtype)
+    rc = np.asarray(raw_clip, img.dtype)
+    f(_p(img), _p(clip), ctypes.c_size_t(w), ctypes.c_size_t(h), _p(m), _p(rc))
+    return img

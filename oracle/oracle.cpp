@@ -712,6 +712,84 @@ template <class T> void raw_details(const T* img, size_t w, size_t h, T sigma, s
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// ---- highlight reconstruction (demosaic.rs:418-502)
+
+template <class T> inline T f32max() { return (T)3.40282346638528859812e+38; }  // f32::MAX in either type
+
+// cfa_clip_levels (demosaic.rs:419-430): per-colour max over every third row,
+// trusted as a clip level only when >= 0.95.
+template <class T> void cfa_clip_levels(const T* cfa, size_t w, size_t h, const uint8_t* pat, T* out3) {
+    T mx[3] = {0, 0, 0};
+    for (size_t y = 0; y < h; y += 3)
+        for (size_t x = 0; x < w; x++) {
+            const int c = pat[((y & 1) << 1) | (x & 1)];
+            mx[c] = vmax(mx[c], cfa[y * w + x]);
+        }
+    for (int c = 0; c < 3; c++) out3[c] = mx[c] >= F(0.95) ? mx[c] : f32max<T>();
+}
+
+// reconstruct_highlights (demosaic.rs:435-502), in place on interleaved RGB.
+template <class T> void reconstruct_highlights(T* img, const T* clip, size_t w, size_t h, const T* mul, const T* raw_clip) {
+    const size_t n = w * h;
+    bool any = false;
+    for (size_t i = 0; i < n && !any; i++) any = clip[i] > (T)0;
+    if (!any) return;
+    T lvl[3];
+    for (int c = 0; c < 3; c++) lvl[c] = raw_clip[c] == f32max<T>() ? f32max<T>() : raw_clip[c] * mul[c] * F(0.97);
+    auto is_clipped = [&](const T* p, int c, size_t i) { return p[c] >= lvl[c] || (clip[i] > (T)0 && p[c] >= lvl[c] * F(0.8)); };
+    const T sigma = vmax((T)(w > h ? w : h) / F(120.0), F(2.0));
+    const T lmin = vmin(vmin(vmin(lvl[0], lvl[1]), lvl[2]), F(4.0));
+    std::vector<T> near_clip(n), wgt(n, (T)0), rr(n, (T)0), br(n, (T)0);
+    box_blur<T>(clip, w, h, 3, near_clip.data());
+#pragma omp parallel for schedule(static)
+    for (isz ii = 0; ii < (isz)n; ii++) {
+        const size_t i = (size_t)ii;
+        const T* p = img + 3 * i;
+        if (near_clip[i] > (T)0 || is_clipped(p, 0, i) || is_clipped(p, 1, i) || is_clipped(p, 2, i) || p[1] <= F(1e-4)) continue;
+        const T y = (p[0] + p[1] + p[2]) / F(3.0) / lmin;
+        const T k = vmin(y * y, F(1.0));
+        wgt[i] = k;
+        rr[i] = vclamp(p[0] / p[1], F(0.1), F(10.0)) * k;
+        br[i] = vclamp(p[2] / p[1], F(0.1), F(10.0)) * k;
+    }
+    std::vector<T> wb(n), rrb(n), brb(n);
+    gaussian_blur<T>(wgt.data(), w, h, sigma, wb.data());
+    gaussian_blur<T>(rr.data(), w, h, sigma, rrb.data());
+    gaussian_blur<T>(br.data(), w, h, sigma, brb.data());
+#pragma omp parallel for schedule(static)
+    for (isz ii = 0; ii < (isz)n; ii++) {
+        const size_t i = (size_t)ii;
+        T* p = img + 3 * i;
+        const bool cl[3] = {is_clipped(p, 0, i), is_clipped(p, 1, i), is_clipped(p, 2, i)};
+        const int nc = (int)cl[0] + (int)cl[1] + (int)cl[2];
+        if (nc == 0) continue;
+        const T m = vmax(vmax(p[0], p[1]), p[2]);
+        if (nc == 3) {
+            p[0] = p[1] = p[2] = m;
+            continue;
+        }
+        const T wv = wb[i];
+        T r = F(1.0), b = F(1.0);
+        if (wv > F(1e-4)) {
+            r = rrb[i] / wv;
+            b = brb[i] / wv;
+        }
+        const T trust = vmin(wv * F(3.0), F(1.0));
+        r = F(1.0) + (r - F(1.0)) * trust;
+        b = F(1.0) + (b - F(1.0)) * trust;
+        const T ratio[3] = {r, F(1.0), b};
+        T g = p[1];
+        for (int c = 0; c < 3; c++)
+            if (!cl[c]) g = vmax(g, p[c] / ratio[c]);
+        for (int c = 0; c < 3; c++)
+            if (cl[c]) p[c] = vmax(p[c], g * ratio[c]);
+        const T top = vmax(vmax(p[0], p[1]), p[2]);
+        const T q = (T)nc / F(3.0);
+        const T t = q * q;
+        for (int c = 0; c < 3; c++) p[c] += (top - p[c]) * t * F(0.5);
+    }
+}
+
 // C ABI
 
 #define OR_BOTH(X) X(float, f32) X(double, f64)
@@ -762,6 +840,12 @@ extern "C" {
     }                                                                                                                              \
     void or_raw_details_##S(const T* img, size_t w, size_t h, T sigma, size_t iters, T* out) {                                     \
         raw_details<T>(img, w, h, sigma, iters, out);                                                                              \
+    }                                                                                                                              \
+    void or_cfa_clip_levels_##S(const T* cfa, size_t w, size_t h, const uint8_t* pat, T* out3) {                                    \
+        cfa_clip_levels<T>(cfa, w, h, pat, out3);                                                                                  \
+    }                                                                                                                              \
+    void or_reconstruct_highlights_##S(T* img, const T* clip, size_t w, size_t h, const T* mul, const T* raw_clip) {              \
+        reconstruct_highlights<T>(img, clip, w, h, mul, raw_clip);                                                                 \
     }                                                                                                                              \
     void or_demosaic_##S(const T* cfa, size_t w, size_t h, const uint8_t* pat, const T* mul, T* rgb, T* clip) {                    \
         rcd_bayer<T>(cfa, w, h, pat, mul, rgb);                                                                                    \
