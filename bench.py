@@ -33,6 +33,31 @@ DEMOSAIC_BYTES_PER_PX = 18      # u16 in + f32 RGB out + f32 clip out
 # FETCH_SIZE / WRITE_SIZE runs calibrated on a 1 GiB copy (profiles/hbm_traffic_counters.md)
 NLM_TRAFFIC_BYTES_PER_PX = 14.09
 MHC_BYTES_PER_PX = 14
+# reconstruct_highlights, algorithmic HBM bytes per pixel when something is
+# clipped: flag scan 4, near-clip box 16, weights 32, three-box gaussian of
+# three planes 3 x 2 x 3 x 8 = 144, apply 40
+HIGHLIGHTS_BYTES_PER_PX = 236
+HL_MUL = (2.0, 1.0, 1.5)
+
+
+def saturate(raw, seed, discs=12):
+    """Plants saturated discs in an RGGB uint16 frame: every site, green
+    sites only, red sites only, in turn."""
+    import numpy as np
+    h, w = raw.shape
+    rng = np.random.default_rng(seed)
+    rad = max(3, min(w, h) // 16)
+    for k in range(discs):
+        cx, cy = int(rng.integers(0, w)), int(rng.integers(0, h))
+        x0, x1, y0, y1 = max(cx - rad, 0), min(cx + rad + 1, w), max(cy - rad, 0), min(cy + rad + 1, h)
+        yy, xx = np.mgrid[y0:y1, x0:x1]
+        m = (xx - cx) ** 2 + (yy - cy) ** 2 <= rad * rad
+        if k % 3 == 1:
+            m &= ((xx + yy) & 1) == 1  # green sites of RGGB
+        elif k % 3 == 2:
+            m &= ((xx & 1) == 0) & ((yy & 1) == 0)  # red sites
+        raw[y0:y1, x0:x1][m] = 16383
+    return raw
 
 
 def log(*a):
@@ -230,6 +255,55 @@ def batched_leg(ctx, args):
     return res
 
 
+def highlights_leg(ctx, args, raw, rgb, clip, w, h):
+    """Engine::base's highlight reconstruction (pipeline/mod.rs:186-187) on
+    the bench frame with saturated discs: RCD demosaic with the clip plane,
+    cfa_clip_levels, reconstruct_highlights; and the same on the unsaturated
+    frame, where the device flag skips the work."""
+    import numpy as np
+    import darkslide_amd as D
+    L = D.lib()
+    n = w * h
+    host = saturate(raw.download((h, w), np.uint16), 7)
+    sat = ctx.alloc(n * 2)
+    sat.upload(host)
+    lv = ctx.alloc(64)
+    pat = D._pat(D.RGGB)
+    white = D._f3((16383.0,) * 3)
+    mul = D._f3(HL_MUL)
+
+    def run(src):
+        ctx._check(L.ds_demosaic_u16_dev(ctx.handle, src.ptr, w, h, w, pat, 512.0, white, mul, D.ALGO_RCD, rgb.ptr, clip.ptr))
+        ctx._check(L.ds_cfa_clip_levels_u16_dev(ctx.handle, src.ptr, w, h, w, pat, 512.0, white, lv.ptr))
+        ctx._check(L.ds_reconstruct_highlights_dev(ctx.handle, rgb.ptr, clip.ptr, w, h, mul, lv.ptr))
+
+    res = {}
+    for name, src in (("saturated", sat), ("unsaturated", raw)):
+        run(src)
+        ctx.sync()
+        ctx.prof_enable(True)
+        ctx.prof_reset()
+        for _ in range(5):
+            run(src)
+        ctx.sync()
+        pr = ctx.prof()
+        ctx.prof_enable(False)
+        st = {k: round(v[0] / max(v[1], 1), 4) for k, v in pr.items() if k in ("cfa_clip_levels", "reconstruct_highlights")}
+        res[name] = {"stage_ms": st}
+        if name == "saturated":
+            cp = clip.download((h, w), np.float32)
+            res[name]["clipped_fraction"] = round(float((cp > 0).mean()), 5)
+            res[name]["levels"] = [float(v) for v in lv.download((3,), np.float32)]
+            ms = st["reconstruct_highlights"]
+            res[name]["hbm_gbs"] = round(HIGHLIGHTS_BYTES_PER_PX * n / (ms * 1e-3) / 1e9, 1)
+            res[name]["hbm_frac"] = round(res[name]["hbm_gbs"] / PEAK_HBM_GBS, 4)
+    res["algorithmic"] = f"{HIGHLIGHTS_BYTES_PER_PX} B/px when any sample is clipped; as-shot mul {list(HL_MUL)}"
+    sat.free()
+    lv.free()
+    log(f"highlights: {res}")
+    return res
+
+
 def stage_errors(ctx, args):
     """Max-abs and RMS error of every stage against the FP32 restatement of
     the reference and against its FP64 twin, on a size x size crop of the
@@ -271,6 +345,16 @@ def stage_errors(ctx, args):
     err("denoise", dn, O.denoise(rgb, 0.0, 10, 3), O.denoise(rgb.astype(np.float64), 0.0, 10, 3))
     rl = ctx.raw_details(dn, 1.0, 20)
     err("raw_details", rl, O.raw_details(dn, 1.0, 20), O.raw_details(dn.astype(np.float64), 1.0, 20))
+    sat = saturate(raw.copy(), 8, discs=6)
+    srgb, sclip = ctx.demosaic_u16(sat, mul=HL_MUL)
+    lv = ctx.cfa_clip_levels_u16(sat)
+    cfa_s = O.cfa_normalize_u16(sat, D.RGGB, 512.0, [16383.0] * 3)
+    err("cfa_clip_levels", lv, O.cfa_clip_levels(cfa_s), O.cfa_clip_levels(cfa_s.astype(np.float64)))
+    hl = ctx.reconstruct_highlights(srgb, sclip, HL_MUL, lv)
+    err("reconstruct_highlights", hl, O.reconstruct_highlights(srgb, sclip, HL_MUL, lv),
+        O.reconstruct_highlights(srgb.astype(np.float64), sclip.astype(np.float64), HL_MUL, lv.astype(np.float64)))
+    out["reconstruct_highlights"]["note"] = ("saturated discs; near-clip is box_blur(clip, 3) > 0, whose FP32 running sums "
+                                             "leave residues that FP64 does not, so FP64 classifies some pixels differently")
     if os.path.exists(args.checkpoint):
         net = D.CNN(ctx, seed=1)
         net.load(args.checkpoint)
@@ -397,6 +481,7 @@ def main():
     learned = None
     if not args.no_learned:
         learned = learned_leg(ctx, args, raw, rgb, den, out, w, h, ms_per_step)
+    highlights = highlights_leg(ctx, args, raw, rgb, clip, w, h)
     batched = None
     if args.batch_frames > 0:
         batched = batched_leg(ctx, args)
@@ -434,6 +519,7 @@ def main():
         "stages_ms": stages,
         "learned": learned,
         "batched": batched,
+        "highlights": highlights,
         "stage_errors": errors,
     }
     print(json.dumps(line), flush=True)

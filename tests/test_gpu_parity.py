@@ -230,6 +230,18 @@ def test_box_blur_exact(ctx, oracle_mod, r):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("w,h,r", [(336, 150, 1), (336, 150, 3), (336, 150, 23), (336, 150, 24), (336, 150, 31), (336, 150, 32),
+                                   (336, 150, 63), (336, 150, 64), (336, 150, 87), (336, 150, 88), (336, 150, 95), (336, 150, 96),
+                                   (4, 7, 2), (100, 3, 40), (64, 64, 5), (1000, 1, 3), (8, 500, 30), (1, 300, 7), (132, 260, 50)])
+def test_box_blur_ring_paths(ctx, oracle_mod, w, h, r):
+    """Widths divisible by 4 take the ring-buffered passes (both ring sizes,
+    and the fallbacks past their radius limits); still bit-identical."""
+    rng = np.random.default_rng(w * 1000 + r)
+    p = rng.random((h, w)).astype(np.float32)
+    got, want = ctx.box_blur(p, r), oracle_mod.box_blur(p, r)
+    assert np.array_equal(got, want), f"bit-equal {np.mean(got == want):.6f}"
+
+
 @pytest.mark.parametrize("sigma", [0.2, 0.5, 0.75, 1.0, 1.9, 2.5, 7.0])
 def test_gaussian_blur_exact(ctx, oracle_mod, sigma):
     rng = np.random.default_rng(int(sigma * 10))
