@@ -63,6 +63,21 @@ __device__ __forceinline__ void stage_tile(Src src, Dst dst) {
     }
 }
 
+// Tile of a 2-D grid, renumbered so that each of the 8 XCDs (which take
+// workgroups round-robin by linear id, each with its own L2) walks one
+// contiguous run of tiles in row-major order.  Neighbouring tiles then share
+// an L2, so the halos they read twice come from L2 instead of HBM.  A
+// bijection of the grid: XCD x owns tiles [x q + min(x, m), + q + (x < m)),
+// q = n / 8, m = n % 8.
+__device__ __forceinline__ void xcd_tile(int& tx, int& ty) {
+    const int nx = (int)gridDim.x, n = nx * (int)gridDim.y;
+    const int b = (int)blockIdx.y * nx + (int)blockIdx.x;
+    const int x = b & 7, k = b >> 3, q = n >> 3, m = n & 7;
+    const int t = x * q + (x < m ? x : m) + k;
+    tx = t % nx;
+    ty = t / nx;
+}
+
 // Runtime-sized variant: batches of KB loads per thread.
 template <int KB, int NT, class Src, class Dst>
 __device__ __forceinline__ void stage_tile_n(int n, Src src, Dst dst) {

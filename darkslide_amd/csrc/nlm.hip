@@ -59,7 +59,10 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
     constexpr int TW = NLM_TW, TH = nlm_th(P), NT = NLM_NT, SEG = NLM_SEG, HS = NLM_HS;
     constexpr int DW = SEG + 2 * P;           // differences per phase-1 task
     constexpr int RPT = (TH + 3) / 4;         // phase-2 rows per thread
-    const float wscale = wp.scale();
+    // The tile is staged scaled by cs = sqrt(-scale), so a window sum of
+    // squared differences is already the exp2 argument; the accumulated
+    // samples carry the same factor, removed once per output.
+    const float cs = sqrtf(-wp.scale());
     const int SP = S + P;
     const int PWL = TW + 2 * SP;              // loaded tile row length
     constexpr int PW = NLM_PW;
@@ -85,7 +88,7 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
         PH * PWL,
         [&](int i) {
             int gx = clampi(ox + i % PWL - SP, 0, w - 1), gy = clampi(oy + i / PWL - SP, 0, h - 1);
-            return in[(size_t)gy * w + gx];
+            return in[(size_t)gy * w + gx] * cs;
         },
         [&](int i, float v) { pt[(i / PWL) * PW + (i % PWL)] = v; });
     auto P_at = [&](int u, int v) -> float { return pt[(v + SP) * PW + (u + SP)]; };
@@ -169,7 +172,7 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
             // no per-lane branches
 #pragma unroll
             for (int k = 0; k < RPT; k++) {
-                const float wt = __builtin_amdgcn_exp2f(s * wscale);
+                const float wt = __builtin_amdgcn_exp2f(-s);
                 acc[k] += wt * qp[k * PW];
                 wsum[k] += wt;
                 if (k + 1 < RPT) s += hc[(k + 2 * P + 1) * HS] - hc[k * HS];
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
     for (int k = 0; k < RPT; k++) {
         const int gx = ox + col, gy = oy + row0 + k;
         if (row0 + k < TH && gx < w && gy < h && (!regions || regions[(gy / NLM_REGION) * nrx + gx / NLM_REGION]))
-            out[(size_t)gy * w + gx] = acc[k] / fmaxf(wsum[k], 1e-12f);
+            out[(size_t)gy * w + gx] = (acc[k] / cs) / fmaxf(wsum[k], 1e-12f);
     }
 }
 

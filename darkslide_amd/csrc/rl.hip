@@ -71,7 +71,9 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
     __shared__ float s_tmp[TMP];
     __shared__ float s_rat[YH * YS];
     const int tid = threadIdx.x;
-    const int ox = blockIdx.x * TX, oy = blockIdx.y * TY;
+    int tx, ty;
+    xcd_tile(tx, ty);
+    const int ox = tx * TX, oy = ty * TY;
     float kk[K];
 #pragma unroll
     for (int i = 0; i < K; i++) kk[i] = taps.k[i];
