@@ -6,7 +6,7 @@
 // tile and loops over all offsets with the running (acc, wsum) of its pixels
 // in registers.  The input tile plus an (S + P)-pixel halo sits in LDS.  Per
 // offset:
-//   phase 1: every thread takes two 8-column row segments of the tile rows
+//   phase 1: every thread takes one 16-column row segment of the tile rows
 //            [-P, TH+P), forms the squared differences and their horizontal
 //            (2P+1)-sums with a sliding window, and writes them to LDS;
 //   phase 2: every thread slides a vertical (2P+1)-window down one column
@@ -20,6 +20,8 @@
 // (enhance.rs:49-53), and the box blur clamps its window to the image
 // (image.rs:279, :298).  Tiles whose halo stays inside the image skip the
 // clamps.
+
+#include <type_traits>
 
 #include "common.h"
 #include "nlm.h"
@@ -35,7 +37,7 @@ namespace {
 constexpr int NLM_TW = 64;        // tile columns
 constexpr int NLM_HR = 64;        // rows of horizontal sums per tile (TH + 2P)
 constexpr int NLM_NT = 256;
-constexpr int NLM_SEG = 8;        // columns per phase-1 task
+constexpr int NLM_SEG = 16;       // columns per phase-1 task
 constexpr int NLM_HS = NLM_TW + 4;  // padded row stride of the horizontal-sum buffer
 constexpr int NLM_MAX_SP = 24;    // halo limit of the tile kernel
 // Input tile row stride: fixed (so every LDS offset inside the offset loop
@@ -47,9 +49,9 @@ __host__ __device__ constexpr int nlm_th(int P) { return NLM_HR - 2 * P; }
 // input tile plus two spare rows read (not stored) by the last row segment
 __host__ __device__ inline int nlm_pt_floats(int SP, int P) { return ((nlm_th(P) + 2 * SP + 2) * NLM_PW + 3) & ~3; }
 
-// Tile kernel.  Phase 1: thread t owns hs row (t % 64) and the 8-column
-// segments (t / 64) and (t / 64 + 4): 512 tasks, exactly two per thread.  The
-// centre samples of its two segments do not depend on the offset and stay in
+// Tile kernel.  Phase 1: thread t owns hs row (t % 64) and the column
+// segments t / 64 + 4 k, k < NTASK (TW / (4 SEG) tasks per thread).  The
+// centre samples of its segments do not depend on the offset and stay in
 // registers for the whole offset loop.  Phase 2: thread t owns column t % 64
 // and a quarter of the tile rows.
 template <int P>
@@ -58,6 +60,8 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
     extern __shared__ float lds[];
     constexpr int TW = NLM_TW, TH = nlm_th(P), NT = NLM_NT, SEG = NLM_SEG, HS = NLM_HS;
     constexpr int DW = SEG + 2 * P;           // differences per phase-1 task
+    constexpr int NTASK = TW / (SEG * (NT / NLM_HR));
+    static_assert(NTASK * SEG * (NT / NLM_HR) == TW && SEG % 4 == 0, "phase-1 tasks must tile the rows");
     constexpr int RPT = (TH + 3) / 4;         // phase-2 rows per thread
     // The tile is staged scaled by cs = sqrt(-scale), so a window sum of
     // squared differences is already the exp2 argument; the accumulated
@@ -97,17 +101,16 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
 
     // phase-1 geometry
     const int v1 = tid % NLM_HR - P;  // tile row of this thread's horizontal sums
-    const int u0a = (tid / NLM_HR) * SEG, u0b = u0a + 4 * SEG;
+    const int u00 = (tid / NLM_HR) * SEG;  // first segment; task k starts at u00 + 4 k SEG
     // window positions (tile coords) of the two segments; at the frame edge the
     // difference plane is evaluated at the clamped position (box_blur's clamp)
     auto cu_of = [&](int u) { return interior ? u : clampi(ox + u, 0, w - 1) - ox; };
     const int cv = interior ? v1 : clampi(oy + v1, 0, h - 1) - oy;
-    float ca[DW], cb[DW];
+    float cc[NTASK][DW];
 #pragma unroll
-    for (int i = 0; i < DW; i++) {
-        ca[i] = P_at(cu_of(u0a - P + i), cv);
-        cb[i] = P_at(cu_of(u0b - P + i), cv);
-    }
+    for (int task = 0; task < NTASK; task++)
+#pragma unroll
+        for (int i = 0; i < DW; i++) cc[task][i] = P_at(cu_of(u00 + 4 * SEG * task - P + i), cv);
 
     // phase-2 geometry
     const int col = tid % TW;
@@ -119,72 +122,81 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
         wsum[k] = 0.0f;
     }
 
-    const int side = 2 * S + 1;
-    const int noff = side * side;
-    int dx = -S, dy = -S;  // offsets in the reference's order: dy outer, dx inner
-    for (int o = 0; o < noff; o++) {
-        float* hs = hs0 + (o & 1) * (NLM_HSR * HS);
-        // ---- phase 1: horizontal window sums of the squared differences
+    // The offset loop is instantiated once for interior tiles (no clamps)
+    // and once for frame-edge tiles, so neither pays for the other's path.
+    auto offsets = [&](auto in_tag) {
+        constexpr bool IN = decltype(in_tag)::value;
+        const int side = 2 * S + 1;
+        const int noff = side * side;
+        int dx = -S, dy = -S;  // offsets in the reference's order: dy outer, dx inner
+        for (int o = 0; o < noff; o++) {
+            float* hs = hs0 + (o & 1) * (NLM_HSR * HS);
+            // ---- phase 1: horizontal window sums of the squared differences
 #pragma unroll
-        for (int task = 0; task < 2; task++) {
-            const float* c = task == 0 ? ca : cb;
-            const int u0 = task == 0 ? u0a : u0b;
-            float d[DW];
-            if (interior) {
-                const float* sp = pt + (cv + dy + SP) * PW + (u0 - P + dx + SP);
+            for (int task = 0; task < NTASK; task++) {
+                const float* c = cc[task];
+                const int u0 = u00 + 4 * SEG * task;
+                float d[DW];
+                if constexpr (IN) {
+                    const float* sp = pt + (cv + dy + SP) * PW + (u0 - P + dx + SP);
 #pragma unroll
-                for (int i = 0; i < DW; i++) {
-                    float e = c[i] - sp[i];
-                    d[i] = e * e;
+                    for (int i = 0; i < DW; i++) {
+                        float e = c[i] - sp[i];
+                        d[i] = e * e;
+                    }
+                } else {
+                    const int sv = clampi(oy + cv + dy, 0, h - 1) - oy;
+#pragma unroll
+                    for (int i = 0; i < DW; i++) {
+                        const int su = clampi(ox + cu_of(u0 - P + i) + dx, 0, w - 1) - ox;
+                        float e = c[i] - P_at(su, sv);
+                        d[i] = e * e;
+                    }
                 }
-            } else {
-                const int sv = clampi(oy + cv + dy, 0, h - 1) - oy;
+                float hsum[SEG];
+                float s = 0.0f;
 #pragma unroll
-                for (int i = 0; i < DW; i++) {
-                    const int su = clampi(ox + cu_of(u0 - P + i) + dx, 0, w - 1) - ox;
-                    float e = c[i] - P_at(su, sv);
-                    d[i] = e * e;
+                for (int i = 0; i < 2 * P + 1; i++) s += d[i];
+#pragma unroll
+                for (int j = 0; j < SEG; j++) {
+                    hsum[j] = s;
+                    if (j + 1 < SEG) s += d[j + 2 * P + 1] - d[j];
+                }
+                float4* dst = reinterpret_cast<float4*>(hs + (v1 + P) * HS + u0);
+#pragma unroll
+                for (int q = 0; q < SEG / 4; q++) dst[q] = make_float4(hsum[4 * q], hsum[4 * q + 1], hsum[4 * q + 2], hsum[4 * q + 3]);
+            }
+            __syncthreads();
+            // ---- phase 2: vertical window sums, weights, accumulation
+            {
+                const float* hc = hs + row0 * HS + col;  // hs row r holds tile row r - P
+                const float* qp = pt + (row0 + dy + SP) * PW + (col + dx + SP);
+                float s = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 2 * P + 1; j++) s += hc[j * HS];
+                // every row segment runs RPT rows (the last one past the tile
+                // reads padding rows of hs and is not stored), so the loop has
+                // no per-lane branches
+#pragma unroll
+                for (int k = 0; k < RPT; k++) {
+                    const float wt = __builtin_amdgcn_exp2f(-s);
+                    acc[k] += wt * qp[k * PW];
+                    wsum[k] += wt;
+                    if (k + 1 < RPT) s += hc[(k + 2 * P + 1) * HS] - hc[k * HS];
                 }
             }
-            float hsum[SEG];
-            float s = 0.0f;
-#pragma unroll
-            for (int i = 0; i < 2 * P + 1; i++) s += d[i];
-#pragma unroll
-            for (int j = 0; j < SEG; j++) {
-                hsum[j] = s;
-                if (j + 1 < SEG) s += d[j + 2 * P + 1] - d[j];
-            }
-            float4* dst = reinterpret_cast<float4*>(hs + (v1 + P) * HS + u0);
-            dst[0] = make_float4(hsum[0], hsum[1], hsum[2], hsum[3]);
-            dst[1] = make_float4(hsum[4], hsum[5], hsum[6], hsum[7]);
-        }
-        __syncthreads();
-        // ---- phase 2: vertical window sums, weights, accumulation
-        {
-            const float* hc = hs + row0 * HS + col;  // hs row r holds tile row r - P
-            const float* qp = pt + (row0 + dy + SP) * PW + (col + dx + SP);
-            float s = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 2 * P + 1; j++) s += hc[j * HS];
-            // every row segment runs RPT rows (the last one past the tile
-            // reads padding rows of hs and is not stored), so the loop has
-            // no per-lane branches
-#pragma unroll
-            for (int k = 0; k < RPT; k++) {
-                const float wt = __builtin_amdgcn_exp2f(-s);
-                acc[k] += wt * qp[k * PW];
-                wsum[k] += wt;
-                if (k + 1 < RPT) s += hc[(k + 2 * P + 1) * HS] - hc[k * HS];
+            // the other hs buffer is written next; the barrier after that phase 1
+            // orders it against this phase 2
+            if (++dx > S) {
+                dx = -S;
+                ++dy;
             }
         }
-        // the other hs buffer is written next; the barrier after that phase 1
-        // orders it against this phase 2
-        if (++dx > S) {
-            dx = -S;
-            ++dy;
-        }
-    }
+    };
+    if (interior)
+        offsets(std::true_type{});
+    else
+        offsets(std::false_type{});
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         const int gx = ox + col, gy = oy + row0 + k;
