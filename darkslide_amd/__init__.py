@@ -44,20 +44,31 @@ class ProfEntry(ctypes.Structure):
 
 
 DENOISE_NONE, DENOISE_NLM, DENOISE_LEARNED = 0, 1, 2
+QUALITY_FULL, QUALITY_DRAFT = 0, 1
+
+
+class RawCfa(ctypes.Structure):
+    """ds_raw_cfa: a Bayer raw buffer with crop, black level table and white levels."""
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_size_t), ("height", ctypes.c_size_t), ("crop_x", ctypes.c_size_t),
+                ("crop_y", ctypes.c_size_t), ("crop_w", ctypes.c_size_t), ("crop_h", ctypes.c_size_t), ("pattern", ctypes.c_uint8 * 4),
+                ("black", ctypes.c_void_p), ("nblack", ctypes.c_size_t), ("black_w", ctypes.c_size_t), ("black_h", ctypes.c_size_t),
+                ("black_cpp", ctypes.c_size_t), ("white", ctypes.c_float * 3)]
 
 
 class Frame(ctypes.Structure):
     """ds_frame: one raw frame of a batch (device pointers)."""
     _fields_ = [("raw", ctypes.c_void_p), ("w", ctypes.c_size_t), ("h", ctypes.c_size_t), ("stride", ctypes.c_size_t),
                 ("pattern", ctypes.c_uint8 * 4), ("black", ctypes.c_float), ("white", ctypes.c_float * 3),
-                ("mul", ctypes.c_float * 3), ("rgb_out", ctypes.c_void_p)]
+                ("mul", ctypes.c_float * 3), ("rgb_out", ctypes.c_void_p),
+                ("black_site", ctypes.c_float * 4), ("per_site", ctypes.c_int)]
 
 
 class PipelineOpts(ctypes.Structure):
     _fields_ = [("algo", ctypes.c_int), ("denoise", ctypes.c_int), ("amount", ctypes.c_float), ("search", ctypes.c_long),
                 ("patch", ctypes.c_size_t), ("raw_details", ctypes.c_int), ("rl_sigma", ctypes.c_float), ("rl_iters", ctypes.c_size_t),
                 ("reference_gating", ctypes.c_int), ("net", ctypes.c_void_p), ("threshold_db", ctypes.c_float),
-                ("streams", ctypes.c_int), ("autotune", ctypes.c_int), ("highlights", ctypes.c_int)]
+                ("streams", ctypes.c_int), ("autotune", ctypes.c_int), ("highlights", ctypes.c_int),
+                ("quality", ctypes.c_int)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -103,6 +114,14 @@ _SIGS = {
     "ds_version": [],
     "ds_demosaic_f32": [_P, _P, _SZ, _SZ, _P, _P, _I, _P, _P],
     "ds_cfa_clip_levels_f32": [_P, _P, _SZ, _SZ, _P, _P],
+    "ds_cfa_ingest": [_P, _P, _P, _P],
+    "ds_cfa_ingest_dev": [_P, _P, _P, _P],
+    "ds_demosaic_raw": [_P, _P, _P, _I, _I, _P, _P],
+    "ds_demosaic_raw_dev": [_P, _P, _P, _I, _I, _P, _P],
+    "ds_cfa_clip_levels_raw": [_P, _P, _P],
+    "ds_cfa_clip_levels_raw_dev": [_P, _P, _P],
+    "ds_superpixel_f32": [_P, _P, _SZ, _SZ, _P, _P, _P, _P],
+    "ds_superpixel_f32_dev": [_P, _P, _SZ, _SZ, _P, _P, _P, _P],
     "ds_cfa_clip_levels_u16": [_P, _P, _SZ, _SZ, _SZ, _P, _F, _P, _P],
     "ds_reconstruct_highlights": [_P, _P, _P, _SZ, _SZ, _P, _P],
     "ds_cfa_clip_levels_f32_dev": [_P, _P, _SZ, _SZ, _P, _P],
@@ -297,6 +316,51 @@ class Context:
                                           _ptr(rgb), _ptr(cl)))
         return (rgb, cl) if clip else rgb
 
+    @staticmethod
+    def _raw_cfa(raw, crop, pattern, black, white, black_dims):
+        raw = np.ascontiguousarray(raw, np.uint16)
+        H, W = raw.shape
+        cx, cy, cw, ch = crop if crop is not None else (0, 0, W, H)
+        b = np.ascontiguousarray(np.asarray(black, np.float32).ravel())
+        bw, bh, bcpp = black_dims if black_dims else (1, 1, 1)
+        wv = list(np.asarray(white, np.float32).ravel())
+        wv = wv * 3 if len(wv) == 1 else wv[:3]
+        r = RawCfa(raw.ctypes.data, W, H, cx, cy, cw, ch, _pat(pattern), b.ctypes.data, b.size, bw, bh, bcpp, _f3(wv))
+        return r, (raw, b), (cw, ch)
+
+    def cfa_ingest(self, raw, crop=None, pattern=RGGB, black=(512.0,), white=(16383.0,), black_dims=None):
+        """decode.rs:276-307 CFA branch -> (normalised crop (h, w) float32, pattern of the crop)."""
+        r, keep, (cw, ch) = self._raw_cfa(raw, crop, pattern, black, white, black_dims)
+        out = np.empty((ch, cw), np.float32)
+        pat = (ctypes.c_uint8 * 4)()
+        self._check(lib().ds_cfa_ingest(self.handle, ctypes.byref(r), _ptr(out), pat))
+        return out, tuple(pat)
+
+    def demosaic_raw(self, raw, crop=None, pattern=RGGB, black=(512.0,), white=(16383.0,), mul=(1.0, 1.0, 1.0), algo=ALGO_RCD,
+                     quality=QUALITY_FULL, black_dims=None):
+        """demosaic(cfa, mul, quality) of the normalised crop -> (rgb, clip)."""
+        r, keep, (cw, ch) = self._raw_cfa(raw, crop, pattern, black, white, black_dims)
+        ow, oh = (cw // 2, ch // 2) if quality == QUALITY_DRAFT else (cw, ch)
+        rgb = np.empty((oh, ow, 3), np.float32)
+        cl = np.empty((oh, ow), np.float32)
+        self._check(lib().ds_demosaic_raw(self.handle, ctypes.byref(r), _f3(mul), algo, quality, _ptr(rgb), _ptr(cl)))
+        return rgb, cl
+
+    def cfa_clip_levels_raw(self, raw, crop=None, pattern=RGGB, black=(512.0,), white=(16383.0,), black_dims=None):
+        r, keep, _ = self._raw_cfa(raw, crop, pattern, black, white, black_dims)
+        out = np.zeros(3, np.float32)
+        self._check(lib().ds_cfa_clip_levels_raw(self.handle, ctypes.byref(r), _ptr(out)))
+        return out
+
+    def superpixel(self, cfa, pattern=RGGB, mul=(1.0, 1.0, 1.0)):
+        """demosaic.rs:44-87 on a normalised CFA -> (rgb (h/2, w/2, 3), clip (h/2, w/2))."""
+        cfa = np.ascontiguousarray(cfa, np.float32)
+        h, w = cfa.shape
+        rgb = np.empty((h // 2, w // 2, 3), np.float32)
+        cl = np.empty((h // 2, w // 2), np.float32)
+        self._check(lib().ds_superpixel_f32(self.handle, _ptr(cfa), w, h, _pat(pattern), _f3(mul), _ptr(rgb), _ptr(cl)))
+        return rgb, cl
+
     def cfa_clip_levels(self, cfa, pattern=RGGB):
         """demosaic.rs:419 cfa_clip_levels on a normalised CFA -> [3] float32."""
         cfa = np.ascontiguousarray(cfa, np.float32)
@@ -393,11 +457,12 @@ class Context:
 
     def pipeline_batch(self, frames, algo=ALGO_RCD, denoise=DENOISE_NLM, amount=0.0, search=10, patch=3, raw_details=True,
                        rl_sigma=1.0, rl_iters=20, reference_gating=False, net=None, threshold_db=0.3, streams=0, autotune=False,
-                       info=True, highlights=False):
+                       info=True, highlights=False, quality=QUALITY_FULL):
         """Runs the batch graph over `frames` (a list of Frame); returns the batch info (or None with info=False)."""
         arr = (Frame * len(frames))(*frames)
         o = PipelineOpts(algo, denoise, amount, search, patch, int(raw_details), rl_sigma, rl_iters, int(reference_gating),
-                         net.handle.value if net is not None else None, threshold_db, streams, int(autotune), int(highlights))
+                         net.handle.value if net is not None else None, threshold_db, streams, int(autotune), int(highlights),
+                         int(quality))
         bi = BatchInfo()
         self._check(lib().ds_pipeline_batch(self.handle, arr, len(frames), ctypes.byref(o), ctypes.byref(bi) if info else None))
         return bi.as_dict() if info else None

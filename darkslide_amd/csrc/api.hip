@@ -381,13 +381,11 @@ Tuned tuned_for(const ds_ctx* c, size_t w, size_t h) {
     return it == root->tuning.end() ? Tuned{} : it->second;
 }
 
-int demosaic_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t* pat, float black,
-                 const float* white, const float* mul, int algo, float* rgb, float* clip) {
-    if (!valid_bayer(pat)) return fail(c, DS_ERR_INVALID, "pattern must be a Bayer 2x2 with a green checkerboard");
-    if (algo != DS_ALGO_RCD && algo != DS_ALGO_MHC) return fail(c, DS_ERR_INVALID, "unknown demosaic algorithm %d", algo);
-    if (!mul || !rgb || (!cfa && !raw)) return fail(c, DS_ERR_INVALID, "null pointer");
-    if (w == 0 || h == 0) return DS_OK;
-    if (stride < w) return fail(c, DS_ERR_INVALID, "stride %zu < width %zu", stride, w);
+// The CFA input of the demosaic-side kernels: a normalised f32 CFA, or raw
+// DN normalised on the fly with black (or, when black_site is set, the
+// black level of each CFA site at (y & 1) * 2 + (x & 1)) and white per colour.
+DemosaicArgs cfa_args(const float* cfa, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t* pat, float black,
+                      const float* black_site, const float* white) {
     DemosaicArgs a{};
     a.cfa_f32 = cfa;
     a.raw_u16 = raw;
@@ -396,10 +394,24 @@ int demosaic_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, siz
     a.h = h;
     std::memcpy(a.pattern, pat, 4);
     a.black = black;
+    a.per_site = black_site ? 1 : 0;
+    for (int k = 0; k < 4; k++) a.black_site[k] = black_site ? black_site[k] : black;
     for (int k = 0; k < 3; k++) {
         a.white[k] = white ? white[k] : 1.0f;
-        a.mul[k] = mul[k];
+        a.mul[k] = 1.0f;
     }
+    return a;
+}
+
+int demosaic_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t* pat, float black,
+                 const float* black_site, const float* white, const float* mul, int algo, float* rgb, float* clip) {
+    if (!valid_bayer(pat)) return fail(c, DS_ERR_INVALID, "pattern must be a Bayer 2x2 with a green checkerboard");
+    if (algo != DS_ALGO_RCD && algo != DS_ALGO_MHC) return fail(c, DS_ERR_INVALID, "unknown demosaic algorithm %d", algo);
+    if (!mul || !rgb || (!cfa && !raw)) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (w == 0 || h == 0) return DS_OK;
+    if (stride < w) return fail(c, DS_ERR_INVALID, "stride %zu < width %zu", stride, w);
+    DemosaicArgs a = cfa_args(cfa, raw, w, h, stride, pat, black, black_site, white);
+    for (int k = 0; k < 3; k++) a.mul[k] = mul[k];
     a.algo = algo;
     a.rgb = rgb;
     a.clip = clip;
@@ -411,24 +423,89 @@ int demosaic_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, siz
 
 // cfa_clip_levels (demosaic.rs:419-430) into levels_dev (3 device floats).
 int clip_levels_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t* pat, float black,
-                    const float* white, float* levels_dev) {
+                    const float* black_site, const float* white, float* levels_dev) {
     if (!valid_bayer(pat)) return fail(c, DS_ERR_INVALID, "pattern must be a Bayer 2x2 with a green checkerboard");
     if (!levels_dev || (!cfa && !raw)) return fail(c, DS_ERR_INVALID, "null pointer");
     if (stride < w) return fail(c, DS_ERR_INVALID, "stride %zu < width %zu", stride, w);
     SLOT(c, hl, float*, S_HL, HL_FLOATS * 4);
-    DemosaicArgs a{};
-    a.cfa_f32 = cfa;
-    a.raw_u16 = raw;
-    a.stride = stride;
-    a.w = w;
-    a.h = h;
-    std::memcpy(a.pattern, pat, 4);
-    a.black = black;
-    for (int k = 0; k < 3; k++) {
-        a.white[k] = white ? white[k] : 1.0f;
-        a.mul[k] = 1.0f;
-    }
+    const DemosaicArgs a = cfa_args(cfa, raw, w, h, stride, pat, black, black_site, white);
     return run(c, "cfa_clip_levels", [&] { return launch_cfa_clip_levels(a, hl + HL_PART_OFF, levels_dev, c->stream); });
+}
+
+// superpixel (demosaic.rs:44-87): rgb (w/2 x h/2 x 3) and clip (w/2 x h/2, or null).
+int superpixel_dev(ds_ctx* c, const float* cfa, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t* pat, float black,
+                   const float* black_site, const float* white, const float* mul, float* rgb, float* clip) {
+    if (!valid_bayer(pat)) return fail(c, DS_ERR_INVALID, "pattern must be a Bayer 2x2 with a green checkerboard");
+    if (!mul || !rgb || (!cfa && !raw)) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (stride < w) return fail(c, DS_ERR_INVALID, "stride %zu < width %zu", stride, w);
+    if (w < 2 || h < 2) return DS_OK;
+    DemosaicArgs a = cfa_args(cfa, raw, w, h, stride, pat, black, black_site, white);
+    for (int k = 0; k < 3; k++) a.mul[k] = mul[k];
+    a.rgb = rgb;
+    a.clip = clip;
+    return run(c, "superpixel", [&] { return launch_superpixel(a, c->stream); });
+}
+
+// Normalised CFA (decode.rs:300-303) of a raw view into out (w x h).
+int normalize_dev(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t* pat, float black, const float* black_site,
+                  const float* white, float* out) {
+    if (!raw || !out) return fail(c, DS_ERR_INVALID, "null pointer");
+    const DemosaicArgs a = cfa_args(nullptr, raw, w, h, stride, pat, black, black_site, white);
+    return run(c, "cfa_normalize", [&] { return launch_cfa_normalize(a, out, c->stream); });
+}
+
+// A ds_raw_cfa reduced to the view the CFA kernels read: pointer at the
+// crop origin, the buffer width as stride, the pattern and per-site black
+// levels of the crop (decode.rs:258-306).
+struct RawView {
+    const uint16_t* raw;
+    size_t w, h, stride;
+    uint8_t pat[4];
+    float black_site[4];
+    float white[3];
+};
+
+int raw_view(ds_ctx* c, const ds_raw_cfa* r, RawView& v) {
+    if (!r || !r->data || !r->black) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (!valid_bayer(r->pattern)) return fail(c, DS_ERR_INVALID, "pattern must be a Bayer 2x2 with a green checkerboard");
+    if (r->crop_x > r->width || r->crop_w > r->width - r->crop_x || r->crop_y > r->height || r->crop_h > r->height - r->crop_y)
+        return fail(c, DS_ERR_INVALID, "crop rectangle outside the %zux%zu buffer", r->width, r->height);
+    if (int rc = check_dims(c, r->crop_w, r->crop_h)) return rc;
+    const size_t bw = r->black_w ? r->black_w : 1, bh = r->black_h ? r->black_h : 1, bcpp = r->black_cpp ? r->black_cpp : 1;
+    if (r->nblack == 0) return fail(c, DS_ERR_INVALID, "empty black level table");
+    if (r->nblack != 1) {
+        if (bw > 2 || bh > 2) return fail(c, DS_ERR_INVALID, "black level table period %zux%zu over 2 is not supported", bw, bh);
+        if (r->nblack < bw * bh * bcpp) return fail(c, DS_ERR_INVALID, "black level table shorter than %zux%zux%zu", bw, bh, bcpp);
+    }
+    for (size_t py = 0; py < 2; py++)
+        for (size_t px = 0; px < 2; px++) {
+            const size_t sx = r->crop_x + px, sy = r->crop_y + py;
+            v.black_site[py * 2 + px] = r->nblack == 1 ? r->black[0] : r->black[((sy % bh) * bw + (sx % bw)) * bcpp];
+            v.pat[py * 2 + px] = r->pattern[((sy & 1) << 1) | (sx & 1)];
+        }
+    v.raw = r->data + r->crop_y * r->width + r->crop_x;
+    v.w = r->crop_w;
+    v.h = r->crop_h;
+    v.stride = r->width;
+    for (int k = 0; k < 3; k++) v.white[k] = r->white[k];
+    return DS_OK;
+}
+
+int demosaic_raw_dev(ds_ctx* c, const RawView& v, const float* mul, int algo, int quality, float* rgb_out, float* clip_out) {
+    if (quality == DS_QUALITY_DRAFT)
+        return superpixel_dev(c, nullptr, v.raw, v.w, v.h, v.stride, v.pat, 0.0f, v.black_site, v.white, mul, rgb_out, clip_out);
+    if (quality != DS_QUALITY_FULL) return fail(c, DS_ERR_INVALID, "unknown quality %d", quality);
+    return demosaic_dev(c, nullptr, v.raw, v.w, v.h, v.stride, v.pat, 0.0f, v.black_site, v.white, mul, algo, rgb_out, clip_out);
+}
+
+// Uploads the rows of the crop of a host raw buffer; v then views the device copy.
+int upload_raw(ds_ctx* c, const ds_raw_cfa* raw, RawView& v) {
+    if (int rc = raw_view(c, raw, v)) return rc;
+    const size_t n = v.h ? (v.h - 1) * v.stride + v.w : 0;
+    SLOT(c, d_raw, uint16_t*, S_IN, (n ? n : 1) * 2);
+    if (n) HIPCHK(c, hipMemcpyAsync(d_raw, v.raw, n * 2, hipMemcpyHostToDevice, c->stream), "H2D");
+    v.raw = d_raw;
+    return DS_OK;
 }
 
 // reconstruct_highlights (demosaic.rs:435-502) in place; raw_clip on the device.
@@ -737,7 +814,7 @@ int make_pairs_dev(ds_ctx* c, uint64_t seed, size_t fw, size_t fh, float iso, lo
     SLOT(c, t, float*, S_T, n * 4);
     int rc = synth_dev(c, seed, fw, fh, 14, iso, rggb, raw);
     if (rc) return rc;
-    if ((rc = demosaic_dev(c, nullptr, raw, fw, fh, fw, rggb, 512.0f, white, ones, DS_ALGO_RCD, rgb, nullptr))) return rc;
+    if ((rc = demosaic_dev(c, nullptr, raw, fw, fh, fw, rggb, 512.0f, nullptr, white, ones, DS_ALGO_RCD, rgb, nullptr))) return rc;
     if ((rc = run(c, "oklab_fwd", [&] { return launch_oklab_fwd(rgb, n, c->ok, L, A, B, c->stream); }))) return rc;
     if ((rc = noise_dev(c, L, fw, fh, sig))) return rc;
     if ((rc = nlm_dev(c, L, fw, fh, NlmH{0.0f, sig, 0.6f}, search, patch, LF))) return rc;
@@ -876,7 +953,8 @@ int autotune_shape(ds_ctx* c, size_t w, size_t h, ds_tuning* out) {
 
 struct FramePlan {
     size_t idx;
-    bool highlights, denoise, learned, rl;
+    bool highlights, denoise, learned, rl, draft;
+    size_t w, h;  // size of the demosaiced image
     float amount;
     float rl_sigma;
     size_t rl_iters;
@@ -885,6 +963,9 @@ struct FramePlan {
 FramePlan plan_frame(const ds_frame& f, size_t idx, const ds_pipeline_opts& o) {
     FramePlan p{};
     p.idx = idx;
+    p.draft = o.quality == DS_QUALITY_DRAFT;
+    p.w = p.draft ? f.w / 2 : f.w;
+    p.h = p.draft ? f.h / 2 : f.h;
     p.highlights = o.highlights != 0;
     p.denoise = o.denoise != DS_DENOISE_NONE;
     p.learned = o.denoise == DS_DENOISE_LEARNED;
@@ -896,7 +977,7 @@ FramePlan plan_frame(const ds_frame& f, size_t idx, const ds_pipeline_opts& o) {
         // Engine::base_for (pipeline/mod.rs:229-252): denoise(img, amount.max(1.0));
         // raw_details(img, 0.75, 8) only at full quality with a long edge > 3200
         p.amount = std::fmax(o.amount, 1.0f);
-        p.rl = p.rl && (f.w > f.h ? f.w : f.h) > 3200;
+        p.rl = p.rl && !p.draft && (p.w > p.h ? p.w : p.h) > 3200;
         p.rl_sigma = 0.75f;
         p.rl_iters = 8;
     }
@@ -905,13 +986,13 @@ FramePlan plan_frame(const ds_frame& f, size_t idx, const ds_pipeline_opts& o) {
 
 // Workspace bytes per slot that one frame's stages ask for (mirrors their SLOT calls).
 void frame_needs(const ds_frame& f, const FramePlan& p, size_t need[S_COUNT]) {
-    const size_t n = f.w * f.h;
+    const size_t n = p.w * p.h;
     auto up = [&](Slot s, size_t b) { need[s] = std::max(need[s], b); };
     if (p.denoise || p.rl) up(S_PIPE_RGB, n * 12);
     if (p.highlights) {
         up(S_PIPE_CLIP, n * 4);
         up(S_HL, HL_FLOATS * 4);
-        up(S_WS, highlights_ws_floats(f.w, f.h) * 4);
+        up(S_WS, highlights_ws_floats(p.w, p.h) * 4);
     }
     if (p.denoise && p.rl) up(S_PIPE_DEN, n * 12);
     if (p.denoise) {
@@ -921,17 +1002,17 @@ void frame_needs(const ds_frame& f, const FramePlan& p, size_t need[S_COUNT]) {
         up(S_LF, n * 4);
         up(S_SIGMA, 64);
         up(S_WS, 12 * n * 4);
-        const size_t cnt = noise_sample_count(f.w, f.h);
+        const size_t cnt = noise_sample_count(p.w, p.h);
         up(S_NOISE, (cnt ? cnt : 1) * sizeof(uint32_t));
         if (p.learned) {
-            const size_t nreg = nlm_regions(f.w, f.h);
-            up(S_REGIONS, (2 * nreg + 64 + nlm_spot_scratch_floats(f.w, f.h)) * 4);
+            const size_t nreg = nlm_regions(p.w, p.h);
+            up(S_REGIONS, (2 * nreg + 64 + nlm_spot_scratch_floats(p.w, p.h)) * 4);
             up(S_X, n * 4);
             up(S_T, n * 4);
-            up(S_ACT, cnn_fused_act_bytes(1, (int)f.h, (int)f.w));
+            up(S_ACT, cnn_fused_act_bytes(1, (int)p.h, (int)p.w));
         }
     }
-    if (p.rl) up(S_WS, rl_workspace_floats(f.w, f.h) * 4);
+    if (p.rl) up(S_WS, rl_workspace_floats(p.w, p.h) * 4);
 }
 
 int ensure_branches(ds_ctx* c, int k) {
@@ -972,37 +1053,41 @@ void retire_stale_graphs(ds_ctx* c) {
 // One frame's stages on branch context b.
 int enqueue_frame(ds_ctx* b, const ds_frame& f, const FramePlan& p, const ds_pipeline_opts& o, int* count_dev) {
     const bool direct = !p.denoise && !p.rl;
+    const size_t w = p.w, h = p.h;
+    const float* bs = f.per_site ? f.black_site : nullptr;
     float* rgb = f.rgb_out;
     if (!direct) {
-        SLOT(b, t, float*, S_PIPE_RGB, f.w * f.h * 12);
+        SLOT(b, t, float*, S_PIPE_RGB, w * h * 12);
         rgb = t;
     }
     float* clip = nullptr;
     if (p.highlights) {
-        SLOT(b, t, float*, S_PIPE_CLIP, f.w * f.h * 4);
+        SLOT(b, t, float*, S_PIPE_CLIP, w * h * 4);
         clip = t;
     }
-    int rc = demosaic_dev(b, nullptr, f.raw, f.w, f.h, f.stride, f.pattern, f.black, f.white, f.mul, o.algo, rgb, clip);
+    // Engine::base (pipeline/mod.rs:184-188): demosaic(cfa, mul, quality)
+    int rc = p.draft ? superpixel_dev(b, nullptr, f.raw, f.w, f.h, f.stride, f.pattern, f.black, bs, f.white, f.mul, rgb, clip)
+                     : demosaic_dev(b, nullptr, f.raw, f.w, f.h, f.stride, f.pattern, f.black, bs, f.white, f.mul, o.algo, rgb, clip);
     if (rc) return rc;
     if (p.highlights) {
-        // Engine::base (pipeline/mod.rs:186-187): rebuild clipped channels at the clip levels of this frame
+        // rebuild clipped channels at the clip levels of the whole CFA
         SLOT(b, hl, float*, S_HL, HL_FLOATS * 4);
-        if ((rc = clip_levels_dev(b, nullptr, f.raw, f.w, f.h, f.stride, f.pattern, f.black, f.white, hl))) return rc;
-        if ((rc = highlights_dev(b, rgb, clip, f.w, f.h, f.mul, hl))) return rc;
+        if ((rc = clip_levels_dev(b, nullptr, f.raw, f.w, f.h, f.stride, f.pattern, f.black, bs, f.white, hl))) return rc;
+        if ((rc = highlights_dev(b, rgb, clip, w, h, f.mul, hl))) return rc;
     }
     const float* cur = rgb;
     if (p.denoise) {
         float* dst = f.rgb_out;
         if (p.rl) {
-            SLOT(b, t, float*, S_PIPE_DEN, f.w * f.h * 12);
+            SLOT(b, t, float*, S_PIPE_DEN, w * h * 12);
             dst = t;
         }
-        rc = p.learned ? denoise_learned_dev(b, o.net, rgb, f.w, f.h, p.amount, o.search, o.patch, o.threshold_db, dst, nullptr, count_dev)
-                       : denoise_dev(b, rgb, f.w, f.h, p.amount, o.search, o.patch, dst);
+        rc = p.learned ? denoise_learned_dev(b, o.net, rgb, w, h, p.amount, o.search, o.patch, o.threshold_db, dst, nullptr, count_dev)
+                       : denoise_dev(b, rgb, w, h, p.amount, o.search, o.patch, dst);
         if (rc) return rc;
         cur = dst;
     }
-    if (p.rl) rc = raw_details_dev(b, cur, f.w, f.h, p.rl_sigma, p.rl_iters, f.rgb_out);
+    if (p.rl) rc = raw_details_dev(b, cur, w, h, p.rl_sigma, p.rl_iters, f.rgb_out);
     return rc;
 }
 
@@ -1016,12 +1101,14 @@ int validate_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipelin
         return fail(c, DS_ERR_INVALID, "search %ld / patch %zu out of range", o.search, o.patch);
     if (o.raw_details && (!(o.rl_sigma >= 0.0f) || o.rl_sigma > 1000.0f)) return fail(c, DS_ERR_INVALID, "RL sigma out of range");
     if (o.streams < 0 || o.streams > DS_MAX_BRANCHES) return fail(c, DS_ERR_INVALID, "streams %d out of range", o.streams);
+    if (o.quality != DS_QUALITY_FULL && o.quality != DS_QUALITY_DRAFT) return fail(c, DS_ERR_INVALID, "unknown quality %d", o.quality);
     for (size_t i = 0; i < n; i++) {
         const ds_frame& f = frames[i];
         if (int rc = check_dims(c, f.w, f.h)) return rc;
         if (!f.raw || !f.rgb_out || f.w == 0 || f.h == 0) return fail(c, DS_ERR_INVALID, "frame %zu: null buffer or empty", i);
         if (f.stride < f.w) return fail(c, DS_ERR_INVALID, "frame %zu: stride < width", i);
         if (!valid_bayer(f.pattern)) return fail(c, DS_ERR_INVALID, "frame %zu: not a Bayer pattern", i);
+        if (o.quality == DS_QUALITY_DRAFT && (f.w < 2 || f.h < 2)) return fail(c, DS_ERR_INVALID, "frame %zu: too small for superpixels", i);
     }
     return DS_OK;
 }
@@ -1047,6 +1134,7 @@ std::vector<uint8_t> batch_key(ds_ctx* c, const ds_frame* frames, size_t n, cons
     put(&o.net, sizeof o.net);
     put(&o.threshold_db, sizeof o.threshold_db);
     put(&o.highlights, sizeof o.highlights);
+    put(&o.quality, sizeof o.quality);
     for (size_t i = 0; i < n; i++) {
         const ds_frame& f = frames[i];
         put(&f.raw, sizeof f.raw);
@@ -1057,6 +1145,8 @@ std::vector<uint8_t> batch_key(ds_ctx* c, const ds_frame* frames, size_t n, cons
         put(&f.black, sizeof f.black);
         put(f.white, sizeof f.white);
         put(f.mul, sizeof f.mul);
+        put(&f.per_site, sizeof f.per_site);
+        if (f.per_site) put(f.black_site, sizeof f.black_site);
         put(&f.rgb_out, sizeof f.rgb_out);
         const Tuned t = tuned_for(c, f.w, f.h);
         put(&t, sizeof t);
@@ -1265,7 +1355,8 @@ int pipeline_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipelin
         HIPCHK(c, hipMemcpy(cnt.data(), counts, n * sizeof(int), hipMemcpyDeviceToHost), "D2H");
         for (size_t i = 0; i < n; i++) {
             info->fallback_regions += (size_t)cnt[i];
-            info->regions += nlm_regions(frames[i].w, frames[i].h);
+            const FramePlan p = plan_frame(frames[i], i, o);
+            info->regions += nlm_regions(p.w, p.h);
         }
     }
     return DS_OK;
@@ -1451,20 +1542,51 @@ int ds_demosaic_u16_dev(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, size
     LOCK(c);
     if (int rc = check_dims(c, w, h)) return rc;
     if (!white) return fail(c, DS_ERR_INVALID, "white is null");
-    return demosaic_dev(c, nullptr, raw, w, h, stride, pattern, black, white, mul, algo, rgb_out, clip_out);
+    return demosaic_dev(c, nullptr, raw, w, h, stride, pattern, black, nullptr, white, mul, algo, rgb_out, clip_out);
 }
 
 int ds_demosaic_f32_dev(ds_ctx* c, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], int algo,
                         float* rgb_out, float* clip_out) {
     LOCK(c);
     if (int rc = check_dims(c, w, h)) return rc;
-    return demosaic_dev(c, cfa, nullptr, w, h, w, pattern, 0.0f, nullptr, mul, algo, rgb_out, clip_out);
+    return demosaic_dev(c, cfa, nullptr, w, h, w, pattern, 0.0f, nullptr, nullptr, mul, algo, rgb_out, clip_out);
+}
+
+int ds_cfa_ingest_dev(ds_ctx* c, const ds_raw_cfa* raw, float* cfa_out, uint8_t pattern_out[4]) {
+    LOCK(c);
+    RawView v{};
+    if (int rc = raw_view(c, raw, v)) return rc;
+    if (!cfa_out || !pattern_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    std::memcpy(pattern_out, v.pat, 4);
+    if (v.w == 0 || v.h == 0) return DS_OK;
+    return normalize_dev(c, v.raw, v.w, v.h, v.stride, v.pat, 0.0f, v.black_site, v.white, cfa_out);
+}
+
+int ds_demosaic_raw_dev(ds_ctx* c, const ds_raw_cfa* raw, const float mul[3], int algo, int quality, float* rgb_out, float* clip_out) {
+    LOCK(c);
+    RawView v{};
+    if (int rc = raw_view(c, raw, v)) return rc;
+    return demosaic_raw_dev(c, v, mul, algo, quality, rgb_out, clip_out);
+}
+
+int ds_cfa_clip_levels_raw_dev(ds_ctx* c, const ds_raw_cfa* raw, float* levels_out) {
+    LOCK(c);
+    RawView v{};
+    if (int rc = raw_view(c, raw, v)) return rc;
+    return clip_levels_dev(c, nullptr, v.raw, v.w, v.h, v.stride, v.pat, 0.0f, v.black_site, v.white, levels_out);
+}
+
+int ds_superpixel_f32_dev(ds_ctx* c, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], float* rgb_out,
+                          float* clip_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    return superpixel_dev(c, cfa, nullptr, w, h, w, pattern, 0.0f, nullptr, nullptr, mul, rgb_out, clip_out);
 }
 
 int ds_cfa_clip_levels_f32_dev(ds_ctx* c, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], float* levels_out) {
     LOCK(c);
     if (int rc = check_dims(c, w, h)) return rc;
-    return clip_levels_dev(c, cfa, nullptr, w, h, w, pattern, 0.0f, nullptr, levels_out);
+    return clip_levels_dev(c, cfa, nullptr, w, h, w, pattern, 0.0f, nullptr, nullptr, levels_out);
 }
 
 int ds_cfa_clip_levels_u16_dev(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
@@ -1472,7 +1594,7 @@ int ds_cfa_clip_levels_u16_dev(ds_ctx* c, const uint16_t* raw, size_t w, size_t 
     LOCK(c);
     if (int rc = check_dims(c, w, h)) return rc;
     if (!white) return fail(c, DS_ERR_INVALID, "white is null");
-    return clip_levels_dev(c, nullptr, raw, w, h, stride, pattern, black, white, levels_out);
+    return clip_levels_dev(c, nullptr, raw, w, h, stride, pattern, black, nullptr, white, levels_out);
 }
 
 int ds_reconstruct_highlights_dev(ds_ctx* c, float* rgb, const float* clip, size_t w, size_t h, const float mul[3], const float* raw_clip) {
@@ -1526,7 +1648,7 @@ int ds_demosaic_f32(ds_ctx* c, const float* cfa, size_t w, size_t h, const uint8
         d_clip = dc;
     }
     HIPCHK(c, hipMemcpyAsync(d_in, cfa, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
-    if (int rc = demosaic_dev(c, d_in, nullptr, w, h, w, pattern, 0.0f, nullptr, mul, algo, d_rgb, d_clip)) return rc;
+    if (int rc = demosaic_dev(c, d_in, nullptr, w, h, w, pattern, 0.0f, nullptr, nullptr, mul, algo, d_rgb, d_clip)) return rc;
     HIPCHK(c, hipMemcpyAsync(rgb_out, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
     if (clip_out) HIPCHK(c, hipMemcpyAsync(clip_out, d_clip, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
@@ -1549,9 +1671,79 @@ int ds_demosaic_u16(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, size_t s
         d_clip = dc;
     }
     HIPCHK(c, hipMemcpyAsync(d_raw, raw, ((h - 1) * stride + w) * 2, hipMemcpyHostToDevice, c->stream), "H2D");
-    if (int rc = demosaic_dev(c, nullptr, d_raw, w, h, stride, pattern, black, white, mul, algo, d_rgb, d_clip)) return rc;
+    if (int rc = demosaic_dev(c, nullptr, d_raw, w, h, stride, pattern, black, nullptr, white, mul, algo, d_rgb, d_clip)) return rc;
     HIPCHK(c, hipMemcpyAsync(rgb_out, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
     if (clip_out) HIPCHK(c, hipMemcpyAsync(clip_out, d_clip, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_cfa_ingest(ds_ctx* c, const ds_raw_cfa* raw, float* cfa_out, uint8_t pattern_out[4]) {
+    LOCK(c);
+    RawView v{};
+    if (int rc = upload_raw(c, raw, v)) return rc;
+    if (!cfa_out || !pattern_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    std::memcpy(pattern_out, v.pat, 4);
+    const size_t n = v.w * v.h;
+    if (n == 0) return DS_OK;
+    SLOT(c, d_out, float*, S_OUT, n * 4);
+    if (int rc = normalize_dev(c, v.raw, v.w, v.h, v.stride, v.pat, 0.0f, v.black_site, v.white, d_out)) return rc;
+    HIPCHK(c, hipMemcpyAsync(cfa_out, d_out, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_demosaic_raw(ds_ctx* c, const ds_raw_cfa* raw, const float mul[3], int algo, int quality, float* rgb_out, float* clip_out) {
+    LOCK(c);
+    RawView v{};
+    if (int rc = upload_raw(c, raw, v)) return rc;
+    if (!rgb_out || !mul) return fail(c, DS_ERR_INVALID, "null pointer");
+    const bool draft = quality == DS_QUALITY_DRAFT;
+    const size_t n = draft ? (v.w / 2) * (v.h / 2) : v.w * v.h;
+    if (n == 0) return DS_OK;
+    SLOT(c, d_rgb, float*, S_OUT, n * 12);
+    float* d_clip = nullptr;
+    if (clip_out) {
+        SLOT(c, dc, float*, S_CLIP, n * 4);
+        d_clip = dc;
+    }
+    if (int rc = demosaic_raw_dev(c, v, mul, algo, quality, d_rgb, d_clip)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb_out, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (clip_out) HIPCHK(c, hipMemcpyAsync(clip_out, d_clip, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_cfa_clip_levels_raw(ds_ctx* c, const ds_raw_cfa* raw, float levels_out[3]) {
+    LOCK(c);
+    RawView v{};
+    if (int rc = upload_raw(c, raw, v)) return rc;
+    if (!levels_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    SLOT(c, d_lv, float*, S_SIGMA, 64);
+    if (int rc = clip_levels_dev(c, nullptr, v.raw, v.w, v.h, v.stride, v.pat, 0.0f, v.black_site, v.white, d_lv)) return rc;
+    HIPCHK(c, hipMemcpyAsync(levels_out, d_lv, 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_superpixel_f32(ds_ctx* c, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], float* rgb_out,
+                      float* clip_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!cfa || !rgb_out || !mul) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t n = w * h, no = (w / 2) * (h / 2);
+    if (no == 0) return DS_OK;
+    SLOT(c, d_in, float*, S_IN, n * 4);
+    SLOT(c, d_rgb, float*, S_OUT, no * 12);
+    float* d_clip = nullptr;
+    if (clip_out) {
+        SLOT(c, dc, float*, S_CLIP, no * 4);
+        d_clip = dc;
+    }
+    HIPCHK(c, hipMemcpyAsync(d_in, cfa, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = superpixel_dev(c, d_in, nullptr, w, h, w, pattern, 0.0f, nullptr, nullptr, mul, d_rgb, d_clip)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb_out, d_rgb, no * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    if (clip_out) HIPCHK(c, hipMemcpyAsync(clip_out, d_clip, no * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;
 }
@@ -1564,7 +1756,7 @@ int ds_cfa_clip_levels_f32(ds_ctx* c, const float* cfa, size_t w, size_t h, cons
     SLOT(c, d_in, float*, S_IN, (n ? n : 1) * 4);
     SLOT(c, d_lv, float*, S_SIGMA, 64);
     if (n) HIPCHK(c, hipMemcpyAsync(d_in, cfa, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
-    if (int rc = clip_levels_dev(c, d_in, nullptr, w, h, w, pattern, 0.0f, nullptr, d_lv)) return rc;
+    if (int rc = clip_levels_dev(c, d_in, nullptr, w, h, w, pattern, 0.0f, nullptr, nullptr, d_lv)) return rc;
     HIPCHK(c, hipMemcpyAsync(levels_out, d_lv, 12, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;
@@ -1580,7 +1772,7 @@ int ds_cfa_clip_levels_u16(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, s
     SLOT(c, d_raw, uint16_t*, S_IN, (n ? n : 1) * 2);
     SLOT(c, d_lv, float*, S_SIGMA, 64);
     if (n) HIPCHK(c, hipMemcpyAsync(d_raw, raw, n * 2, hipMemcpyHostToDevice, c->stream), "H2D");
-    if (int rc = clip_levels_dev(c, nullptr, d_raw, w, h, stride, pattern, black, white, d_lv)) return rc;
+    if (int rc = clip_levels_dev(c, nullptr, d_raw, w, h, stride, pattern, black, nullptr, white, d_lv)) return rc;
     HIPCHK(c, hipMemcpyAsync(levels_out, d_lv, 12, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;

@@ -42,17 +42,19 @@ struct CfaSrc {
     const float* f32;
     const uint16_t* u16;
     size_t stride;  // elements per row
-    float black;
+    float blk[4];   // black level per CFA site, (y & 1) * 2 + (x & 1)
     float white[3];
     float mul[3];
     Bayer pat;
     int w, h;
 
-    // normalised CFA value at an in-image pixel: decode.rs:302-303
+    // normalised CFA value at an in-image pixel: decode.rs:300-303, with the
+    // black level black_at(x, y) of a table of period <= 2
     __device__ __forceinline__ float cfa(int x, int y, int c) const {
         if (f32) return f32[(size_t)y * stride + x];
-        float raw = (float)u16[(size_t)y * stride + x];
-        return (raw - black) / fmaxr(white[c] - black, 1.0f);
+        const float raw = (float)u16[(size_t)y * stride + x];
+        const float b = (y & 1) ? ((x & 1) ? blk[3] : blk[2]) : ((x & 1) ? blk[1] : blk[0]);
+        return (raw - b) / fmaxr(white[c] - b, 1.0f);
     }
 };
 
@@ -446,7 +448,7 @@ CfaSrc make_src(const DemosaicArgs& a) {
     src.f32 = a.cfa_f32;
     src.u16 = a.raw_u16;
     src.stride = a.stride;
-    src.black = a.black;
+    for (int i = 0; i < 4; i++) src.blk[i] = a.per_site ? a.black_site[i] : a.black;
     for (int c = 0; c < 3; c++) {
         src.white[c] = a.white[c];
         src.mul[c] = a.mul[c];
@@ -509,7 +511,62 @@ __global__ __launch_bounds__(256) void k_clip_final(const float* __restrict__ pa
     }
 }
 
+// superpixel (demosaic.rs:44-87), Bayer: output pixel (x, y) from the 2x2
+// tile at (2x, 2y), samples taken in the reference's order (dy outer).
+__global__ __launch_bounds__(256) void k_superpixel(CfaSrc src, float* __restrict__ rgb, float* __restrict__ clip) {
+    const int ow = src.w / 2, oh = src.h / 2;
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= ow || y >= oh) return;
+    float acc[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};
+    int n[3] = {0, 0, 0}, clipped = 0;
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+            const int sx = 2 * x + dx, sy = 2 * y + dy;
+            const int c = src.pat.at(sx, sy);
+            const float v = src.cfa(sx, sy, c);
+            clipped += v >= kClip ? 1 : 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (k == c) {
+                    mx[k] = fmaxr(mx[k], v);
+                    acc[k] += v;
+                    n[k]++;
+                }
+        }
+    const size_t o = (size_t)y * ow + x;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float v = mx[k] >= kClip ? mx[k] : (n[k] > 0 ? acc[k] / (float)n[k] : 0.0f);
+        rgb[3 * o + k] = v * src.mul[k];
+    }
+    if (clip) clip[o] = fminf((float)clipped / 4.0f * 2.0f, 1.0f);
+}
+
+// normalised CFA (decode.rs:300-303)
+__global__ __launch_bounds__(256) void k_cfa_normalize(CfaSrc src, float* __restrict__ out) {
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= src.w) return;
+    out[(size_t)y * src.w + x] = src.cfa(x, y, src.pat.at(x, y));
+}
+
 }  // namespace
+
+hipError_t launch_superpixel(const DemosaicArgs& a, hipStream_t stream) {
+    const CfaSrc src = make_src(a);
+    const int ow = (int)(a.w / 2), oh = (int)(a.h / 2);
+    if (ow == 0 || oh == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_superpixel, dim3((unsigned)((ow + 63) / 64), (unsigned)((oh + 3) / 4)), dim3(256), 0, stream, src, a.rgb, a.clip);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfa_normalize(const DemosaicArgs& a, float* out, hipStream_t stream) {
+    const CfaSrc src = make_src(a);
+    if (a.w == 0 || a.h == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cfa_normalize, dim3((unsigned)((a.w + 255) / 256), (unsigned)a.h), dim3(256), 0, stream, src, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_cfa_clip_levels(const DemosaicArgs& a, float* partial, float* out3, hipStream_t stream) {
     const CfaSrc src = make_src(a);

@@ -69,6 +69,46 @@ int ds_demosaic_f32(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uin
 int ds_demosaic_u16(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4], float black,
                     const float white[3], const float mul[3], int algo, float* rgb_out, float* clip_out);
 
+/* ---- raw ingestion and Draft demosaic ----------------------------------- */
+
+/* A Bayer raw buffer as load_raw has it after rawler's decode
+ * (decode.rs:188-307): `width` x `height` samples, the crop rectangle kept
+ * (crop_area or active_area, decode.rs:251-254), the CFA colour at the
+ * buffer origin, the black level table black_at (decode.rs:258-263: one
+ * value, or black_w x black_h entries of black_cpp values, channel 0 used)
+ * and the white level per colour (white_at, decode.rs:290).  Black tables
+ * whose period exceeds 2 (non-Bayer sensors) are refused. */
+typedef struct ds_raw_cfa {
+    const uint16_t* data; /* host or device per entry point */
+    size_t width, height;
+    size_t crop_x, crop_y, crop_w, crop_h;
+    uint8_t pattern[4];
+    const float* black; /* host pointer, nblack values */
+    size_t nblack, black_w, black_h, black_cpp;
+    float white[3];
+} ds_raw_cfa;
+
+#define DS_QUALITY_FULL 0
+#define DS_QUALITY_DRAFT 1
+
+/* The CfaImage load_raw builds (decode.rs:292-306): the normalised crop
+ * (crop_w x crop_h floats) and its pattern (CfaPattern::shifted, decode.rs:38). */
+int ds_cfa_ingest(ds_ctx* ctx, const ds_raw_cfa* raw, float* cfa_out, uint8_t pattern_out[4]);
+
+/* demosaic(cfa, mul, quality) (demosaic.rs:20) of the crop, normalised on
+ * the fly: DS_QUALITY_FULL gives crop_w x crop_h (RCD or MHC), DS_QUALITY_DRAFT
+ * the superpixel image (demosaic.rs:44-87) of crop_w/2 x crop_h/2.  clip_out
+ * (same size as one plane of rgb_out) may be NULL. */
+int ds_demosaic_raw(ds_ctx* ctx, const ds_raw_cfa* raw, const float mul[3], int algo, int quality, float* rgb_out, float* clip_out);
+
+/* cfa_clip_levels (demosaic.rs:419) of the crop's normalised CFA. */
+int ds_cfa_clip_levels_raw(ds_ctx* ctx, const ds_raw_cfa* raw, float levels_out[3]);
+
+/* demosaic(cfa, mul, Quality::Draft) on a normalised CFA: superpixel
+ * (demosaic.rs:44-87), rgb_out (w/2)*(h/2)*3, clip_out (w/2)*(h/2) or NULL. */
+int ds_superpixel_f32(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], float* rgb_out,
+                      float* clip_out);
+
 /* ---- highlight reconstruction ------------------------------------------ */
 
 /* demosaic.rs:419 cfa_clip_levels(cfa): per-colour max of the normalised
@@ -137,6 +177,12 @@ int ds_demosaic_u16_dev(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, si
                         const float white[3], const float mul[3], int algo, float* rgb_out, float* clip_out);
 int ds_demosaic_f32_dev(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], int algo,
                         float* rgb_out, float* clip_out);
+/* raw->data, cfa_out, rgb_out, clip_out and levels_out on the device */
+int ds_cfa_ingest_dev(ds_ctx* ctx, const ds_raw_cfa* raw, float* cfa_out, uint8_t pattern_out[4]);
+int ds_demosaic_raw_dev(ds_ctx* ctx, const ds_raw_cfa* raw, const float mul[3], int algo, int quality, float* rgb_out, float* clip_out);
+int ds_cfa_clip_levels_raw_dev(ds_ctx* ctx, const ds_raw_cfa* raw, float* levels_out);
+int ds_superpixel_f32_dev(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], float* rgb_out,
+                          float* clip_out);
 /* levels_out: 3 device floats */
 int ds_cfa_clip_levels_f32_dev(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], float* levels_out);
 int ds_cfa_clip_levels_u16_dev(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4],
@@ -269,7 +315,9 @@ typedef struct ds_frame {
     float black;
     float white[3];
     float mul[3];
-    float* rgb_out; /* w*h interleaved RGB */
+    float* rgb_out; /* w*h interleaved RGB (w/2 * h/2 at DS_QUALITY_DRAFT) */
+    float black_site[4]; /* with per_site set: black level of each CFA site, (y & 1) * 2 + (x & 1), overriding black */
+    int per_site;
 } ds_frame;
 
 typedef struct ds_pipeline_opts {
@@ -287,6 +335,7 @@ typedef struct ds_pipeline_opts {
     int streams;          /* graph branches, 1..DS_MAX_BRANCHES; 0 = 2, or tuned when autotune is set */
     int autotune;
     int highlights;       /* 1: reconstruct_highlights after demosaic, as Engine::base does (pipeline/mod.rs:186-187) */
+    int quality;          /* DS_QUALITY_FULL (RCD / MHC) or DS_QUALITY_DRAFT (superpixel, half size; no RL under reference_gating) */
 } ds_pipeline_opts;
 
 typedef struct ds_batch_info {

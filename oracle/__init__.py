@@ -306,3 +306,35 @@ def reconstruct_highlights(img, clip, mul, raw_clip):
     rc = np.asarray(raw_clip, img.dtype)
     f(_p(img), _p(clip), ctypes.c_size_t(w), ctypes.c_size_t(h), _p(m), _p(rc))
     return img
+
+
+def cfa_ingest_u16(raw, crop, pattern, black, white, black_dims=None, dtype=np.float32):
+    """decode.rs:276-307 CFA branch: raw (H, W) uint16 buffer, crop = (x, y, w, h), pattern at the
+    buffer origin, black = values (1, or bw*bh*bcpp with black_dims = (bw, bh, bcpp)), white = 1 or
+    >= 3 values -> (cfa (h, w), pattern of the crop)."""
+    raw = np.ascontiguousarray(raw, np.uint16)
+    H, W = raw.shape
+    cx, cy, cw, ch = crop
+    assert cx + cw <= W and cy + ch <= H
+    b = np.ascontiguousarray(black, dtype).ravel()
+    wv = np.ascontiguousarray(white, dtype).ravel()
+    bw, bh, bcpp = black_dims if black_dims else (1, 1, 1)
+    out = np.empty((ch, cw), dtype)
+    pat_out = (ctypes.c_uint8 * 4)()
+    f, _ = _fn("or_cfa_ingest_u16", dtype)
+    S = ctypes.c_size_t
+    f(_p(raw), S(W), S(cx), S(cy), S(cw), S(ch), _pat(pattern), _p(b), S(b.size), S(bw), S(bh), S(bcpp), _p(wv), S(wv.size),
+      _p(out), pat_out)
+    return out, tuple(pat_out)
+
+
+def superpixel(cfa, pattern=(0, 1, 1, 2), mul=(1, 1, 1)):
+    """demosaic.rs:44-87 (Bayer) -> (rgb (h/2, w/2, 3), clip (h/2, w/2))."""
+    cfa = np.ascontiguousarray(cfa)
+    h, w = cfa.shape
+    rgb = np.empty((h // 2, w // 2, 3), cfa.dtype)
+    clip = np.empty((h // 2, w // 2), cfa.dtype)
+    f, _ = _fn("or_superpixel", cfa.dtype)
+    m = np.asarray(mul, cfa.dtype)
+    f(_p(cfa), ctypes.c_size_t(w), ctypes.c_size_t(h), _pat(pattern), _p(m), _p(rgb), _p(clip))
+    return rgb, clip

@@ -560,6 +560,69 @@ template <class T> void cfa_normalize_u16(const uint16_t* raw, size_t w, size_t 
         }
 }
 
+// load_raw's CFA branch (decode.rs:276-307, the non-float case) on a raw
+// buffer of `width` samples per row: the crop rectangle, the colour of the
+// buffer position (pattern given at the buffer origin), the black level
+// black_at(sx, sy, 0) from a bw x bh table of bcpp values per entry (or one
+// value), white_at(c) from one or >= 3 values, and the pattern of the
+// cropped image (CfaPattern::shifted, decode.rs:38-46).
+template <class T>
+void cfa_ingest_u16(const uint16_t* raw, size_t width, size_t crop_x, size_t crop_y, size_t cw, size_t ch, const uint8_t* pat, const T* black,
+                    size_t nblack, size_t bw, size_t bh, size_t bcpp, const T* white, size_t nwhite, T* out, uint8_t* pat_out) {
+    bw = bw ? bw : 1;
+    bh = bh ? bh : 1;
+    bcpp = bcpp ? bcpp : 1;
+    auto black_at = [&](size_t x, size_t y, size_t c) -> T {
+        if (nblack == 1) return black[0];
+        return black[((y % bh) * bw + (x % bw)) * bcpp + (c < bcpp - 1 ? c : bcpp - 1)];
+    };
+    auto white_at = [&](int c) -> T { return nwhite >= 3 ? white[(size_t)c < nwhite - 1 ? (size_t)c : nwhite - 1] : white[0]; };
+#pragma omp parallel for schedule(static)
+    for (isz yy = 0; yy < (isz)ch; yy++) {
+        const size_t sy = (size_t)yy + crop_y;
+        for (size_t x = 0; x < cw; x++) {
+            const size_t sx = x + crop_x;
+            const int c = bayer_color(pat, sx, sy);
+            const T raw_v = (T)raw[sy * width + sx];
+            const T b = black_at(sx, sy, 0);
+            out[(size_t)yy * cw + x] = (raw_v - b) / vmax(white_at(c) - b, (T)1);
+        }
+    }
+    for (size_t y = 0; y < 2; y++)
+        for (size_t x = 0; x < 2; x++) pat_out[y * 2 + x] = (uint8_t)bayer_color(pat, x + crop_x, y + crop_y);
+}
+
+// superpixel (demosaic.rs:44-87) for a Bayer CFA: each 2x2 tile becomes one
+// pixel, w/2 x h/2.  A colour with a sample >= 0.995 keeps its max, else the
+// mean of its samples; the clip value is the clipped fraction times 2, at
+// most 1.
+template <class T> void superpixel(const T* cfa, size_t w, size_t h, const uint8_t* pat, const T* mul, T* rgb, T* clip) {
+    const size_t ow = w / 2, oh = h / 2;
+#pragma omp parallel for schedule(static)
+    for (isz yy = 0; yy < (isz)oh; yy++) {
+        const size_t y = (size_t)yy;
+        for (size_t x = 0; x < ow; x++) {
+            T acc[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+            unsigned n[3] = {0, 0, 0}, clipped = 0;
+            for (size_t dy = 0; dy < 2; dy++)
+                for (size_t dx = 0; dx < 2; dx++) {
+                    const size_t sx = x * 2 + dx, sy = y * 2 + dy;
+                    const int c = bayer_color(pat, sx, sy);
+                    const T v = cfa[sy * w + sx];
+                    if (v >= F(0.995)) clipped++;
+                    mx[c] = vmax(mx[c], v);
+                    acc[c] += v;
+                    n[c]++;
+                }
+            for (int c = 0; c < 3; c++) {
+                const T v = mx[c] >= F(0.995) ? mx[c] : (n[c] > 0 ? acc[c] / (T)n[c] : (T)0);
+                rgb[3 * (y * ow + x) + c] = v * mul[c];
+            }
+            clip[y * ow + x] = vmin((T)clipped / F(4.0) * F(2.0), (T)1);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Enhance: enhance.rs
 
@@ -800,6 +863,14 @@ extern "C" {
     void or_cfa_normalize_u16_##S(const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t* pat, T black,              \
                                   const T* white, T* out) {                                                                        \
         cfa_normalize_u16<T>(raw, w, h, stride, pat, black, white, out);                                                            \
+    }                                                                                                                              \
+    void or_cfa_ingest_u16_##S(const uint16_t* raw, size_t width, size_t cx, size_t cy, size_t cw, size_t ch, const uint8_t* pat,     \
+                               const T* black, size_t nblack, size_t bw, size_t bh, size_t bcpp, const T* white, size_t nwhite, T* out,   \
+                               uint8_t* pat_out) {                                                                                   \
+        cfa_ingest_u16<T>(raw, width, cx, cy, cw, ch, pat, black, nblack, bw, bh, bcpp, white, nwhite, out, pat_out);                 \
+    }                                                                                                                              \
+    void or_superpixel_##S(const T* cfa, size_t w, size_t h, const uint8_t* pat, const T* mul, T* rgb, T* clip) {                  \
+        superpixel<T>(cfa, w, h, pat, mul, rgb, clip);                                                                             \
     }                                                                                                                              \
     void or_mhc_bayer_##S(const T* cfa, size_t w, size_t h, const uint8_t* pat, const T* mul, T* rgb) {                            \
         mhc_bayer<T>(cfa, w, h, pat, mul, rgb);                                                                                    \
