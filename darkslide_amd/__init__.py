@@ -47,6 +47,17 @@ DENOISE_NONE, DENOISE_NLM, DENOISE_LEARNED = 0, 1, 2
 QUALITY_FULL, QUALITY_DRAFT = 0, 1
 
 
+class NrParams(ctypes.Structure):
+    """ds_nr_params: NoiseReduction (params.rs:319-326), 0..100 units."""
+    _fields_ = [("luminance", ctypes.c_float), ("luminance_detail", ctypes.c_float), ("luminance_contrast", ctypes.c_float),
+                ("color", ctypes.c_float), ("color_detail", ctypes.c_float), ("color_smoothness", ctypes.c_float)]
+
+
+class SharpenParams(ctypes.Structure):
+    """ds_sharpen_params: Sharpening (params.rs:304-309)."""
+    _fields_ = [("amount", ctypes.c_float), ("radius", ctypes.c_float), ("detail", ctypes.c_float), ("masking", ctypes.c_float)]
+
+
 class RawCfa(ctypes.Structure):
     """ds_raw_cfa: a Bayer raw buffer with crop, black level table and white levels."""
     _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_size_t), ("height", ctypes.c_size_t), ("crop_x", ctypes.c_size_t),
@@ -115,6 +126,14 @@ _SIGS = {
     "ds_demosaic_f32": [_P, _P, _SZ, _SZ, _P, _P, _I, _P, _P],
     "ds_cfa_clip_levels_f32": [_P, _P, _SZ, _SZ, _P, _P],
     "ds_cfa_ingest": [_P, _P, _P, _P],
+    "ds_noise_reduction": [_P, _P, _SZ, _SZ, _P, _P, _F],
+    "ds_noise_reduction_dev": [_P, _P, _SZ, _SZ, _P, _P, _F],
+    "ds_sharpen": [_P, _P, _SZ, _SZ, _P, _P, _F],
+    "ds_sharpen_dev": [_P, _P, _SZ, _SZ, _P, _P, _F],
+    "ds_fast_guided_filter": [_P, _P, _P, _SZ, _SZ, _SZ, _F, _SZ, _P],
+    "ds_fast_guided_filter_dev": [_P, _P, _P, _SZ, _SZ, _SZ, _F, _SZ, _P],
+    "ds_plane_resize": [_P, _P, _SZ, _SZ, _SZ, _SZ, _P],
+    "ds_plane_resize_dev": [_P, _P, _SZ, _SZ, _SZ, _SZ, _P],
     "ds_cfa_ingest_dev": [_P, _P, _P, _P],
     "ds_demosaic_raw": [_P, _P, _P, _I, _I, _P, _P],
     "ds_demosaic_raw_dev": [_P, _P, _P, _I, _I, _P, _P],
@@ -360,6 +379,38 @@ class Context:
         cl = np.empty((h // 2, w // 2), np.float32)
         self._check(lib().ds_superpixel_f32(self.handle, _ptr(cfa), w, h, _pat(pattern), _f3(mul), _ptr(rgb), _ptr(cl)))
         return rgb, cl
+
+    def noise_reduction(self, rgb, nr, field=None, scale=1.0):
+        """pipeline/detail.rs:13 noise_reduction -> new (h, w, 3) array; nr = (luminance, luminance_detail,
+        luminance_contrast, color, color_detail, color_smoothness)."""
+        out = np.array(rgb, np.float32, order="C", copy=True)
+        h, w, _ = out.shape
+        f = None if field is None else np.ascontiguousarray(field, np.float32)
+        self._check(lib().ds_noise_reduction(self.handle, _ptr(out), w, h, ctypes.byref(NrParams(*nr)), _ptr(f), float(scale)))
+        return out
+
+    def sharpen(self, rgb, sh, field=None, scale=1.0):
+        """pipeline/detail.rs:96 sharpen -> new array; sh = (amount, radius, detail, masking)."""
+        out = np.array(rgb, np.float32, order="C", copy=True)
+        h, w, _ = out.shape
+        f = None if field is None else np.ascontiguousarray(field, np.float32)
+        self._check(lib().ds_sharpen(self.handle, _ptr(out), w, h, ctypes.byref(SharpenParams(*sh)), _ptr(f), float(scale)))
+        return out
+
+    def fast_guided_filter(self, I, p, r, eps, sub):
+        I = np.ascontiguousarray(I, np.float32)
+        p = np.ascontiguousarray(p, np.float32)
+        h, w = I.shape
+        out = np.empty_like(I)
+        self._check(lib().ds_fast_guided_filter(self.handle, _ptr(I), _ptr(p), w, h, int(r), float(eps), int(sub), _ptr(out)))
+        return out
+
+    def plane_resize(self, p, w2, h2):
+        p = np.ascontiguousarray(p, np.float32)
+        h, w = p.shape
+        out = np.empty((h2, w2), np.float32)
+        self._check(lib().ds_plane_resize(self.handle, _ptr(p), w, h, int(w2), int(h2), _ptr(out)))
+        return out
 
     def cfa_clip_levels(self, cfa, pattern=RGGB):
         """demosaic.rs:419 cfa_clip_levels on a normalised CFA -> [3] float32."""

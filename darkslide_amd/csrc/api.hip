@@ -17,6 +17,7 @@
 #include "darkslide_gpu.h"
 #include "demosaic.h"
 #include "denoise.h"
+#include "detail.h"
 #include "highlights.h"
 #include "nlm.h"
 #include "box.h"
@@ -506,6 +507,30 @@ int upload_raw(ds_ctx* c, const ds_raw_cfa* raw, RawView& v) {
     if (n) HIPCHK(c, hipMemcpyAsync(d_raw, v.raw, n * 2, hipMemcpyHostToDevice, c->stream), "H2D");
     v.raw = d_raw;
     return DS_OK;
+}
+
+NrParams nr_params(const ds_nr_params& p) {
+    return NrParams{p.luminance, p.luminance_detail, p.luminance_contrast, p.color, p.color_detail, p.color_smoothness};
+}
+
+int noise_reduction_dev(ds_ctx* c, float* rgb, size_t w, size_t h, const ds_nr_params* nr, const float* field, float field_max,
+                        float scale) {
+    SLOT(c, ws, float*, S_WS, (nr_ws_floats(w, h) ? nr_ws_floats(w, h) : 1) * 4);
+    const NrParams P = nr_params(*nr);
+    return run(c, "noise_reduction", [&] { return launch_noise_reduction(rgb, w, h, P, field, field_max, scale, c->ok, ws, c->stream); });
+}
+
+int sharpen_dev(ds_ctx* c, float* rgb, size_t w, size_t h, const ds_sharpen_params* sh, const float* field, float scale) {
+    SLOT(c, ws, float*, S_WS, (sharpen_ws_floats(w, h) ? sharpen_ws_floats(w, h) : 1) * 4);
+    const SharpParams P{sh->amount, sh->radius, sh->detail, sh->masking};
+    return run(c, "sharpen", [&] { return launch_sharpen(rgb, w, h, P, field, scale, ws, c->stream); });
+}
+
+int fgf_dev(ds_ctx* c, const float* I, const float* p, size_t w, size_t h, size_t r, float eps, size_t sub, float* out) {
+    if (w * h == 0) return DS_OK;
+    if (r > 1024) return fail(c, DS_ERR_INVALID, "radius %zu too large", r);
+    SLOT(c, ws, float*, S_WS, fgf_ws_floats(w, h, sub) * 4);
+    return run(c, "fast_guided_filter", [&] { return launch_fast_guided_filter(I, p, w, h, r, eps, sub, ws, out, c->stream); });
 }
 
 // reconstruct_highlights (demosaic.rs:435-502) in place; raw_clip on the device.
@@ -1552,6 +1577,46 @@ int ds_demosaic_f32_dev(ds_ctx* c, const float* cfa, size_t w, size_t h, const u
     return demosaic_dev(c, cfa, nullptr, w, h, w, pattern, 0.0f, nullptr, nullptr, mul, algo, rgb_out, clip_out);
 }
 
+int ds_noise_reduction_dev(ds_ctx* c, float* rgb, size_t w, size_t h, const ds_nr_params* nr, const float* noise_field, float scale) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!nr || (!rgb && w * h)) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    float fmax = 0.0f;
+    if (noise_field) {
+        SLOT(c, red, float*, S_HL, HL_FLOATS * 4);
+        HIPCHK(c, launch_field_max(noise_field, n, red + HL_PART_OFF, red, c->stream), "field max");
+        HIPCHK(c, hipMemcpyAsync(&fmax, red, 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    }
+    return noise_reduction_dev(c, rgb, w, h, nr, noise_field, fmax, scale);
+}
+
+int ds_sharpen_dev(ds_ctx* c, float* rgb, size_t w, size_t h, const ds_sharpen_params* sh, const float* sharpness_field, float scale) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!sh || (!rgb && w * h)) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (w * h == 0) return DS_OK;
+    return sharpen_dev(c, rgb, w, h, sh, sharpness_field, scale);
+}
+
+int ds_fast_guided_filter_dev(ds_ctx* c, const float* I, const float* p, size_t w, size_t h, size_t r, float eps, size_t sub, float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if ((!I || !p || !out) && w * h) return fail(c, DS_ERR_INVALID, "null pointer");
+    return fgf_dev(c, I, p, w, h, r, eps, sub, out);
+}
+
+int ds_plane_resize_dev(ds_ctx* c, const float* in, size_t w, size_t h, size_t w2, size_t h2, float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (int rc = check_dims(c, w2, h2)) return rc;
+    if ((!in || !out) && w2 * h2) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (w2 * h2 && w * h == 0) return fail(c, DS_ERR_INVALID, "resize of an empty plane");
+    return run(c, "plane_resize", [&] { return launch_plane_resize(in, w, h, w2, h2, out, c->stream); });
+}
+
 int ds_cfa_ingest_dev(ds_ctx* c, const ds_raw_cfa* raw, float* cfa_out, uint8_t pattern_out[4]) {
     LOCK(c);
     RawView v{};
@@ -1674,6 +1739,81 @@ int ds_demosaic_u16(ds_ctx* c, const uint16_t* raw, size_t w, size_t h, size_t s
     if (int rc = demosaic_dev(c, nullptr, d_raw, w, h, stride, pattern, black, nullptr, white, mul, algo, d_rgb, d_clip)) return rc;
     HIPCHK(c, hipMemcpyAsync(rgb_out, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
     if (clip_out) HIPCHK(c, hipMemcpyAsync(clip_out, d_clip, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_noise_reduction(ds_ctx* c, float* rgb, size_t w, size_t h, const ds_nr_params* nr, const float* noise_field, float scale) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!nr || (!rgb && w * h)) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    SLOT(c, d_rgb, float*, S_OUT, n * 12);
+    float* d_field = nullptr;
+    float fmax = 0.0f;
+    if (noise_field) {
+        for (size_t i = 0; i < n; i++) fmax = std::fmax(fmax, noise_field[i]);  // the fold at detail.rs:34
+        SLOT(c, df, float*, S_CLIP, n * 4);
+        HIPCHK(c, hipMemcpyAsync(df, noise_field, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        d_field = df;
+    }
+    HIPCHK(c, hipMemcpyAsync(d_rgb, rgb, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = noise_reduction_dev(c, d_rgb, w, h, nr, d_field, fmax, scale)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_sharpen(ds_ctx* c, float* rgb, size_t w, size_t h, const ds_sharpen_params* sh, const float* sharpness_field, float scale) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!sh || (!rgb && w * h)) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    SLOT(c, d_rgb, float*, S_OUT, n * 12);
+    float* d_field = nullptr;
+    if (sharpness_field) {
+        SLOT(c, df, float*, S_CLIP, n * 4);
+        HIPCHK(c, hipMemcpyAsync(df, sharpness_field, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        d_field = df;
+    }
+    HIPCHK(c, hipMemcpyAsync(d_rgb, rgb, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = sharpen_dev(c, d_rgb, w, h, sh, d_field, scale)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_fast_guided_filter(ds_ctx* c, const float* I, const float* p, size_t w, size_t h, size_t r, float eps, size_t sub, float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if ((!I || !p || !out) && w * h) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    SLOT(c, dI, float*, S_IN, n * 4);
+    SLOT(c, dP, float*, S_IN2, n * 4);
+    SLOT(c, dO, float*, S_OUT, n * 4);
+    HIPCHK(c, hipMemcpyAsync(dI, I, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    HIPCHK(c, hipMemcpyAsync(dP, p, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = fgf_dev(c, dI, dP, w, h, r, eps, sub, dO)) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, dO, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_plane_resize(ds_ctx* c, const float* in, size_t w, size_t h, size_t w2, size_t h2, float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (int rc = check_dims(c, w2, h2)) return rc;
+    if ((!in || !out) && w2 * h2) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (w2 * h2 == 0) return DS_OK;
+    if (w * h == 0) return fail(c, DS_ERR_INVALID, "resize of an empty plane");
+    SLOT(c, dI, float*, S_IN, w * h * 4);
+    SLOT(c, dO, float*, S_OUT, w2 * h2 * 4);
+    HIPCHK(c, hipMemcpyAsync(dI, in, w * h * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    HIPCHK(c, launch_plane_resize(dI, w, h, w2, h2, dO, c->stream), "plane_resize");
+    HIPCHK(c, hipMemcpyAsync(out, dO, w2 * h2 * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;
 }

@@ -135,13 +135,12 @@ hipError_t launch_gaussian_blur(const float* in, float* ws, float* out, size_t w
     return e;
 }
 
-hipError_t launch_guided_filter(const float* I, const float* P, size_t w, size_t h, size_t r, float eps, float* ws, float* out,
-                                hipStream_t st) {
+hipError_t launch_guided_coeffs(const float* I, const float* P, size_t w, size_t h, size_t r, float eps, float* ws, hipStream_t st) {
     const size_t n = w * h;
     if (n == 0) return hipSuccess;
     float* t[8];
     for (int k = 0; k < 8; k++) t[k] = ws + (size_t)k * n;
-    BoxPlanes h4{}, v4{}, ab{}, hab{}, vab{};
+    BoxPlanes h4{}, v4{}, hab{}, vab{};
     CBoxPlanes c4{}, cab{}, chab{};
     for (int k = 0; k < 4; k++) {
         h4.p[k] = t[k];
@@ -160,12 +159,19 @@ hipError_t launch_guided_filter(const float* I, const float* P, size_t w, size_t
     chab.p[1] = t[3];
     vab.p[0] = t[4];
     vab.p[1] = t[5];
-    (void)ab;
     e = hipGetLastError();
     if (e == hipSuccess) e = box_h_planes(cab, hab, 2, w, h, r, st);
     if (e == hipSuccess) e = box_v_planes(chab, vab, 2, w, h, r, st);
+    return e;
+}
+
+hipError_t launch_guided_filter(const float* I, const float* P, size_t w, size_t h, size_t r, float eps, float* ws, float* out,
+                                hipStream_t st) {
+    const size_t n = w * h;
+    if (n == 0) return hipSuccess;
+    hipError_t e = launch_guided_coeffs(I, P, w, h, r, eps, ws, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gf1_final, dim3(nblk(n)), dim3(256), 0, st, I, t[4], t[5], n, out);
+    hipLaunchKernelGGL(k_gf1_final, dim3(nblk(n)), dim3(256), 0, st, I, ws + 4 * n, ws + 5 * n, n, out);
     return hipGetLastError();
 }
 

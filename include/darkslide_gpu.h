@@ -125,6 +125,31 @@ int ds_cfa_clip_levels_u16(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h,
  * result.  As called by Engine::base (pipeline/mod.rs:186-187). */
 int ds_reconstruct_highlights(ds_ctx* ctx, float* rgb, const float* clip, size_t w, size_t h, const float mul[3], const float raw_clip[3]);
 
+/* ---- detail (pipeline/detail.rs) ----------------------------------------- */
+
+/* NoiseReduction (params.rs:319-326) and Sharpening (params.rs:304-309), in
+ * the reference's 0..100 units. */
+typedef struct ds_nr_params {
+    float luminance, luminance_detail, luminance_contrast, color, color_detail, color_smoothness;
+} ds_nr_params;
+typedef struct ds_sharpen_params {
+    float amount, radius, detail, masking;
+} ds_sharpen_params;
+
+/* detail::noise_reduction(img, nr, fields, scale, ok) (pipeline/detail.rs:13),
+ * in place on interleaved linear ProPhoto (w*h*3).  noise_field: the
+ * LocalFields noise plane (w*h, the reference's units) or NULL when no mask
+ * sets it; scale: output pixels per full-resolution pixel. */
+int ds_noise_reduction(ds_ctx* ctx, float* rgb, size_t w, size_t h, const ds_nr_params* nr, const float* noise_field, float scale);
+
+/* detail::sharpen(img, sh, fields, scale) (pipeline/detail.rs:96), in place;
+ * sharpness_field: the LocalFields sharpness plane or NULL. */
+int ds_sharpen(ds_ctx* ctx, float* rgb, size_t w, size_t h, const ds_sharpen_params* sh, const float* sharpness_field, float scale);
+
+/* fast_guided_filter(i, p, r, eps, sub) (image.rs:408) and Plane::resize(w2, h2) (image.rs:238). */
+int ds_fast_guided_filter(ds_ctx* ctx, const float* I, const float* p, size_t w, size_t h, size_t r, float eps, size_t sub, float* out);
+int ds_plane_resize(ds_ctx* ctx, const float* in, size_t w, size_t h, size_t w2, size_t h2, float* out);
+
 /* ---- enhance ----------------------------------------------------------- */
 
 /* enhance.rs:10 estimate_noise(p) */
@@ -183,6 +208,13 @@ int ds_demosaic_raw_dev(ds_ctx* ctx, const ds_raw_cfa* raw, const float mul[3], 
 int ds_cfa_clip_levels_raw_dev(ds_ctx* ctx, const ds_raw_cfa* raw, float* levels_out);
 int ds_superpixel_f32_dev(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], const float mul[3], float* rgb_out,
                           float* clip_out);
+/* detail stages on device buffers (noise_field / sharpness_field too); the
+ * noise reduction reads the field's maximum back to the host when a field is given. */
+int ds_noise_reduction_dev(ds_ctx* ctx, float* rgb, size_t w, size_t h, const ds_nr_params* nr, const float* noise_field,
+                           float scale);
+int ds_sharpen_dev(ds_ctx* ctx, float* rgb, size_t w, size_t h, const ds_sharpen_params* sh, const float* sharpness_field, float scale);
+int ds_fast_guided_filter_dev(ds_ctx* ctx, const float* I, const float* p, size_t w, size_t h, size_t r, float eps, size_t sub, float* out);
+int ds_plane_resize_dev(ds_ctx* ctx, const float* in, size_t w, size_t h, size_t w2, size_t h2, float* out);
 /* levels_out: 3 device floats */
 int ds_cfa_clip_levels_f32_dev(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], float* levels_out);
 int ds_cfa_clip_levels_u16_dev(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4],

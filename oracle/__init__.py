@@ -338,3 +338,59 @@ def superpixel(cfa, pattern=(0, 1, 1, 2), mul=(1, 1, 1)):
     m = np.asarray(mul, cfa.dtype)
     f(_p(cfa), ctypes.c_size_t(w), ctypes.c_size_t(h), _pat(pattern), _p(m), _p(rgb), _p(clip))
     return rgb, clip
+
+
+def plane_resize(p, w2, h2):
+    """image.rs:238-262 Plane::resize."""
+    p = np.ascontiguousarray(p)
+    h, w = p.shape
+    out = np.empty((h2, w2), p.dtype)
+    f, _ = _fn("or_plane_resize", p.dtype)
+    f(_p(p), ctypes.c_size_t(w), ctypes.c_size_t(h), ctypes.c_size_t(w2), ctypes.c_size_t(h2), _p(out))
+    return out
+
+
+def fast_guided_filter(I, p, r, eps, sub):
+    """image.rs:408-431."""
+    I = np.ascontiguousarray(I)
+    p = np.ascontiguousarray(p, I.dtype)
+    h, w = I.shape
+    out = np.empty_like(I)
+    f, ct = _fn("or_fast_guided_filter", I.dtype)
+    f(_p(I), _p(p), ctypes.c_size_t(w), ctypes.c_size_t(h), ctypes.c_size_t(r), ct(eps), ctypes.c_size_t(sub), _p(out))
+    return out
+
+
+def noise_reduction(img, nr, field=None, scale=1.0):
+    """pipeline/detail.rs:13-92 on a copy of img; nr = (luminance, luminance_detail, luminance_contrast,
+    color, color_detail, color_smoothness) in the reference's 0..100 units."""
+    img = np.array(img, copy=True, order="C")
+    h, w, _ = img.shape
+    f, ct = _fn("or_noise_reduction", img.dtype)
+    prm = np.asarray(nr, img.dtype)
+    fld = None if field is None else np.ascontiguousarray(field, img.dtype)
+    f(_p(img), ctypes.c_size_t(w), ctypes.c_size_t(h), _p(prm), _p(fld) if fld is not None else None, ct(scale))
+    return img
+
+
+def sharpen(img, sh, field=None, scale=1.0):
+    """pipeline/detail.rs:96-139 on a copy of img; sh = (amount, radius, detail, masking)."""
+    img = np.array(img, copy=True, order="C")
+    h, w, _ = img.shape
+    f, ct = _fn("or_sharpen", img.dtype)
+    prm = np.asarray(sh, img.dtype)
+    fld = None if field is None else np.ascontiguousarray(field, img.dtype)
+    f(_p(img), ctypes.c_size_t(w), ctypes.c_size_t(h), _p(prm), _p(fld) if fld is not None else None, ct(scale))
+    return img
+
+
+def tone_enc(v, dtype=np.float32):
+    f, ct = _fn("or_tone_enc", dtype, None)
+    f.restype = ct
+    return float(f(ct(v)))
+
+
+def tone_dec(v, dtype=np.float32):
+    f, ct = _fn("or_tone_dec", dtype, None)
+    f.restype = ct
+    return float(f(ct(v)))

@@ -853,6 +853,218 @@ template <class T> void reconstruct_highlights(T* img, const T* clip, size_t w, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Detail: pipeline/detail.rs (noise_reduction, sharpen) and the helpers they
+// use: Plane::resize / sample_bilinear (image.rs:217-262), fast_guided_filter
+// (image.rs:408-431), srgb_encode / srgb_decode (color.rs:185-192), enc / dec
+// and smoothstep (pipeline/tone.rs:16-19, :328-335).
+
+template <class T> inline T t_pow(T a, T b);
+template <> inline float t_pow<float>(float a, float b) { return powf(a, b); }
+template <> inline double t_pow<double>(double a, double b) { return pow(a, b); }
+template <class T> inline T t_round(T v);
+template <> inline float t_round<float>(float v) { return roundf(v); }
+template <> inline double t_round<double>(double v) { return round(v); }
+
+template <class T> inline T srgb_encode(T v) { return v <= F(0.0031308) ? F(12.92) * vmax(v, (T)0) : F(1.055) * t_pow(v, F(1.0) / F(2.4)) - F(0.055); }
+template <class T> inline T srgb_decode(T v) { return v <= F(0.04045) ? v / F(12.92) : t_pow((v + F(0.055)) / F(1.055), F(2.4)); }
+template <class T> inline T tone_enc(T v) { return v <= (T)1 ? srgb_encode(v) : (T)1 + (v - (T)1) / F(2.4); }
+template <class T> inline T tone_dec(T e) { return e <= (T)1 ? srgb_decode(e) : (T)1 + (e - (T)1) * F(2.4); }
+template <class T> inline T smoothstep(T a, T b, T x) {
+    const T t = vclamp((x - a) / (b - a), (T)0, (T)1);
+    return t * t * (F(3.0) - F(2.0) * t);
+}
+
+// Plane::resize (image.rs:238-262): box average when shrinking by more than
+// 1.5x on an axis, else bilinear at pixel centres.
+template <class T> void plane_resize(const T* p, size_t w, size_t h, size_t w2, size_t h2, T* out) {
+    if (w2 == w && h2 == h) {
+        std::memcpy(out, p, w * h * sizeof(T));
+        return;
+    }
+    // Plane::resize computes the scale in f32 whatever the element type
+    const float sx = (float)w / (float)w2, sy = (float)h / (float)h2;
+    auto at = [&](isz x, isz y) { return p[(size_t)iclamp(y, 0, (isz)h - 1) * w + (size_t)iclamp(x, 0, (isz)w - 1)]; };
+    if (sx > 1.5f || sy > 1.5f) {
+#pragma omp parallel for schedule(static)
+        for (isz yy = 0; yy < (isz)h2; yy++) {
+            const size_t y = (size_t)yy;
+            for (size_t x = 0; x < w2; x++) {
+                const size_t x0 = (size_t)((float)x * sx);
+                size_t x1 = (size_t)((float)(x + 1) * sx);
+                x1 = x1 < x0 + 1 ? x0 + 1 : (x1 > w ? w : x1);
+                const size_t y0 = (size_t)((float)y * sy);
+                size_t y1 = (size_t)((float)(y + 1) * sy);
+                y1 = y1 < y0 + 1 ? y0 + 1 : (y1 > h ? h : y1);
+                T acc = 0;
+                for (size_t v = y0; v < y1; v++)
+                    for (size_t u = x0; u < x1; u++) acc += p[v * w + u];
+                out[y * w2 + x] = acc / (T)((x1 - x0) * (y1 - y0));
+            }
+        }
+    } else {
+#pragma omp parallel for schedule(static)
+        for (isz yy = 0; yy < (isz)h2; yy++) {
+            const size_t y = (size_t)yy;
+            for (size_t x = 0; x < w2; x++) {
+                const float fx = ((float)x + 0.5f) * sx - 0.5f, fy = ((float)y + 0.5f) * sy - 0.5f;
+                const float x0 = std::floor(fx), y0 = std::floor(fy);
+                const T ax = (T)(fx - x0), ay = (T)(fy - y0);
+                const isz ix = (isz)x0, iy = (isz)y0;
+                const T a = at(ix, iy) * ((T)1 - ax) + at(ix + 1, iy) * ax;
+                const T b = at(ix, iy + 1) * ((T)1 - ax) + at(ix + 1, iy + 1) * ax;
+                out[y * w2 + x] = a + (b - a) * ay;
+            }
+        }
+    }
+}
+
+// fast_guided_filter (image.rs:408-431): coefficients on a 1/sub grid.
+template <class T> void fast_guided_filter(const T* I, const T* p, size_t w, size_t h, size_t r, T eps, size_t sub, T* out) {
+    if (sub <= 1 || w / sub < 8 || h / sub < 8) {
+        guided_filter(I, p, w, h, r, eps, out);
+        return;
+    }
+    const size_t sw = w / sub, sh = h / sub, n = sw * sh, rs = r / sub > 1 ? r / sub : 1;
+    std::vector<T> is(n), ps(n), ip(n), ii(n), mi(n), mp(n), mip(n), mii(n), a(n), b(n), ba(n), bb(n), ma(w * h), mb(w * h);
+    plane_resize(I, w, h, sw, sh, is.data());
+    plane_resize(p, w, h, sw, sh, ps.data());
+    for (size_t k = 0; k < n; k++) {
+        ip[k] = is[k] * ps[k];
+        ii[k] = is[k] * is[k];
+    }
+    box_blur(is.data(), sw, sh, rs, mi.data());
+    box_blur(ps.data(), sw, sh, rs, mp.data());
+    box_blur(ip.data(), sw, sh, rs, mip.data());
+    box_blur(ii.data(), sw, sh, rs, mii.data());
+    for (size_t k = 0; k < n; k++) {
+        const T cov = mip[k] - mi[k] * mp[k];
+        const T var = mii[k] - mi[k] * mi[k];
+        a[k] = cov / (var + eps);
+        b[k] = mp[k] - a[k] * mi[k];
+    }
+    box_blur(a.data(), sw, sh, rs, ba.data());
+    box_blur(b.data(), sw, sh, rs, bb.data());
+    plane_resize(ba.data(), sw, sh, w, h, ma.data());
+    plane_resize(bb.data(), sw, sh, w, h, mb.data());
+    for (size_t k = 0; k < w * h; k++) out[k] = ma[k] * I[k] + mb[k];
+}
+
+// detail::noise_reduction (pipeline/detail.rs:13-92) in place on interleaved
+// ProPhoto.  nr = {luminance, luminance_detail, luminance_contrast, color,
+// color_detail, color_smoothness} (params.rs:319-326); field: the
+// LocalFields noise plane or null.
+template <class T> void noise_reduction(T* img, size_t w, size_t h, const T* nr, const T* field, T scale) {
+    const bool local = field != nullptr;
+    const T lum = nr[0] / F(100.0), col = nr[3] / F(100.0);
+    if (lum <= (T)0 && col <= (T)0 && !local) return;
+    if (scale < F(0.34)) return;
+    const size_t n = w * h;
+    const T eff = vclamp(scale, F(0.05), F(1.0));
+    Oklab<T> ok;
+    std::vector<T> l(n), a(n), b(n);
+    for (size_t i = 0; i < n; i++) {
+        T lab[3];
+        ok.from_prophoto(img + 3 * i, lab);
+        l[i] = lab[0];
+        a[i] = lab[1];
+        b[i] = lab[2];
+    }
+    bool changed = false;
+    if (lum > (T)0 || local) {
+        T fmax = 0;
+        if (local)
+            for (size_t i = 0; i < n; i++) fmax = vmax(fmax, field[i]);
+        const T max_strength = vclamp(lum + (local ? fmax / F(100.0) : (T)0), (T)0, (T)1);
+        if (max_strength > (T)0) {
+            const T detail = nr[1] / F(100.0);
+            const T sigma = F(0.004) + max_strength * F(0.045) * t_sqrt(eff);
+            const T q = sigma * (F(1.4) - detail);
+            const T eps = q * q;
+            const T rr1 = t_round(F(1.5) * eff), rr2 = t_round(F(4.0) * eff);
+            const size_t r1 = rr1 > (T)1 ? (size_t)rr1 : 1, r2 = rr2 > (T)1 ? (size_t)rr2 : 1;
+            std::vector<T> f1(n), f2(n);
+            guided_filter(l.data(), l.data(), w, h, r1, eps, f1.data());
+            guided_filter(f1.data(), f1.data(), w, h, r2, eps * F(0.5), f2.data());
+            const T contrast = nr[2] / F(100.0);
+            for (size_t i = 0; i < n; i++) {
+                T s = lum + (local ? field[i] : (T)0) / F(100.0);
+                s = vclamp(s, (T)0, (T)1);
+                const T filt = f2[i] + (l[i] - f2[i]) * contrast * F(0.35);
+                l[i] = l[i] + (filt - l[i]) * vmin(s / vmax(max_strength, F(1e-6)), (T)1);
+            }
+            changed = true;
+        }
+    }
+    if (col > (T)0) {
+        const T detail = nr[4] / F(100.0), smooth = nr[5] / F(100.0);
+        const T rf = vmax(t_round((F(2.0) + col * F(10.0)) * (F(1.0) + smooth * F(1.5)) * eff), F(1.0));
+        const size_t r = (size_t)rf;
+        const T q = F(0.002) + (F(1.0) - detail) * F(0.02);
+        const T eps = q * q;
+        const size_t sub = r >= 8 ? 4 : (r >= 3 ? 2 : 1);
+        std::vector<T> fa(n), fb(n);
+        fast_guided_filter(l.data(), a.data(), w, h, r, eps, sub, fa.data());
+        fast_guided_filter(l.data(), b.data(), w, h, r, eps, sub, fb.data());
+        const T k = vmin(col, F(1.0));
+        for (size_t i = 0; i < n; i++) {
+            a[i] = a[i] + (fa[i] - a[i]) * k;
+            b[i] = b[i] + (fb[i] - b[i]) * k;
+        }
+        changed = true;
+    }
+    if (changed)
+        for (size_t i = 0; i < n; i++) {
+            const T lab[3] = {l[i], a[i], b[i]};
+            T o[3];
+            ok.to_prophoto(lab, o);
+            for (int c = 0; c < 3; c++) img[3 * i + c] = vmax(o[c], (T)0);
+        }
+}
+
+// detail::sharpen (pipeline/detail.rs:96-139) in place.  sh = {amount,
+// radius, detail, masking} (params.rs:304-309); field: the LocalFields
+// sharpness plane or null.
+template <class T> void sharpen(T* img, size_t w, size_t h, const T* sh, const T* field, T scale) {
+    const bool local = field != nullptr;
+    if (sh[0] <= (T)0 && !local) return;
+    const T sigma = sh[1] * vmin(scale, F(1.0));
+    if (sigma < F(0.25)) return;
+    const size_t n = w * h;
+    std::vector<T> e(n), blur(n), s, m;
+    for (size_t i = 0; i < n; i++) e[i] = tone_enc(vmax(luma_prophoto(img + 3 * i), (T)0));
+    gaussian_blur(e.data(), w, h, sigma, blur.data());
+    const T detail = sh[2] / F(100.0);
+    const bool mask = sh[3] > (T)0;
+    if (mask) {
+        s.resize(n);
+        m.resize(n);
+        gaussian_blur(e.data(), w, h, F(1.0) * vmax(scale, F(0.5)), s.data());
+        const T t = sh[3] / F(100.0) * F(0.06);
+        auto at = [&](isz x, isz y) { return s[(size_t)iclamp(y, 0, (isz)h - 1) * w + (size_t)iclamp(x, 0, (isz)w - 1)]; };
+        for (size_t y = 0; y < h; y++)
+            for (size_t x = 0; x < w; x++) {
+                const T gx = at((isz)x + 1, (isz)y) - at((isz)x - 1, (isz)y);
+                const T gy = at((isz)x, (isz)y + 1) - at((isz)x, (isz)y - 1);
+                const T g = t_sqrt(gx * gx + gy * gy);
+                m[y * w + x] = smoothstep(t * F(0.5), t * F(1.5) + F(1e-4), g);
+            }
+    }
+    for (size_t i = 0; i < n; i++) {
+        const T amt = (sh[0] + (local ? field[i] : (T)0) * F(1.5)) / F(100.0) * F(1.6);
+        if (amt == (T)0) continue;
+        const T d = e[i] - blur[i];
+        const T damp = detail + (F(1.0) - detail) * t_exp(-std::fabs(d) / F(0.04));
+        T dv = d * damp;
+        if (mask) dv *= m[i];
+        const T e2 = vmax(e[i] + amt * dv, (T)0);
+        const T y0 = vmax(tone_dec(e[i]), F(1e-6));
+        const T y1 = tone_dec(e2);
+        const T r = y1 / y0;
+        for (int c = 0; c < 3; c++) img[3 * i + c] = img[3 * i + c] * r;
+    }
+}
+
 // C ABI
 
 #define OR_BOTH(X) X(float, f32) X(double, f64)
@@ -872,6 +1084,16 @@ extern "C" {
     void or_superpixel_##S(const T* cfa, size_t w, size_t h, const uint8_t* pat, const T* mul, T* rgb, T* clip) {                  \
         superpixel<T>(cfa, w, h, pat, mul, rgb, clip);                                                                             \
     }                                                                                                                              \
+    void or_plane_resize_##S(const T* p, size_t w, size_t h, size_t w2, size_t h2, T* out) { plane_resize<T>(p, w, h, w2, h2, out); } \
+    void or_fast_guided_filter_##S(const T* I, const T* p, size_t w, size_t h, size_t r, T eps, size_t sub, T* out) {             \
+        fast_guided_filter<T>(I, p, w, h, r, eps, sub, out);                                                                       \
+    }                                                                                                                              \
+    void or_noise_reduction_##S(T* img, size_t w, size_t h, const T* nr, const T* field, T scale) {                                \
+        noise_reduction<T>(img, w, h, nr, field, scale);                                                                           \
+    }                                                                                                                              \
+    void or_sharpen_##S(T* img, size_t w, size_t h, const T* sh, const T* field, T scale) { sharpen<T>(img, w, h, sh, field, scale); } \
+    T or_tone_enc_##S(T v) { return tone_enc<T>(v); }                                                                              \
+    T or_tone_dec_##S(T v) { return tone_dec<T>(v); }                                                                              \
     void or_mhc_bayer_##S(const T* cfa, size_t w, size_t h, const uint8_t* pat, const T* mul, T* rgb) {                            \
         mhc_bayer<T>(cfa, w, h, pat, mul, rgb);                                                                                    \
     }                                                                                                                              \
