@@ -58,6 +58,11 @@ class SharpenParams(ctypes.Structure):
     _fields_ = [("amount", ctypes.c_float), ("radius", ctypes.c_float), ("detail", ctypes.c_float), ("masking", ctypes.c_float)]
 
 
+# SharpenFor (export.rs:63-68) and SharpenAmount (export.rs:71-75)
+SHARPEN_FOR_NONE, SHARPEN_FOR_SCREEN, SHARPEN_FOR_MATTE, SHARPEN_FOR_GLOSSY = 0, 1, 2, 3
+SHARPEN_AMOUNT_LOW, SHARPEN_AMOUNT_STANDARD, SHARPEN_AMOUNT_HIGH = 0, 1, 2
+
+
 class RawCfa(ctypes.Structure):
     """ds_raw_cfa: a Bayer raw buffer with crop, black level table and white levels."""
     _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_size_t), ("height", ctypes.c_size_t), ("crop_x", ctypes.c_size_t),
@@ -134,6 +139,10 @@ _SIGS = {
     "ds_fast_guided_filter_dev": [_P, _P, _P, _SZ, _SZ, _SZ, _F, _SZ, _P],
     "ds_plane_resize": [_P, _P, _SZ, _SZ, _SZ, _SZ, _P],
     "ds_plane_resize_dev": [_P, _P, _SZ, _SZ, _SZ, _SZ, _P],
+    "ds_super_res_refine": [_P, _P, _SZ, _SZ],
+    "ds_super_res_refine_dev": [_P, _P, _SZ, _SZ],
+    "ds_output_sharpen": [_P, _P, _SZ, _SZ, _I, _I],
+    "ds_output_sharpen_dev": [_P, _P, _SZ, _SZ, _I, _I],
     "ds_cfa_ingest_dev": [_P, _P, _P, _P],
     "ds_demosaic_raw": [_P, _P, _P, _I, _I, _P, _P],
     "ds_demosaic_raw_dev": [_P, _P, _P, _I, _I, _P, _P],
@@ -395,6 +404,20 @@ class Context:
         h, w, _ = out.shape
         f = None if field is None else np.ascontiguousarray(field, np.float32)
         self._check(lib().ds_sharpen(self.handle, _ptr(out), w, h, ctypes.byref(SharpenParams(*sh)), _ptr(f), float(scale)))
+        return out
+
+    def super_res_refine(self, rgb):
+        """enhance.rs:122 super_res_refine -> new array."""
+        out = np.array(rgb, np.float32, order="C", copy=True)
+        h, w, _ = out.shape
+        self._check(lib().ds_super_res_refine(self.handle, _ptr(out), w, h))
+        return out
+
+    def output_sharpen(self, rgb, mode, amount):
+        """export.rs:199 output_sharpen -> new array; mode SHARPEN_FOR_*, amount SHARPEN_AMOUNT_*."""
+        out = np.array(rgb, np.float32, order="C", copy=True)
+        h, w, _ = out.shape
+        self._check(lib().ds_output_sharpen(self.handle, _ptr(out), w, h, int(mode), int(amount)))
         return out
 
     def fast_guided_filter(self, I, p, r, eps, sub):

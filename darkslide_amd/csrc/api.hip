@@ -533,6 +533,17 @@ int fgf_dev(ds_ctx* c, const float* I, const float* p, size_t w, size_t h, size_
     return run(c, "fast_guided_filter", [&] { return launch_fast_guided_filter(I, p, w, h, r, eps, sub, ws, out, c->stream); });
 }
 
+// export.rs:244-247 post-steps in place on a device image.
+int export_detail_dev(ds_ctx* c, float* rgb, size_t w, size_t h, bool superres, int mode, int amount) {
+    if (!superres && (mode < DS_SHARPEN_FOR_NONE || mode > DS_SHARPEN_FOR_GLOSSY || amount < DS_SHARPEN_AMOUNT_LOW ||
+                      amount > DS_SHARPEN_AMOUNT_HIGH))
+        return fail(c, DS_ERR_INVALID, "output sharpen mode %d / amount %d out of range", mode, amount);
+    if (w * h == 0 || (!superres && mode == DS_SHARPEN_FOR_NONE)) return DS_OK;
+    SLOT(c, ws, float*, S_WS, export_detail_ws_floats(w, h) * 4);
+    if (superres) return run(c, "super_res_refine", [&] { return launch_super_res_refine(rgb, w, h, ws, c->stream); });
+    return run(c, "output_sharpen", [&] { return launch_output_sharpen(rgb, w, h, mode, amount, ws, c->stream); });
+}
+
 // reconstruct_highlights (demosaic.rs:435-502) in place; raw_clip on the device.
 int highlights_dev(ds_ctx* c, float* rgb, const float* clip, size_t w, size_t h, const float* mul, const float* raw_clip) {
     const size_t n = w * h;
@@ -1510,6 +1521,27 @@ const char* ds_last_error(const ds_ctx* c) { return c ? c->err.c_str() : g_creat
         if (rc_) return rc_;                          \
     }
 
+static int export_detail_host(ds_ctx* c, float* rgb, size_t w, size_t h, bool superres, int mode, int amount) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    const size_t n = w * h;
+    if (!rgb && n) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (n == 0) return export_detail_dev(c, rgb, 0, 0, superres, mode, amount);
+    SLOT(c, d_rgb, float*, S_OUT, n * 12);
+    HIPCHK(c, hipMemcpyAsync(d_rgb, rgb, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = export_detail_dev(c, d_rgb, w, h, superres, mode, amount)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb, d_rgb, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+static int export_detail_device(ds_ctx* c, float* rgb, size_t w, size_t h, bool superres, int mode, int amount) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!rgb && w * h) return fail(c, DS_ERR_INVALID, "null pointer");
+    return export_detail_dev(c, rgb, w, h, superres, mode, amount);
+}
+
 int ds_dev_alloc(ds_ctx* c, size_t bytes, void** dptr) {
     LOCK(c);
     if (!dptr) return fail(c, DS_ERR_INVALID, "dptr is null");
@@ -1816,6 +1848,18 @@ int ds_plane_resize(ds_ctx* c, const float* in, size_t w, size_t h, size_t w2, s
     HIPCHK(c, hipMemcpyAsync(out, dO, w2 * h2 * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     return DS_OK;
+}
+
+int ds_super_res_refine(ds_ctx* c, float* rgb, size_t w, size_t h) { return export_detail_host(c, rgb, w, h, true, 0, 0); }
+
+int ds_output_sharpen(ds_ctx* c, float* rgb, size_t w, size_t h, int mode, int amount) {
+    return export_detail_host(c, rgb, w, h, false, mode, amount);
+}
+
+int ds_super_res_refine_dev(ds_ctx* c, float* rgb, size_t w, size_t h) { return export_detail_device(c, rgb, w, h, true, 0, 0); }
+
+int ds_output_sharpen_dev(ds_ctx* c, float* rgb, size_t w, size_t h, int mode, int amount) {
+    return export_detail_device(c, rgb, w, h, false, mode, amount);
 }
 
 int ds_cfa_ingest(ds_ctx* c, const ds_raw_cfa* raw, float* cfa_out, uint8_t pattern_out[4]) {

@@ -33,6 +33,17 @@ hipError_t launch_noise_reduction(float* rgb, size_t w, size_t h, const NrParams
 hipError_t launch_sharpen(float* rgb, size_t w, size_t h, const SharpParams& sh, const float* field, float scale, float* ws,
                           hipStream_t st);
 
+// The export post-steps after render_full (export.rs:244-247), in place on
+// interleaved ProPhoto.  ws: export_detail_ws_floats(w, h) floats.
+size_t export_detail_ws_floats(size_t w, size_t h);
+// enhance.rs:122-131 super_res_refine
+hipError_t launch_super_res_refine(float* rgb, size_t w, size_t h, float* ws, hipStream_t st);
+// export.rs:199-223 output_sharpen; mode SharpenFor (export.rs:63-68),
+// amount SharpenAmount (export.rs:71-75), numbered in declaration order.
+enum { OUT_SHARPEN_NONE = 0, OUT_SHARPEN_SCREEN = 1, OUT_SHARPEN_MATTE = 2, OUT_SHARPEN_GLOSSY = 3 };
+enum { OUT_AMOUNT_LOW = 0, OUT_AMOUNT_STANDARD = 1, OUT_AMOUNT_HIGH = 2 };
+hipError_t launch_output_sharpen(float* rgb, size_t w, size_t h, int mode, int amount, float* ws, hipStream_t st);
+
 // max(0, max of field) (the fold at detail.rs:34) into out (1 device float);
 // partial: FIELD_MAX_PARTS floats.
 constexpr int FIELD_MAX_PARTS = 256;

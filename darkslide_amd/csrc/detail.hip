@@ -312,4 +312,73 @@ hipError_t launch_sharpen(float* rgb, size_t w, size_t h, const SharpParams& sh,
     return hipGetLastError();
 }
 
+// enhance.rs:123: y = luminance().max(0).sqrt()
+__global__ void k_superres_y(const float* __restrict__ rgb, size_t n, float* __restrict__ y) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float l = 0.2880402f * rgb[3 * i] + 0.7118741f * rgb[3 * i + 1] + 0.0000857f * rgb[3 * i + 2];
+    y[i] = sqrtf(fmaxr(l, 0.0f));
+}
+
+// enhance.rs:125-130
+__global__ void k_superres_apply(float* __restrict__ rgb, const float* __restrict__ y, const float* __restrict__ b, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float yv = y[i];
+    const float d = clampr(yv - b[i], -0.05f, 0.05f);
+    const float nv = fmaxr(yv + d * 0.9f, 0.0f);
+    const float r = clampr((nv * nv) / fmaxr(yv * yv, 1e-8f), 0.5f, 2.0f);
+    rgb[3 * i] = rgb[3 * i] * r;
+    rgb[3 * i + 1] = rgb[3 * i + 1] * r;
+    rgb[3 * i + 2] = rgb[3 * i + 2] * r;
+}
+
+// export.rs:216-222
+__global__ void k_outsharp_apply(float* __restrict__ rgb, const float* __restrict__ l, const float* __restrict__ b, size_t n, float k) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float lv = l[i];
+    const float d = lv - b[i];
+    const float e2 = fmaxr(lv + k * clampr(d, -0.1f, 0.1f), 0.0f);
+    const float y0 = fmaxr(tone_dec(lv), 1e-6f);
+    const float r = tone_dec(e2) / y0;
+    rgb[3 * i] = rgb[3 * i] * r;
+    rgb[3 * i + 1] = rgb[3 * i + 1] * r;
+    rgb[3 * i + 2] = rgb[3 * i + 2] * r;
+}
+
+size_t export_detail_ws_floats(size_t w, size_t h) { return 4 * w * h; }
+
+hipError_t launch_super_res_refine(float* rgb, size_t w, size_t h, float* ws, hipStream_t st) {
+    const size_t n = w * h;
+    if (n == 0) return hipSuccess;
+    float* y = ws;
+    float* b = ws + n;
+    hipLaunchKernelGGL(k_superres_y, dim3(nblk(n)), dim3(256), 0, st, rgb, n, y);
+    hipError_t e = launch_gaussian_blur(y, ws + 2 * n, b, w, h, 1.2f, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_superres_apply, dim3(nblk(n)), dim3(256), 0, st, rgb, y, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_output_sharpen(float* rgb, size_t w, size_t h, int mode, int amount, float* ws, hipStream_t st) {
+    float radius, base;
+    switch (mode) {
+        case OUT_SHARPEN_SCREEN: radius = 0.6f, base = 0.45f; break;
+        case OUT_SHARPEN_MATTE: radius = 1.0f, base = 0.8f; break;
+        case OUT_SHARPEN_GLOSSY: radius = 0.8f, base = 0.6f; break;
+        default: return hipSuccess;
+    }
+    const float k = base * (amount == OUT_AMOUNT_LOW ? 0.6f : amount == OUT_AMOUNT_HIGH ? 1.5f : 1.0f);
+    const size_t n = w * h;
+    if (n == 0) return hipSuccess;
+    float* l = ws;
+    float* b = ws + n;
+    hipLaunchKernelGGL(k_sharp_enc, dim3(nblk(n)), dim3(256), 0, st, rgb, n, l);
+    hipError_t e = launch_gaussian_blur(l, ws + 2 * n, b, w, h, radius, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_outsharp_apply, dim3(nblk(n)), dim3(256), 0, st, rgb, l, b, n, k);
+    return hipGetLastError();
+}
+
 }  // namespace ds

@@ -150,6 +150,17 @@ int ds_sharpen(ds_ctx* ctx, float* rgb, size_t w, size_t h, const ds_sharpen_par
 int ds_fast_guided_filter(ds_ctx* ctx, const float* I, const float* p, size_t w, size_t h, size_t r, float eps, size_t sub, float* out);
 int ds_plane_resize(ds_ctx* ctx, const float* in, size_t w, size_t h, size_t w2, size_t h2, float* out);
 
+/* The export post-steps after render_full (export.rs:244-247), in place on
+ * interleaved ProPhoto.  ds_super_res_refine replaces
+ * enhance::super_res_refine(img) (enhance.rs:122); ds_output_sharpen replaces
+ * output_sharpen(img, mode, amount) (export.rs:199) with mode a SharpenFor
+ * (export.rs:63) and amount a SharpenAmount (export.rs:71), numbered in
+ * declaration order; other values are DS_ERR_INVALID. */
+enum { DS_SHARPEN_FOR_NONE = 0, DS_SHARPEN_FOR_SCREEN = 1, DS_SHARPEN_FOR_MATTE = 2, DS_SHARPEN_FOR_GLOSSY = 3 };
+enum { DS_SHARPEN_AMOUNT_LOW = 0, DS_SHARPEN_AMOUNT_STANDARD = 1, DS_SHARPEN_AMOUNT_HIGH = 2 };
+int ds_super_res_refine(ds_ctx* ctx, float* rgb, size_t w, size_t h);
+int ds_output_sharpen(ds_ctx* ctx, float* rgb, size_t w, size_t h, int mode, int amount);
+
 /* ---- enhance ----------------------------------------------------------- */
 
 /* enhance.rs:10 estimate_noise(p) */
@@ -215,6 +226,8 @@ int ds_noise_reduction_dev(ds_ctx* ctx, float* rgb, size_t w, size_t h, const ds
 int ds_sharpen_dev(ds_ctx* ctx, float* rgb, size_t w, size_t h, const ds_sharpen_params* sh, const float* sharpness_field, float scale);
 int ds_fast_guided_filter_dev(ds_ctx* ctx, const float* I, const float* p, size_t w, size_t h, size_t r, float eps, size_t sub, float* out);
 int ds_plane_resize_dev(ds_ctx* ctx, const float* in, size_t w, size_t h, size_t w2, size_t h2, float* out);
+int ds_super_res_refine_dev(ds_ctx* ctx, float* rgb, size_t w, size_t h);
+int ds_output_sharpen_dev(ds_ctx* ctx, float* rgb, size_t w, size_t h, int mode, int amount);
 /* levels_out: 3 device floats */
 int ds_cfa_clip_levels_f32_dev(ds_ctx* ctx, const float* cfa, size_t w, size_t h, const uint8_t pattern[4], float* levels_out);
 int ds_cfa_clip_levels_u16_dev(ds_ctx* ctx, const uint16_t* raw, size_t w, size_t h, size_t stride, const uint8_t pattern[4],

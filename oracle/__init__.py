@@ -384,6 +384,25 @@ def sharpen(img, sh, field=None, scale=1.0):
     return img
 
 
+def super_res_refine(img):
+    """enhance.rs:122-131 on a copy of img."""
+    img = np.array(img, copy=True, order="C")
+    h, w, _ = img.shape
+    f, _ct = _fn("or_super_res_refine", img.dtype)
+    f(_p(img), ctypes.c_size_t(w), ctypes.c_size_t(h))
+    return img
+
+
+def output_sharpen(img, mode, amount):
+    """export.rs:199-223 on a copy of img; mode 0..3 = None, Screen, Matte,
+    Glossy; amount 0..2 = Low, Standard, High."""
+    img = np.array(img, copy=True, order="C")
+    h, w, _ = img.shape
+    f, _ct = _fn("or_output_sharpen", img.dtype)
+    f(_p(img), ctypes.c_size_t(w), ctypes.c_size_t(h), ctypes.c_int(mode), ctypes.c_int(amount))
+    return img
+
+
 def tone_enc(v, dtype=np.float32):
     f, ct = _fn("or_tone_enc", dtype, None)
     f.restype = ct

@@ -1065,6 +1065,46 @@ template <class T> void sharpen(T* img, size_t w, size_t h, const T* sh, const T
     }
 }
 
+// enhance.rs:122-131 super_res_refine in place: square-root luminance, its
+// gaussian (sigma 1.2), a clamped detail boost, applied as a clamped ratio.
+template <class T> void super_res_refine(T* img, size_t w, size_t h) {
+    const size_t n = w * h;
+    std::vector<T> y(n), b(n);
+    for (size_t i = 0; i < n; i++) y[i] = t_sqrt(vmax(luma_prophoto(img + 3 * i), (T)0));
+    gaussian_blur(y.data(), w, h, F(1.2), b.data());
+    for (size_t i = 0; i < n; i++) {
+        const T d = vclamp(y[i] - b[i], F(-0.05), F(0.05));
+        const T nv = vmax(y[i] + d * F(0.9), (T)0);
+        const T r = (nv * nv) / vmax(y[i] * y[i], F(1e-8));
+        const T rc = vclamp(r, F(0.5), F(2.0));
+        for (int c = 0; c < 3; c++) img[3 * i + c] = img[3 * i + c] * rc;
+    }
+}
+
+// export.rs:199-223 output_sharpen in place.  mode: 0 None, 1 Screen,
+// 2 Matte, 3 Glossy (SharpenFor); amt: 0 Low, 1 Standard, 2 High.
+template <class T> void output_sharpen(T* img, size_t w, size_t h, int mode, int amt) {
+    T radius, base;
+    switch (mode) {
+        case 1: radius = F(0.6), base = F(0.45); break;
+        case 2: radius = F(1.0), base = F(0.8); break;
+        case 3: radius = F(0.8), base = F(0.6); break;
+        default: return;
+    }
+    const T k = base * (amt == 0 ? F(0.6) : amt == 2 ? F(1.5) : F(1.0));
+    const size_t n = w * h;
+    std::vector<T> l(n), b(n);
+    for (size_t i = 0; i < n; i++) l[i] = tone_enc(vmax(luma_prophoto(img + 3 * i), (T)0));
+    gaussian_blur(l.data(), w, h, radius, b.data());
+    for (size_t i = 0; i < n; i++) {
+        const T d = l[i] - b[i];
+        const T e2 = vmax(l[i] + k * vclamp(d, F(-0.1), F(0.1)), (T)0);
+        const T y0 = vmax(tone_dec(l[i]), F(1e-6));
+        const T r = tone_dec(e2) / y0;
+        for (int c = 0; c < 3; c++) img[3 * i + c] = img[3 * i + c] * r;
+    }
+}
+
 // C ABI
 
 #define OR_BOTH(X) X(float, f32) X(double, f64)
@@ -1092,6 +1132,8 @@ extern "C" {
         noise_reduction<T>(img, w, h, nr, field, scale);                                                                           \
     }                                                                                                                              \
     void or_sharpen_##S(T* img, size_t w, size_t h, const T* sh, const T* field, T scale) { sharpen<T>(img, w, h, sh, field, scale); } \
+    void or_super_res_refine_##S(T* img, size_t w, size_t h) { super_res_refine<T>(img, w, h); }                                   \
+    void or_output_sharpen_##S(T* img, size_t w, size_t h, int mode, int amt) { output_sharpen<T>(img, w, h, mode, amt); }         \
     T or_tone_enc_##S(T v) { return tone_enc<T>(v); }                                                                              \
     T or_tone_dec_##S(T v) { return tone_dec<T>(v); }                                                                              \
     void or_mhc_bayer_##S(const T* cfa, size_t w, size_t h, const uint8_t* pat, const T* mul, T* rgb) {                            \

@@ -132,3 +132,41 @@ def test_detail_rejects_bad_input(ctx):
         ctx._check(D.lib().ds_noise_reduction(ctx.handle, None, 4, 4, None, None, ctypes.c_float(1.0)))
     with pytest.raises(D.DarkslideError):
         ctx.plane_resize(np.zeros((0, 4), np.float32), 2, 2)
+
+
+@pytest.mark.parametrize("h,w", [(96, 128), (37, 211), (300, 200), (1, 9)])
+def test_super_res_refine_exact(ctx, oracle_mod, h, w):
+    """sqrt, the exact gaussian (sigma 1.2), IEEE division and clamps: bit for bit."""
+    img = scene(h, w, seed=h + 1)
+    img[h // 2, w // 3] = 0.0025  # a pixel whose ratio clamps at 2
+    got = ctx.super_res_refine(img)
+    want = oracle_mod.super_res_refine(img)
+    assert not np.array_equal(want, img)
+    assert np.array_equal(got, want), np.abs(got - want).max()
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("amt", [0, 1, 2])
+def test_output_sharpen(ctx, oracle_mod, mode, amt):
+    """The tone curves use powf (device vs glibc: an ulp or two): 2e-5."""
+    O = oracle_mod
+    img = scene(120, 90, seed=mode * 3 + amt)
+    img[:, 45:] *= 2.5  # values above 1 take the curves' linear extension
+    got = ctx.output_sharpen(img, mode, amt)
+    want = O.output_sharpen(img, mode, amt)
+    f64 = O.output_sharpen(img.astype(np.float64), mode, amt)
+    err, e64, r64 = [float(np.abs(a - b).max()) for a, b in ((got, want), (got, f64), (want, f64))]
+    say(f"output_sharpen mode {mode} amount {amt}: vs fp32 {err:.3g}, vs fp64 {e64:.3g} (fp32 oracle vs fp64 {r64:.3g})")
+    assert not np.array_equal(want, img)
+    assert err <= TOL and e64 <= max(TOL, 1.5 * r64)
+
+
+def test_export_post_steps_identities_and_errors(ctx):
+    import darkslide_amd as D
+    img = scene(32, 48, seed=9)
+    assert np.array_equal(ctx.output_sharpen(img, D.SHARPEN_FOR_NONE, D.SHARPEN_AMOUNT_HIGH), img)
+    with pytest.raises(D.DarkslideError):
+        ctx.output_sharpen(img, 4, 1)
+    with pytest.raises(D.DarkslideError):
+        ctx.output_sharpen(img, 1, 3)
+    assert ctx.super_res_refine(np.zeros((0, 5, 3), np.float32)).shape == (0, 5, 3)
