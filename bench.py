@@ -14,6 +14,7 @@ roofline.  The CPU baseline is the C++ restatement of the reference path
 Prints one JSON line on stdout.
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -38,6 +39,16 @@ MHC_BYTES_PER_PX = 14
 # three planes 3 x 2 x 3 x 8 = 144, apply 40
 HIGHLIGHTS_BYTES_PER_PX = 236
 HL_MUL = (2.0, 1.0, 1.5)
+# Algorithmic HBM bytes per pixel of the detail and export stages: each
+# elementwise pass reads and writes its planes once, each separable gaussian
+# pass reads and writes one plane.  Sharpen (masking 0), super_res_refine and
+# output_sharpen: luminance 16, gaussian 2 x 8, ratio 32 (RGB in and out, two
+# planes in).  Noise reduction at the reference's defaults (colour 25, so a
+# fast guided filter at sub 4 per chroma plane): Oklab split 24, per chroma
+# plane 8 to resize L and the plane down and 8 for the coefficients' apply,
+# the two mixes 24, the recomposition 24; the guided filters on the 1/16-size
+# planes are left out.
+DETAIL_BYTES_PER_PX = {"sharpen": 64, "super_res_refine": 64, "output_sharpen": 64, "noise_reduction": 104}
 
 
 def saturate(raw, seed, discs=12):
@@ -304,6 +315,50 @@ def highlights_leg(ctx, args, raw, rgb, clip, w, h):
     return res
 
 
+def detail_leg(ctx, rgb, w, h):
+    """The render detail stages (pipeline/detail.rs) and the export post-steps
+    (export.rs:244-247), in place on the bench's resident 24 MP image, timed
+    by the library's per-call events: five calls each after one warm-up."""
+    import darkslide_amd as D
+    L = D.lib()
+    cases = [
+        ("noise_reduction", "defaults (colour 25)", lambda: L.ds_noise_reduction_dev(
+            ctx.handle, rgb.ptr, w, h, ctypes.byref(D.NrParams(0, 50, 0, 25, 50, 50)), None, 1.0)),
+        ("noise_reduction", "luminance 50, colour 25", lambda: L.ds_noise_reduction_dev(
+            ctx.handle, rgb.ptr, w, h, ctypes.byref(D.NrParams(50, 50, 0, 25, 50, 50)), None, 1.0)),
+        ("sharpen", "defaults (40, 1.0, 25, 0)", lambda: L.ds_sharpen_dev(
+            ctx.handle, rgb.ptr, w, h, ctypes.byref(D.SharpenParams(40, 1.0, 25, 0)), None, 1.0)),
+        ("sharpen", "masking 40", lambda: L.ds_sharpen_dev(
+            ctx.handle, rgb.ptr, w, h, ctypes.byref(D.SharpenParams(40, 1.0, 25, 40)), None, 1.0)),
+        ("super_res_refine", "", lambda: L.ds_super_res_refine_dev(ctx.handle, rgb.ptr, w, h)),
+        ("output_sharpen", "Screen, Standard", lambda: L.ds_output_sharpen_dev(
+            ctx.handle, rgb.ptr, w, h, D.SHARPEN_FOR_SCREEN, D.SHARPEN_AMOUNT_STANDARD)),
+    ]
+    n = w * h
+    res = []
+    for stage, label, call in cases:
+        ctx._check(call())
+        ctx.sync()
+        ctx.prof_enable(True)
+        ctx.prof_reset()
+        for _ in range(5):
+            ctx._check(call())
+        ctx.sync()
+        pr = ctx.prof()
+        ctx.prof_enable(False)
+        t, k = pr[stage]
+        ms = t / max(k, 1)
+        row = {"stage": stage, "settings": label, "ms": round(ms, 4), "mp_s": round(n / 1e6 / (ms * 1e-3), 1)}
+        if label.startswith("defaults") or stage in ("super_res_refine", "output_sharpen"):
+            bpp = DETAIL_BYTES_PER_PX[stage]
+            row["algorithmic_bytes_per_px"] = bpp
+            row["hbm_gbs"] = round(bpp * n / (ms * 1e-3) / 1e9, 1)
+            row["hbm_frac"] = round(row["hbm_gbs"] / PEAK_HBM_GBS, 4)
+        res.append(row)
+    log(f"detail: {res}")
+    return res
+
+
 def stage_errors(ctx, args):
     """Max-abs and RMS error of every stage against the FP32 restatement of
     the reference and against its FP64 twin, on a size x size crop of the
@@ -355,6 +410,12 @@ def stage_errors(ctx, args):
         O.reconstruct_highlights(srgb.astype(np.float64), sclip.astype(np.float64), HL_MUL, lv.astype(np.float64)))
     out["reconstruct_highlights"]["note"] = ("saturated discs; near-clip is box_blur(clip, 3) > 0, whose FP32 running sums "
                                              "leave residues that FP64 does not, so FP64 classifies some pixels differently")
+    r64d = rl.astype(np.float64)
+    for nr in [(0, 50, 0, 25, 50, 50), (50, 50, 0, 25, 50, 50)]:
+        err(f"noise_reduction{nr}", ctx.noise_reduction(rl, nr), O.noise_reduction(rl, nr), O.noise_reduction(r64d, nr))
+    err("sharpen(40,1,25,0)", ctx.sharpen(rl, (40, 1.0, 25, 0)), O.sharpen(rl, (40, 1.0, 25, 0)), O.sharpen(r64d, (40, 1.0, 25, 0)))
+    err("super_res_refine", ctx.super_res_refine(rl), O.super_res_refine(rl), O.super_res_refine(r64d))
+    err("output_sharpen(Screen,Standard)", ctx.output_sharpen(rl, 1, 1), O.output_sharpen(rl, 1, 1), O.output_sharpen(r64d, 1, 1))
     if os.path.exists(args.checkpoint):
         net = D.CNN(ctx, seed=1)
         net.load(args.checkpoint)
@@ -482,6 +543,7 @@ def main():
     if not args.no_learned:
         learned = learned_leg(ctx, args, raw, rgb, den, out, w, h, ms_per_step)
     highlights = highlights_leg(ctx, args, raw, rgb, clip, w, h)
+    detail = detail_leg(ctx, rgb, w, h)
     batched = None
     if args.batch_frames > 0:
         batched = batched_leg(ctx, args)
@@ -520,6 +582,7 @@ def main():
         "learned": learned,
         "batched": batched,
         "highlights": highlights,
+        "detail": detail,
         "stage_errors": errors,
     }
     print(json.dumps(line), flush=True)

@@ -14,8 +14,8 @@
 // dependent add per element, and the next chunk's loads are in flight while
 // the current chunk is walked.  Plain planes go through ring-buffered
 // kernels that read every input once (16-byte loads for the horizontal
-// pass); inputs computed on the fly (the guided-filter products) and
-// unaligned rows use k_boxx_h, which forms the window differences from two
+// pass); rows whose width is not a multiple of 4 (or planes not 16-byte
+// aligned) use k_boxx_h, which forms the window differences from two
 // coalesced loads per element.  Rows/columns x planes give the parallelism:
 // ~(h/64) * planes waves for the horizontal pass, ~(w/64) * planes for the
 // vertical one.
@@ -294,7 +294,15 @@ __global__ __launch_bounds__(MW_THREADS) void k_boxh_ring(CBoxPlanes in, BoxPlan
 // step s - D or earlier, and rows landing during step s never evict a row
 // the walk still needs when RING > 32 D + 2 r + 32.
 constexpr int VS = 32;
-template <int RING, int D>
+// A 16-byte direct global-to-LDS load, outside the kernel template: in the
+// kernel body the host pass drops the kernel's launch stub without a
+// diagnostic when the size is 16 (a gfx950-only operand).
+__device__ __forceinline__ void lds_load16(const float* g, float* lds) { __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0); }
+// V16: rows are read as 16-byte units (w % 4 == 0, planes 16-byte aligned),
+// one load covering 4 rows x 64 columns, lane l taking row l / 16 and
+// columns 4 (l % 16) .. + 3, so the units land in the ring's row-major slots
+// as they are; the prefetch start is rounded up to a whole 4-row group.
+template <int RING, int D, bool V16>
 __global__ __launch_bounds__(MW_THREADS) void k_boxv_ring(CBoxPlanes in, BoxPlanes out, int w, int h, int r, float norm) {
     if (out.gate && *out.gate == 0) return;
     __shared__ float ring[RING * BX];
@@ -308,8 +316,18 @@ __global__ __launch_bounds__(MW_THREADS) void k_boxv_ring(CBoxPlanes in, BoxPlan
     const uint32_t ux = (uint32_t)x;
     auto slot = [&](int y) -> float& { return ring[(y & (RING - 1)) * BX + lane]; };
     auto row_in = [&](int y) { return src + (size_t)y * (size_t)w; };  // uniform row base
-    const int A0 = VS * D + r + 1;
-    for (int y = wave; y < min(A0, h); y += MOVERS + 1) __builtin_amdgcn_global_load_lds(row_in(y) + ux, ring + (y & (RING - 1)) * BX, 4, 0, 0);
+    const int A0 = V16 ? (VS * D + r + 1 + 3) & ~3 : VS * D + r + 1;
+    // 16-byte loader lane: row within its 4-row group, first column of its unit
+    const int qrow = lane >> 4;
+    const uint32_t qcol = (uint32_t)min((int)blockIdx.x * BX + 4 * (lane & 15), w - 4);
+    auto load_quad = [&](int c) {  // rows c .. c + 3 (c % 4 == 0), clamped per lane
+        lds_load16(row_in(min(c + qrow, h - 1)) + qcol, ring + (c & (RING - 1)) * BX);
+    };
+    if (V16) {
+        for (int g = wave; 4 * g < A0; g += MOVERS + 1) load_quad(4 * g);
+    } else {
+        for (int y = wave; y < min(A0, h); y += MOVERS + 1) __builtin_amdgcn_global_load_lds(row_in(y) + ux, ring + (y & (RING - 1)) * BX, 4, 0, 0);
+    }
     __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     __syncthreads();
     float acc = 0.0f;
@@ -318,6 +336,17 @@ __global__ __launch_bounds__(MW_THREADS) void k_boxv_ring(CBoxPlanes in, BoxPlan
     auto store_step = [&](int s) {
         const float* ot = otile + (s & 1) * VS * BX;
         const int Y0 = VS * s;
+        if (V16) {
+            // lane l: rows l / 16 + 4 i, columns 4 (l % 16) .. + 3 as one unit
+            float4 u[VS / 4];
+#pragma unroll
+            for (int i = 0; i < VS / 4; i++) u[i] = *reinterpret_cast<const float4*>(ot + (qrow + 4 * i) * BX + 4 * (lane & 15));
+            if ((int)blockIdx.x * BX + 4 * (lane & 15) >= w) return;
+#pragma unroll
+            for (int i = 0; i < VS / 4; i++)
+                if (Y0 + qrow + 4 * i < h) *reinterpret_cast<float4*>(dst + (size_t)(Y0 + qrow + 4 * i) * (size_t)w + qcol) = u[i];
+            return;
+        }
         float v[VS];
 #pragma unroll
         for (int j = 0; j < VS; j++) v[j] = ot[j * BX + lane];
@@ -355,12 +384,18 @@ __global__ __launch_bounds__(MW_THREADS) void k_boxv_ring(CBoxPlanes in, BoxPlan
             __builtin_amdgcn_s_waitcnt(WAITCNT_LGKM0);
         } else if (wave <= 2) {
             const int base = A0 + VS * s + (VS / 2) * (wave - 1);
+            if (V16) {
 #pragma unroll
-            for (int i = 0; i < VS / 2; i++) {
-                const int c = base + i;
-                __builtin_amdgcn_global_load_lds(row_in(min(c, h - 1)) + ux, ring + (c & (RING - 1)) * BX, 4, 0, 0);
+                for (int i = 0; i < VS / 8; i++) load_quad(base + 4 * i);
+                __builtin_amdgcn_s_waitcnt(waitcnt_vm((VS / 8) * (D - 1)));
+            } else {
+#pragma unroll
+                for (int i = 0; i < VS / 2; i++) {
+                    const int c = base + i;
+                    __builtin_amdgcn_global_load_lds(row_in(min(c, h - 1)) + ux, ring + (c & (RING - 1)) * BX, 4, 0, 0);
+                }
+                __builtin_amdgcn_s_waitcnt(waitcnt_vm((VS / 2) * (D - 1)));
             }
-            __builtin_amdgcn_s_waitcnt(waitcnt_vm((VS / 2) * (D - 1)));
         } else if (s > 0) {
             store_step(s - 1);
         }
@@ -425,46 +460,42 @@ hipError_t box_h_planes(CBoxPlanes in, BoxPlanes out, int planes, size_t w, size
     dim3 grid((unsigned)((h + BX - 1) / BX), (unsigned)planes);
     const int ri = (int)r;
 #define DS_BOXH(RC, RM) hipLaunchKernelGGL((k_boxh_ring<RC, 3, RM>), grid, dim3(MW_THREADS), 0, st, in, out, (int)w, (int)h, ri, norm)
-    if (r <= 56) {
-        switch (ri & 3) {
-            case 0: DS_BOXH(256, 0); break;
-            case 1: DS_BOXH(256, 1); break;
-            case 2: DS_BOXH(256, 2); break;
-            default: DS_BOXH(256, 3); break;
-        }
-    } else {
-        switch (ri & 3) {
-            case 0: DS_BOXH(512, 0); break;
-            case 1: DS_BOXH(512, 1); break;
-            case 2: DS_BOXH(512, 2); break;
-            default: DS_BOXH(512, 3); break;
-        }
+#define DS_BOXH_RM(RC)              \
+    switch (ri & 3) {               \
+        case 0: DS_BOXH(RC, 0); break; \
+        case 1: DS_BOXH(RC, 1); break; \
+        case 2: DS_BOXH(RC, 2); break; \
+        default: DS_BOXH(RC, 3); break; \
     }
+    if (r <= 56) {
+        DS_BOXH_RM(256)
+    } else {
+        DS_BOXH_RM(512)
+    }
+#undef DS_BOXH_RM
 #undef DS_BOXH
     return hipGetLastError();
 }
 
-hipError_t box_h_guided(BoxInGuided in, BoxPlanes out, size_t w, size_t h, size_t r, hipStream_t st) {
-    return run_h(in, out, 6, w, h, r, st);
-}
-
-hipError_t box_h_guide1(BoxInGuide1 in, BoxPlanes out, size_t w, size_t h, size_t r, hipStream_t st) {
-    return run_h(in, out, 4, w, h, r, st);
-}
-
 hipError_t box_v_planes(CBoxPlanes in, BoxPlanes out, int planes, size_t w, size_t h, size_t r, hipStream_t st) {
     const float norm = 1.0f / (float)(2 * r + 1);
-    if (r <= 191) {
-        // ring sizes per the bound RING > 32 D + 2 r + 32 with D = 3
-        dim3 grid((unsigned)((w + BX - 1) / BX), (unsigned)planes);
-        if (r <= 63)
-            hipLaunchKernelGGL((k_boxv_ring<256, 3>), grid, dim3(MW_THREADS), 0, st, in, out, (int)w, (int)h, (int)r, norm);
-        else
-            hipLaunchKernelGGL((k_boxv_ring<512, 3>), grid, dim3(MW_THREADS), 0, st, in, out, (int)w, (int)h, (int)r, norm);
-    } else {
-        dim3 grid((unsigned)((w + 63) / 64), (unsigned)planes);
+    const dim3 grid((unsigned)((w + BX - 1) / BX), (unsigned)planes);
+    bool v16 = w % 4 == 0;
+    for (int k = 0; k < planes && v16; k++) v16 = aligned16(in.p[k]) && aligned16(out.p[k]);
+    // ring sizes from the schedule's bound RING > 32 D + 2 r + 32 with D = 3
+    // (+ 3 when the prefetch start is rounded to a 4-row group)
+#define DS_BOXV(RING, V) hipLaunchKernelGGL((k_boxv_ring<RING, 3, V>), grid, dim3(MW_THREADS), 0, st, in, out, (int)w, (int)h, (int)r, norm)
+    if (v16 && r <= 60)
+        DS_BOXV(256, true);
+    else if (v16 && r <= 188)
+        DS_BOXV(512, true);
+    else if (r <= 63)
+        DS_BOXV(256, false);
+    else if (r <= 191)
+        DS_BOXV(512, false);
+    else
         hipLaunchKernelGGL(k_boxx_v_any, grid, dim3(64), 0, st, in, out, (int)w, (int)h, (int)r, norm);
-    }
+#undef DS_BOXV
     return hipGetLastError();
 }
 

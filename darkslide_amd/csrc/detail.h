@@ -55,5 +55,13 @@ hipError_t launch_plane_resize(const float* in, size_t w, size_t h, size_t w2, s
 size_t fgf_ws_floats(size_t w, size_t h, size_t sub);
 hipError_t launch_fast_guided_filter(const float* I, const float* P, size_t w, size_t h, size_t r, float eps, size_t sub, float* ws,
                                      float* out, hipStream_t st);
+// The chroma step of noise_reduction (detail.rs:79-82): A and B, in place,
+// each mixed by k towards fast_guided_filter(I, plane, r, eps, sub).  The
+// two filters share the guide, so its resized copy and the means of it and
+// of its square (bit-identical in the two calls) are formed once.
+// ws: fgf_pair_ws_floats(w, h, sub).
+size_t fgf_pair_ws_floats(size_t w, size_t h, size_t sub);
+hipError_t launch_fgf_pair_mix(const float* I, float* A, float* B, size_t w, size_t h, size_t r, float eps, size_t sub, float k, float* ws,
+                               hipStream_t st);
 
 }  // namespace ds

@@ -70,6 +70,93 @@ __global__ void k_fgf_final(const float* __restrict__ I, const float* __restrict
     out[i] = a * I[i] + b;
 }
 
+// The bilinear taps of one output pixel (as in bilinear above), shared by
+// several coefficient planes of the same size.
+struct BilinearTap {
+    size_t i00, i10, i01, i11;
+    float ax, ay;
+    __device__ __forceinline__ BilinearTap(int w, int h, int x, int y, float sx, float sy) {
+        const float fx = ((float)x + 0.5f) * sx - 0.5f, fy = ((float)y + 0.5f) * sy - 0.5f;
+        const float fx0 = floorf(fx), fy0 = floorf(fy);
+        ax = fx - fx0;
+        ay = fy - fy0;
+        const int ix = (int)fx0, iy = (int)fy0;
+        const size_t r0 = (size_t)clampi(iy, 0, h - 1) * w, r1 = (size_t)clampi(iy + 1, 0, h - 1) * w;
+        const int u0 = clampi(ix, 0, w - 1), u1 = clampi(ix + 1, 0, w - 1);
+        i00 = r0 + u0;
+        i10 = r0 + u1;
+        i01 = r1 + u0;
+        i11 = r1 + u1;
+    }
+    __device__ __forceinline__ float operator()(const float* __restrict__ p) const {
+        const float a = p[i00] * (1.0f - ax) + p[i10] * ax;
+        const float b = p[i01] * (1.0f - ax) + p[i11] * ax;
+        return a + (b - a) * ay;
+    }
+};
+
+// image.rs:395-401 for two filter inputs that share the guide: pa, pb ->
+// (a1, b1), (a2, b2) from the means of [i, pa, pb, i pa, i pb, i i].
+__global__ void k_fgf_pair_products(const float* __restrict__ is, const float* __restrict__ pa, const float* __restrict__ pb, size_t n,
+                                    float* __restrict__ ipa, float* __restrict__ ipb, float* __restrict__ ii) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float v = is[i];
+    ipa[i] = v * pa[i];
+    ipb[i] = v * pb[i];
+    ii[i] = v * v;
+}
+
+__global__ void k_fgf_pair_coef(const float* __restrict__ mi_, const float* __restrict__ mpa_, const float* __restrict__ mpb_,
+                                const float* __restrict__ mipa_, const float* __restrict__ mipb_, const float* __restrict__ mii_, size_t n,
+                                float eps, float* __restrict__ a1, float* __restrict__ b1, float* __restrict__ a2, float* __restrict__ b2) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float mi = mi_[i];
+    const float var = mii_[i] - mi * mi;
+    {
+        const float mp = mpa_[i];
+        const float av = (mipa_[i] - mi * mp) / (var + eps);
+        a1[i] = av;
+        b1[i] = mp - av * mi;
+    }
+    {
+        const float mp = mpb_[i];
+        const float av = (mipb_[i] - mi * mp) / (var + eps);
+        a2[i] = av;
+        b2[i] = mp - av * mi;
+    }
+}
+
+// image.rs:428-430 (or :404 without subsampling) for both planes, fused
+// with detail.rs:80-81: o + (a I + b - o) k, in place.
+template <bool FAST>
+__global__ void k_fgf_pair_final_mix(const float* __restrict__ I, const float* __restrict__ ma1, const float* __restrict__ mb1,
+                                     const float* __restrict__ ma2, const float* __restrict__ mb2, int sw, int sh, int w, int h, float sx,
+                                     float sy, float k, float* __restrict__ A, float* __restrict__ B) {
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const size_t i = (size_t)y * w + x;
+    float a1, b1, a2, b2;
+    if (FAST) {
+        const BilinearTap t(sw, sh, x, y, sx, sy);
+        a1 = t(ma1);
+        b1 = t(mb1);
+        a2 = t(ma2);
+        b2 = t(mb2);
+    } else {
+        a1 = ma1[i];
+        b1 = mb1[i];
+        a2 = ma2[i];
+        b2 = mb2[i];
+    }
+    const float iv = I[i];
+    const float fa = a1 * iv + b1, fb = a2 * iv + b2;
+    const float oa = A[i], ob = B[i];
+    A[i] = oa + (fa - oa) * k;
+    B[i] = ob + (fb - ob) * k;
+}
+
 // detail.rs:44-55: the filtered luminance blended in by the local strength.
 __global__ void k_nr_luma(float* __restrict__ L, const float* __restrict__ f2, const float* __restrict__ field, size_t n, float lum, float contrast,
                           float max_strength) {
@@ -80,12 +167,6 @@ __global__ void k_nr_luma(float* __restrict__ L, const float* __restrict__ f2, c
     const float l = L[i], f = f2[i];
     const float filt = f + (l - f) * contrast * 0.35f;
     L[i] = l + (filt - l) * fminf(s / fmaxr(max_strength, 1e-6f), 1.0f);
-}
-
-// detail.rs:80-81
-__global__ void k_nr_mix(float* __restrict__ a, const float* __restrict__ f, size_t n, float k) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) a[i] = a[i] + (f[i] - a[i]) * k;
 }
 
 // detail.rs:85-89: Oklab back to ProPhoto (color.rs:415-418), clamped at 0.
@@ -220,9 +301,70 @@ hipError_t launch_fast_guided_filter(const float* I, const float* P, size_t w, s
     return hipGetLastError();
 }
 
+size_t fgf_pair_ws_floats(size_t w, size_t h, size_t sub) {
+    const bool fast = !(sub <= 1 || w / sub < 8 || h / sub < 8);
+    return 12 * (fast ? (w / sub) * (h / sub) : w * h);
+}
+
+hipError_t launch_fgf_pair_mix(const float* I, float* A, float* B, size_t w, size_t h, size_t r, float eps, size_t sub, float k, float* ws,
+                               hipStream_t st) {
+    if (w == 0 || h == 0) return hipSuccess;
+    const bool fast = !(sub <= 1 || w / sub < 8 || h / sub < 8);
+    const size_t sw = fast ? w / sub : w, sh = fast ? h / sub : h, ns = sw * sh;
+    const size_t rs = fast ? (r / sub > 1 ? r / sub : 1) : r;
+    float* q[12];
+    for (int j = 0; j < 12; j++) q[j] = ws + (size_t)j * ns;
+    const float *is = I, *pa = A, *pb = B;
+    hipError_t e = hipSuccess;
+    if (fast) {
+        e = launch_plane_resize(I, w, h, sw, sh, q[0], st);
+        if (e == hipSuccess) e = launch_plane_resize(A, w, h, sw, sh, q[1], st);
+        if (e == hipSuccess) e = launch_plane_resize(B, w, h, sw, sh, q[2], st);
+        if (e != hipSuccess) return e;
+        is = q[0];
+        pa = q[1];
+        pb = q[2];
+    }
+    hipLaunchKernelGGL(k_fgf_pair_products, dim3(nblk(ns)), dim3(256), 0, st, is, pa, pb, ns, q[3], q[4], q[5]);
+    CBoxPlanes hin{};
+    BoxPlanes hout{}, vout{};
+    const float* src[6] = {is, pa, pb, q[3], q[4], q[5]};
+    for (int j = 0; j < 6; j++) {
+        hin.p[j] = src[j];
+        hout.p[j] = q[6 + j];
+        vout.p[j] = q[j];
+    }
+    CBoxPlanes vin{};
+    for (int j = 0; j < 6; j++) vin.p[j] = q[6 + j];
+    e = box_h_planes(hin, hout, 6, sw, sh, rs, st);
+    if (e == hipSuccess) e = box_v_planes(vin, vout, 6, sw, sh, rs, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fgf_pair_coef, dim3(nblk(ns)), dim3(256), 0, st, q[0], q[1], q[2], q[3], q[4], q[5], ns, eps, q[6], q[7], q[8],
+                       q[9]);
+    CBoxPlanes cin{}, cmid{};
+    BoxPlanes mid{}, fin{};
+    for (int j = 0; j < 4; j++) {
+        cin.p[j] = q[6 + j];
+        mid.p[j] = q[j];
+        cmid.p[j] = q[j];
+        fin.p[j] = q[6 + j];
+    }
+    e = box_h_planes(cin, mid, 4, sw, sh, rs, st);
+    if (e == hipSuccess) e = box_v_planes(cmid, fin, 4, sw, sh, rs, st);
+    if (e != hipSuccess) return e;
+    const dim3 grid(nblk(w), (unsigned)h);
+    if (fast)
+        hipLaunchKernelGGL(k_fgf_pair_final_mix<true>, grid, dim3(256), 0, st, I, q[6], q[7], q[8], q[9], (int)sw, (int)sh, (int)w, (int)h,
+                           (float)sw / (float)w, (float)sh / (float)h, k, A, B);
+    else
+        hipLaunchKernelGGL(k_fgf_pair_final_mix<false>, grid, dim3(256), 0, st, I, q[6], q[7], q[8], q[9], (int)sw, (int)sh, (int)w, (int)h,
+                           1.0f, 1.0f, k, A, B);
+    return hipGetLastError();
+}
+
 size_t nr_ws_floats(size_t w, size_t h) {
     const size_t n = w * h;
-    return 5 * n + 8 * n + 10 * ((w / 2) * (h / 2));  // L A B f1 f2, guided filter, fast guided filter at sub >= 2
+    return 15 * n;  // L A B f1 f2 and the luminance guided filters' 8 planes, or L A B and the chroma pair's 12
 }
 
 hipError_t launch_noise_reduction(float* rgb, size_t w, size_t h, const NrParams& nr, const float* field, float field_max, float scale,
@@ -242,7 +384,6 @@ hipError_t launch_noise_reduction(float* rgb, size_t w, size_t h, const NrParams
     float* f1 = ws + 3 * n;
     float* f2 = ws + 4 * n;
     float* gws = ws + 5 * n;
-    float* fws = ws + 13 * n;
     hipError_t e = launch_oklab_fwd(rgb, n, ok, L, A, B, st);
     if (e != hipSuccess) return e;
     bool changed = false;
@@ -270,13 +411,8 @@ hipError_t launch_noise_reduction(float* rgb, size_t w, size_t h, const NrParams
         const float q = 0.002f + (1.0f - detail) * 0.02f;
         const float eps = q * q;
         const size_t sub = r >= 8 ? 4 : (r >= 3 ? 2 : 1);
-        float* fw = sub <= 1 || w / sub < 8 || h / sub < 8 ? gws : fws;
-        e = launch_fast_guided_filter(L, A, w, h, r, eps, sub, fw, f1, st);
-        if (e == hipSuccess) e = launch_fast_guided_filter(L, B, w, h, r, eps, sub, fw, f2, st);
+        e = launch_fgf_pair_mix(L, A, B, w, h, r, eps, sub, fminf(col, 1.0f), ws + 3 * n, st);
         if (e != hipSuccess) return e;
-        const float k = fminf(col, 1.0f);
-        hipLaunchKernelGGL(k_nr_mix, dim3(nblk(n)), dim3(256), 0, st, A, f1, n, k);
-        hipLaunchKernelGGL(k_nr_mix, dim3(nblk(n)), dim3(256), 0, st, B, f2, n, k);
         changed = true;
     }
     if (changed) hipLaunchKernelGGL(k_nr_compose, dim3(nblk(n)), dim3(256), 0, st, L, A, B, n, ok, rgb);

@@ -9,6 +9,8 @@
 #include "rl.h"
 #include "stages.h"
 
+#include <initializer_list>
+
 #pragma clang fp contract(off)
 
 namespace ds {
@@ -48,6 +50,16 @@ __global__ void k_gf1_coef(const float* __restrict__ mi_, const float* __restric
     const float av = cov / (var + eps);
     a[i] = av;
     b[i] = mp - av * mi;
+}
+
+// image.rs:388-389: i p (when ip is set) and i i
+__global__ void k_gf1_products(const float* __restrict__ I, const float* __restrict__ P, size_t n, float* __restrict__ ip,
+                               float* __restrict__ ii) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = I[i];
+    if (ip) ip[i] = v * P[i];
+    ii[i] = v * v;
 }
 
 // image.rs:404
@@ -140,28 +152,45 @@ hipError_t launch_guided_coeffs(const float* I, const float* P, size_t w, size_t
     if (n == 0) return hipSuccess;
     float* t[8];
     for (int k = 0; k < 8; k++) t[k] = ws + (size_t)k * n;
-    BoxPlanes h4{}, v4{}, hab{}, vab{};
-    CBoxPlanes c4{}, cab{}, chab{};
-    for (int k = 0; k < 4; k++) {
-        h4.p[k] = t[k];
-        c4.p[k] = t[k];
-        v4.p[k] = t[4 + k];
+    auto planes = [&](std::initializer_list<float*> l) {
+        BoxPlanes o{};
+        int k = 0;
+        for (float* q : l) o.p[k++] = q;
+        return o;
+    };
+    auto cplanes = [&](std::initializer_list<const float*> l) {
+        CBoxPlanes o{};
+        int k = 0;
+        for (const float* q : l) o.p[k++] = q;
+        return o;
+    };
+    // The products are formed once (image.rs:388-389) so every blur is a
+    // pass over plain planes.  A self-guided filter (p == i) has
+    // box(p) == box(i) and box(i p) == box(i i) bit for bit, so those two
+    // planes are blurred once and read twice.
+    const float *mi, *mp, *mip, *mii;
+    hipError_t e;
+    if (I == P) {
+        hipLaunchKernelGGL(k_gf1_products, dim3(nblk(n)), dim3(256), 0, st, I, I, n, (float*)nullptr, t[2]);
+        e = box_h_planes(cplanes({I, t[2]}), planes({t[0], t[1]}), 2, w, h, r, st);
+        if (e == hipSuccess) e = box_v_planes(cplanes({t[0], t[1]}), planes({t[2], t[3]}), 2, w, h, r, st);
+        mi = mp = t[2];
+        mip = mii = t[3];
+    } else {
+        hipLaunchKernelGGL(k_gf1_products, dim3(nblk(n)), dim3(256), 0, st, I, P, n, t[2], t[3]);
+        e = box_h_planes(cplanes({I, P, t[2], t[3]}), planes({t[0], t[1], t[4], t[5]}), 4, w, h, r, st);
+        if (e == hipSuccess) e = box_v_planes(cplanes({t[0], t[1], t[4], t[5]}), planes({t[2], t[3], t[6], t[7]}), 4, w, h, r, st);
+        mi = t[2];
+        mp = t[3];
+        mip = t[6];
+        mii = t[7];
     }
-    hipError_t e = box_h_guide1(BoxInGuide1{I, P}, h4, w, h, r, st);
-    if (e == hipSuccess) e = box_v_planes(c4, v4, 4, w, h, r, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gf1_coef, dim3(nblk(n)), dim3(256), 0, st, t[4], t[5], t[6], t[7], n, eps, t[0], t[1]);
-    cab.p[0] = t[0];
-    cab.p[1] = t[1];
-    hab.p[0] = t[2];
-    hab.p[1] = t[3];
-    chab.p[0] = t[2];
-    chab.p[1] = t[3];
-    vab.p[0] = t[4];
-    vab.p[1] = t[5];
+    hipLaunchKernelGGL(k_gf1_coef, dim3(nblk(n)), dim3(256), 0, st, mi, mp, mip, mii, n, eps, t[0], t[1]);
     e = hipGetLastError();
-    if (e == hipSuccess) e = box_h_planes(cab, hab, 2, w, h, r, st);
-    if (e == hipSuccess) e = box_v_planes(chab, vab, 2, w, h, r, st);
+    // the coefficients' means (image.rs:402-403) -> planes 4 and 5
+    if (e == hipSuccess) e = box_h_planes(cplanes({t[0], t[1]}), planes({t[2], t[3]}), 2, w, h, r, st);
+    if (e == hipSuccess) e = box_v_planes(cplanes({t[2], t[3]}), planes({t[4], t[5]}), 2, w, h, r, st);
     return e;
 }
 

@@ -242,6 +242,21 @@ def test_box_blur_ring_paths(ctx, oracle_mod, w, h, r):
     assert np.array_equal(got, want), f"bit-equal {np.mean(got == want):.6f}"
 
 
+# Every ring variant on both sides of its radius limit: row passes use the
+# 256-column ring to r 56 and the 512-column one to r 184; column passes over
+# widths divisible by 4 move 16-byte units (256-row ring to r 60, 512 to
+# r 188), other widths 4-byte ones (to r 63 / 191).  Tall and wide planes
+# give many blocks per plane.
+@pytest.mark.parametrize("w,h,r", [(336, 150, 3), (336, 150, 56), (336, 150, 57), (336, 150, 60), (336, 150, 61), (336, 150, 184),
+                                   (336, 150, 185), (336, 150, 188), (336, 150, 189), (336, 150, 191), (336, 150, 192),
+                                   (72, 16400, 1), (72, 16400, 57), (16400, 72, 2), (16400, 72, 61), (16401, 72, 30), (333, 150, 62)])
+def test_box_blur_ring_limits(ctx, oracle_mod, w, h, r):
+    rng = np.random.default_rng(w + 7 * r)
+    p = rng.random((h, w)).astype(np.float32)
+    got, want = ctx.box_blur(p, r), oracle_mod.box_blur(p, r)
+    assert np.array_equal(got, want), f"bit-equal {np.mean(got == want):.6f}"
+
+
 @pytest.mark.parametrize("sigma", [0.2, 0.5, 0.75, 1.0, 1.9, 2.5, 7.0])
 def test_gaussian_blur_exact(ctx, oracle_mod, sigma):
     rng = np.random.default_rng(int(sigma * 10))
