@@ -129,6 +129,7 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
         const int side = 2 * S + 1;
         const int noff = side * side;
         int dx = -S, dy = -S;  // offsets in the reference's order: dy outer, dx inner
+        int doff = -S * PW - S;  // dy * PW + dx, advanced with the offset
         for (int o = 0; o < noff; o++) {
             float* hs = hs0 + (o & 1) * (NLM_HSR * HS);
             // ---- phase 1: horizontal window sums of the squared differences
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
                 const int u0 = u00 + 4 * SEG * task;
                 float d[DW];
                 if constexpr (IN) {
-                    const float* sp = pt + (cv + dy + SP) * PW + (u0 - P + dx + SP);
+                    const float* sp = pt + (cv + SP) * PW + (u0 - P + SP) + doff;
 #pragma unroll
                     for (int i = 0; i < DW; i++) {
                         float e = c[i] - sp[i];
@@ -153,6 +154,9 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
                         d[i] = e * e;
                     }
                 }
+                // sliding window sums, as the reference's box blur slides them
+                // (a prefix-sum form saves an operation per sum but moves the
+                // denoise composite outside the reference's own spread)
                 float hsum[SEG];
                 float s = 0.0f;
 #pragma unroll
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
             // ---- phase 2: vertical window sums, weights, accumulation
             {
                 const float* hc = hs + row0 * HS + col;  // hs row r holds tile row r - P
-                const float* qp = pt + (row0 + dy + SP) * PW + (col + dx + SP);
+                const float* qp = pt + (row0 + SP) * PW + (col + SP) + doff;
                 float s = 0.0f;
 #pragma unroll
                 for (int j = 0; j < 2 * P + 1; j++) s += hc[j * HS];
@@ -187,9 +191,11 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
             }
             // the other hs buffer is written next; the barrier after that phase 1
             // orders it against this phase 2
+            ++doff;
             if (++dx > S) {
                 dx = -S;
                 ++dy;
+                doff += PW - side;
             }
         }
     };

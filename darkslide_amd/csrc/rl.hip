@@ -52,6 +52,36 @@ __global__ void k_rl_apply(const float* __restrict__ rgb, const float* __restric
     out[3 * i + 2] = rgb[3 * i + 2] * r;
 }
 
+// Interior staging: the ROWS x COLS block of a plane at g (row stride w)
+// into LDS rows of stride LS.  Rows go to waves, columns to lanes; the row
+// base is uniform per wave (scalar arithmetic) and each lane adds its
+// column, so a load costs one vector add instead of the clamped 2-D index of
+// stage_tile.
+template <int ROWS, int COLS, int LS, int NT>
+__device__ __forceinline__ void stage_rows(const float* __restrict__ g, int w, float* __restrict__ lds) {
+    constexpr int NW = NT / 64, RPW = (ROWS + NW - 1) / NW, CH = (COLS + 63) / 64;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    float r[RPW][CH];
+    const float* rp = g + (size_t)wave * (size_t)w;  // row wave + NW i, advanced by NW rows
+    const size_t step = (size_t)NW * (size_t)w;
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            const int row = wave + NW * i, col = lane + 64 * c;
+            if (row < ROWS && col < COLS) r[i][c] = rp[col];
+        }
+        rp += step;
+    }
+#pragma unroll
+    for (int i = 0; i < RPW; i++)
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            const int row = wave + NW * i, col = lane + 64 * c;
+            if (row < ROWS && col < COLS) lds[row * LS + col] = r[i][c];
+        }
+}
+
 template <int R, int TY>
 __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est, const float* __restrict__ yp, float* __restrict__ est_out,
                                                    int w, int h, Taps taps) {
@@ -66,10 +96,15 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
     // horizontal passes) or along a row (the vertical passes) hit distinct banks
     constexpr int ES = EW | 1, YS = YW | 1, MS = MW | 1, QS = QW | 1;
     constexpr int TMP = (MH * MS > QH * QS) ? MH * MS : QH * QS;
+    static_assert(YH * YS <= EH * ES, "ratio fits in the est area");
+    // Two LDS planes, so that more blocks fit per CU (the iteration is bound
+    // by how many tiles are in flight, not by arithmetic): y is read from
+    // global memory where the ratio needs it, and the ratio overwrites est
+    // once the first pass has read it (the est values of the final update are
+    // kept in registers).
     __shared__ float s_est[EH * ES];
-    __shared__ float s_y[YH * YS];
     __shared__ float s_tmp[TMP];
-    __shared__ float s_rat[YH * YS];
+    float* const s_rat = s_est;
     const int tid = threadIdx.x;
     int tx, ty;
     xcd_tile(tx, ty);
@@ -78,21 +113,38 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
 #pragma unroll
     for (int i = 0; i < K; i++) kk[i] = taps.k[i];
 
-    stage_tile<EH * EW, NT>(
-        [&](int i) {
-            int u = i % EW - 2 * R, v = i / EW - 2 * R;
-            return est[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
-        },
-        [&](int i, float v) { s_est[(i / EW) * ES + i % EW] = v; });
-    stage_tile<YH * YW, NT>(
-        [&](int i) {
-            int u = i % YW - R, v = i / YW - R;
-            return yp[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
-        },
-        [&](int i, float v) { s_y[(i / YW) * YS + i % YW] = v; });
+    const bool interior = ox >= 2 * R && oy >= 2 * R && ox + TX + 2 * R <= w && oy + TY + 2 * R <= h;
+    if (interior) {
+        stage_rows<EH, EW, ES, NT>(est + (size_t)(oy - 2 * R) * w + (ox - 2 * R), w, s_est);
+    } else {
+        stage_tile<EH * EW, NT>(
+            [&](int i) {
+                int u = i % EW - 2 * R, v = i / EW - 2 * R;
+                return est[(size_t)clampi(oy + v, 0, h - 1) * w + clampi(ox + u, 0, w - 1)];
+            },
+            [&](int i, float v) { s_est[(i / EW) * ES + i % EW] = v; });
+    }
     __syncthreads();
     auto E = [&](int u, int v) -> float { return s_est[(v + 2 * R) * ES + (u + 2 * R)]; };
-    const bool interior = ox >= 2 * R && oy >= 2 * R && ox + TX + 2 * R <= w && oy + TY + 2 * R <= h;
+    // est at the outputs of this thread's final-update tasks (interior: task
+    // t -> column t % TX, rows 8 (t / TX) ..; edge: pixels tid + k NT)
+    constexpr int NV2 = (TX * (TY / B) + NT - 1) / NT, NE = (TX * TY + NT - 1) / NT;
+    constexpr int NEV = NV2 * B > NE ? NV2 * B : NE;
+    float ev[NEV];
+    if (interior) {
+#pragma unroll
+        for (int q = 0; q < NV2; q++) {
+            const int t = tid + q * NT;
+#pragma unroll
+            for (int j = 0; j < B; j++) ev[q * B + j] = t < TX * (TY / B) ? E(t % TX, (t / TX) * B + j) : 0.0f;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NE; k++) {
+            const int i = tid + k * NT;
+            ev[k] = i < TX * TY ? E(i % TX, i / TX) : 0.0f;
+        }
+    }
 
     if (interior) {
         // Register-blocked passes: each task produces B consecutive outputs
@@ -121,6 +173,14 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         constexpr int V1R = (YH + B - 1) / B;
         for (int t = tid; t < YW * V1R; t += NT) {
             const int u = t % YW - R, v0 = (t / YW) * B - R;
+            // y of the task's rows, loaded before the taps are summed
+            float yv[B];
+            const float* yr = yp + (size_t)(oy + v0) * (size_t)w + (ox + u);
+#pragma unroll
+            for (int j = 0; j < B; j++) {
+                yv[j] = v0 + j < TY + R ? *yr : 0.0f;
+                yr += w;
+            }
             const float* src = s_tmp + (v0 - R + 2 * R) * MS + (u + R);
             float x[B + 2 * R];
 #pragma unroll
@@ -130,10 +190,7 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
                 float acc = 0.0f;
 #pragma unroll
                 for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
-                if (v0 + j < TY + R) {
-                    const int idx = (v0 + j + R) * YS + (u + R);
-                    s_rat[idx] = s_y[idx] / fmaxr(acc, 1e-6f);
-                }
+                if (v0 + j < TY + R) s_rat[(v0 + j + R) * YS + (u + R)] = yv[j] / fmaxr(acc, 1e-6f);
             }
         }
         __syncthreads();
@@ -157,18 +214,23 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         __syncthreads();
         // vertical blur -> corr, est update: rows [0, TY), cols [0, TX)
         constexpr int V2R = TY / B;
-        for (int t = tid; t < QW * V2R; t += NT) {
+#pragma unroll
+        for (int q = 0; q < NV2; q++) {
+            const int t = tid + q * NT;
+            if (t >= QW * V2R) break;
             const int u = t % QW, v0 = (t / QW) * B;
             const float* src = s_tmp + v0 * QS + u;
             float x[B + 2 * R];
 #pragma unroll
             for (int i = 0; i < B + 2 * R; i++) x[i] = src[i * QS];
+            float* o = est_out + (size_t)(oy + v0) * (size_t)w + (ox + u);
 #pragma unroll
             for (int j = 0; j < B; j++) {
                 float acc = 0.0f;
 #pragma unroll
                 for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
-                est_out[(size_t)(oy + v0 + j) * w + (ox + u)] = clampr(E(u, v0 + j) * acc, 0.0f, 64.0f);
+                *o = clampr(ev[q * B + j] * acc, 0.0f, 64.0f);
+                o += w;
             }
         }
         return;
@@ -194,8 +256,8 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         float acc = 0.0f;
 #pragma unroll
         for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(cv + j - R + 2 * R) * MS + (u + R)];
-        const int idx = (v + R) * YS + (u + R);
-        s_rat[idx] = s_y[idx] / fmaxr(acc, 1e-6f);
+        const float yv = yp[(size_t)(oy + cv) * w + (ox + cu_of(u))];  // y at the clamped pixel
+        s_rat[(v + R) * YS + (u + R)] = yv / fmaxr(acc, 1e-6f);
     }
     __syncthreads();
     for (int i = tid; i < QH * QW; i += NT) {
@@ -207,14 +269,16 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
         s_tmp[(v + R) * QS + u] = acc;
     }
     __syncthreads();
-    for (int i = tid; i < TX * TY; i += NT) {
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const int i = tid + k * NT;
         int u = i % TX, v = i / TX;
         int gx = ox + u, gy = oy + v;
-        if (gx >= w || gy >= h) continue;
+        if (i >= TX * TY || gx >= w || gy >= h) continue;
         float acc = 0.0f;
 #pragma unroll
         for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(v + j - R + R) * QS + u];
-        est_out[(size_t)gy * w + gx] = clampr(E(u, v) * acc, 0.0f, 64.0f);
+        est_out[(size_t)gy * w + gx] = clampr(ev[k] * acc, 0.0f, 64.0f);
     }
 }
 
