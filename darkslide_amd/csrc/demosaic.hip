@@ -107,15 +107,21 @@ constexpr int H_PQ = 4;    // pq
 constexpr int H_RB = 3;    // r, b after step 4.3
 constexpr int H_FLAG = 1;  // clip flags
 
+// LDS of one RCD tile, in three regions reused as planes die (stage numbers
+// as in rcd_body):
+//   cm, flag       stages 0-6
+//   b: bv | bh     stages 1-2, then g | vd | pq (g, vd 3-6; pq 4-5)
+//   c: lpf | vh    stages 1-3, then bp | bq (3b-4), then r43 | b43 (5-6)
+// vd holds disc(vh) at the tile's G sites, so vh can die after stage 3.
+// Two 64 x 32 tiles fit in one CU's LDS.
 template <int TX, int TY>
 struct RcdLds {
+    static constexpr int mx(int a, int b) { return a > b ? a : b; }
+    static constexpr int NB = mx(2 * Tile<TX, TY, H_BVH>::N, Tile<TX, TY, H_G>::N + TX * TY + Tile<TX, TY, H_PQ>::N);
+    static constexpr int NC = mx(Tile<TX, TY, H_LPF>::N + Tile<TX, TY, H_VH>::N, mx(2 * Tile<TX, TY, H_G>::N, 2 * Tile<TX, TY, H_RB>::N));
     float cm[Tile<TX, TY, H_CM>::N];
-    // bv/bh, then bp/bq, then r43/b43 share this space
-    float scratch_a[2 * Tile<TX, TY, H_BVH>::N];
-    float vh[Tile<TX, TY, H_VH>::N];
-    float lpf[Tile<TX, TY, H_LPF>::N];
-    float g[Tile<TX, TY, H_G>::N];
-    float pq[Tile<TX, TY, H_PQ>::N];
+    float b[NB];
+    float c[NC];
     uint8_t flag[Tile<TX, TY, H_FLAG>::N];
 };
 
@@ -132,13 +138,14 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
     const int rbp = pat.rb_parity();
 
     Tile<TX, TY, H_CM> cm{L.cm};
-    Tile<TX, TY, H_BVH> bv{L.scratch_a}, bh{L.scratch_a + Tile<TX, TY, H_BVH>::N};
-    Tile<TX, TY, H_G> bp{L.scratch_a}, bq{L.scratch_a + Tile<TX, TY, H_G>::N};
-    Tile<TX, TY, H_RB> r43{L.scratch_a}, b43{L.scratch_a + Tile<TX, TY, H_RB>::N};
-    Tile<TX, TY, H_VH> vh{L.vh};
-    Tile<TX, TY, H_LPF> lpf{L.lpf};
-    Tile<TX, TY, H_G> gg{L.g};
-    Tile<TX, TY, H_PQ> pq{L.pq};
+    Tile<TX, TY, H_BVH> bv{L.b}, bh{L.b + Tile<TX, TY, H_BVH>::N};
+    Tile<TX, TY, H_G> gg{L.b};
+    float* const vdt = L.b + Tile<TX, TY, H_G>::N;  // TX x TY, G sites only
+    Tile<TX, TY, H_PQ> pq{L.b + Tile<TX, TY, H_G>::N + TX * TY};
+    Tile<TX, TY, H_LPF> lpf{L.c};
+    Tile<TX, TY, H_VH> vh{L.c + Tile<TX, TY, H_LPF>::N};
+    Tile<TX, TY, H_G> bp{L.c}, bq{L.c + Tile<TX, TY, H_G>::N};
+    Tile<TX, TY, H_RB> r43{L.c}, b43{L.c + Tile<TX, TY, H_RB>::N};
 
     auto in_img = [&](int gx, int gy) { return INTERIOR || (gx >= 0 && gx < w && gy >= 0 && gy < h); };
     auto in_box = [&](int gx, int gy, int b) { return INTERIOR || (gx >= b && gx < w - b && gy >= b && gy < h - b); };
@@ -240,7 +247,14 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
         }
         gg(tx, ty) = gv;
     }
-    // bp / bq (demosaic.rs:248-260) into the bv/bh space, free since stage 2
+    // disc(vh) at the tile's G sites, for stage 6
+    for (int i = tid; i < TX * TY / 2; i += NT) {
+        const int ty = i / (TX / 2);
+        const int tx = 2 * (i % (TX / 2)) + ((rbp + 1 + ty) & 1);
+        vdt[ty * TX + tx] = disc(vh, tx, ty);
+    }
+    __syncthreads();
+    // bp / bq (demosaic.rs:248-260) over lpf / vh, free since the barrier above
     for (int i = tid; i < Tile<TX, TY, H_G>::N; i += NT) {
         int tx = i % Tile<TX, TY, H_G>::W - H_G, ty = i / Tile<TX, TY, H_G>::W - H_G;
         int gx = ox + tx, gy = oy + ty;
@@ -326,7 +340,7 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
             float gc = cval(tx, ty);
             o[1] = gc;
             if (in_box(gx, gy, kBorder)) {
-                float vd = disc(vh, tx, ty);
+                float vd = vdt[ty * TX + tx];
                 auto g = [&](int dx, int dy) { return gg(tx + dx, ty + dy); };
 #pragma unroll
                 for (int k = 0; k < 2; k++) {
@@ -372,7 +386,6 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
 template <int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) void k_rcd(CfaSrc src, float* __restrict__ rgb, float* __restrict__ clip) {
     static_assert((TX % 2) == 0 && (TY % 2) == 0, "tile origin must keep Bayer parity");
-    static_assert(2 * Tile<TX, TY, H_RB>::N <= 2 * Tile<TX, TY, H_BVH>::N, "r43/b43 alias");
     __shared__ RcdLds<TX, TY> L;
     const int ox = blockIdx.x * TX, oy = blockIdx.y * TY;
     const bool interior = ox - H_CM >= kBorder && oy - H_CM >= kBorder && ox + TX + H_CM <= src.w - kBorder &&
