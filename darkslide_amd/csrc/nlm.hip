@@ -40,23 +40,27 @@ constexpr int NLM_NT = 256;
 constexpr int NLM_SEG = 16;       // columns per phase-1 task
 constexpr int NLM_HS = NLM_TW + 4;  // padded row stride of the horizontal-sum buffer
 constexpr int NLM_MAX_SP = 24;    // halo limit of the tile kernel
-// Input tile row stride: fixed (so every LDS offset inside the offset loop
-// is an immediate) and odd (lanes walking down one column hit distinct banks).
-constexpr int NLM_PW = NLM_TW + 2 * NLM_MAX_SP + 1;
+constexpr int NLM_NARROW_SP = 13; // halo limit of the narrow instance (search 10, patch 3)
+// Input tile row stride: fixed per instance (so every LDS offset inside the
+// offset loop is an immediate) and odd (lanes walking down one column hit
+// distinct banks).  The narrow instance, with one horizontal-sum buffer and a
+// second barrier per offset, fits three tiles per CU instead of two.
+constexpr int NLM_PW_WIDE = NLM_TW + 2 * NLM_MAX_SP + 1;
+constexpr int NLM_PW_NARROW = NLM_TW + 2 * NLM_NARROW_SP + 1;
 constexpr int NLM_HSR = NLM_HR + 4;  // sum rows incl. padding read by the last row segment
 
 __host__ __device__ constexpr int nlm_th(int P) { return NLM_HR - 2 * P; }
 // input tile plus two spare rows read (not stored) by the last row segment
-__host__ __device__ inline int nlm_pt_floats(int SP, int P) { return ((nlm_th(P) + 2 * SP + 2) * NLM_PW + 3) & ~3; }
+__host__ __device__ inline int nlm_pt_floats(int SP, int P, int PW) { return ((nlm_th(P) + 2 * SP + 2) * PW + 3) & ~3; }
 
 // Tile kernel.  Phase 1: thread t owns hs row (t % 64) and the column
 // segments t / 64 + 4 k, k < NTASK (TW / (4 SEG) tasks per thread).  The
 // centre samples of its segments do not depend on the offset and stay in
 // registers for the whole offset loop.  Phase 2: thread t owns column t % 64
 // and a quarter of the tile rows.
-template <int P>
-__global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S,
-                                                        NlmWeight wp, const uint8_t* __restrict__ regions) {
+template <int P, int PW, int HSB>
+__global__ __launch_bounds__(NLM_NT, 4 - HSB) void k_nlm_tile(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S,
+                                                              NlmWeight wp, const uint8_t* __restrict__ regions) {
     extern __shared__ float lds[];
     constexpr int TW = NLM_TW, TH = nlm_th(P), NT = NLM_NT, SEG = NLM_SEG, HS = NLM_HS;
     constexpr int DW = SEG + 2 * P;           // differences per phase-1 task
@@ -69,10 +73,9 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
     const float cs = sqrtf(-wp.scale());
     const int SP = S + P;
     const int PWL = TW + 2 * SP;              // loaded tile row length
-    constexpr int PW = NLM_PW;
     const int PH = TH + 2 * SP;
     float* pt = lds;                          // input tile, PH x PW
-    float* hs0 = lds + nlm_pt_floats(SP, P);  // horizontal sums, 2 x HSR x HS
+    float* hs0 = lds + nlm_pt_floats(SP, P, PW);  // horizontal sums, HSB x HSR x HS
     const int tid = threadIdx.x;
     const int ox = blockIdx.x * TW, oy = blockIdx.y * TH;
     // region mode: only tiles that touch a flagged region run, and they store
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
         int dx = -S, dy = -S;  // offsets in the reference's order: dy outer, dx inner
         int doff = -S * PW - S;  // dy * PW + dx, advanced with the offset
         for (int o = 0; o < noff; o++) {
-            float* hs = hs0 + (o & 1) * (NLM_HSR * HS);
+            float* hs = HSB == 2 ? hs0 + (o & 1) * (NLM_HSR * HS) : hs0;
             // ---- phase 1: horizontal window sums of the squared differences
 #pragma unroll
             for (int task = 0; task < NTASK; task++) {
@@ -189,8 +192,10 @@ __global__ __launch_bounds__(NLM_NT, 2) void k_nlm_tile(const float* __restrict_
                     if (k + 1 < RPT) s += hc[(k + 2 * P + 1) * HS] - hc[k * HS];
                 }
             }
-            // the other hs buffer is written next; the barrier after that phase 1
-            // orders it against this phase 2
+            // with two buffers the other one is written next, and the barrier
+            // after that phase 1 orders it against this phase 2; with one,
+            // phase 1 of the next offset waits here
+            if constexpr (HSB == 1) __syncthreads();
             ++doff;
             if (++dx > S) {
                 dx = -S;
@@ -429,25 +434,33 @@ __global__ __launch_bounds__(64) void k_nlm_spot_final(const float* __restrict__
     }
 }
 
-template <int P>
-hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeight wp, const uint8_t* regions, hipStream_t stream) {
-    size_t lds = nlm_tile_lds_bytes(S, P);
+template <int P, int PW, int HSB>
+hipError_t launch_tile_pw(const float* in, float* out, int w, int h, int S, NlmWeight wp, const uint8_t* regions, hipStream_t stream) {
+    const size_t lds = (size_t)(nlm_pt_floats(S + P, P, PW) + HSB * NLM_HSR * NLM_HS) * sizeof(float);
     dim3 grid((unsigned)((w + NLM_TW - 1) / NLM_TW), (unsigned)((h + nlm_th(P) - 1) / nlm_th(P)));
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_nlm_tile<P>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)nlm_tile_lds_bytes(NLM_MAX_SP - P, P));
+        const int sp_max = PW == NLM_PW_NARROW ? NLM_NARROW_SP : NLM_MAX_SP;
+        const size_t max_lds = (size_t)(nlm_pt_floats(sp_max, P, PW) + HSB * NLM_HSR * NLM_HS) * sizeof(float);
+        hipError_t e = hipFuncSetAttribute((const void*)k_nlm_tile<P, PW, HSB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_nlm_tile<P>, grid, dim3(NLM_NT), lds, stream, in, out, w, h, S, wp, regions);
+    hipLaunchKernelGGL((k_nlm_tile<P, PW, HSB>), grid, dim3(NLM_NT), lds, stream, in, out, w, h, S, wp, regions);
     return hipGetLastError();
+}
+
+template <int P>
+hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeight wp, const uint8_t* regions, hipStream_t stream) {
+    if (S + P <= NLM_NARROW_SP) return launch_tile_pw<P, NLM_PW_NARROW, 1>(in, out, w, h, S, wp, regions, stream);
+    return launch_tile_pw<P, NLM_PW_WIDE, 2>(in, out, w, h, S, wp, regions, stream);
 }
 
 }  // namespace
 
 size_t nlm_tile_lds_bytes(int S, int P) {
-    return (size_t)(nlm_pt_floats(S + P, P) + 2 * NLM_HSR * NLM_HS) * sizeof(float);
+    if (S + P <= NLM_NARROW_SP) return (size_t)(nlm_pt_floats(S + P, P, NLM_PW_NARROW) + NLM_HSR * NLM_HS) * sizeof(float);
+    return (size_t)(nlm_pt_floats(S + P, P, NLM_PW_WIDE) + 2 * NLM_HSR * NLM_HS) * sizeof(float);
 }
 
 namespace {
