@@ -54,50 +54,71 @@ __global__ void k_noise_sample(const float* __restrict__ p, size_t w, size_t h, 
     vals[k] = bits;
 }
 
-// Exact upper median of the valid samples (select_nth_unstable at len/2) by
-// a 4 x 8-bit MSB-first radix select in one workgroup, then
-// sigma = v / 0.6745 / sqrt(20).
-constexpr int SEL_NT = 1024;
-__global__ __launch_bounds__(SEL_NT) void k_noise_select(const uint32_t* __restrict__ vals, size_t count, float sqrt20,
+// Exact element of rank `target` (select_nth_unstable at len / 2, the valid
+// count coming from the sampling geometry) by a 4 x 8-bit MSB-first radix
+// select in one workgroup, then sigma = v / 0.6745 / sqrt(20).  Border
+// markers (+inf) sort after every finite sample, so they never reach the
+// selected rank.  Each wave counts into its own histogram (the top bytes of
+// the samples cluster in a few bins, which a shared histogram serialises),
+// and the bucket holding the rank is found with a wave prefix scan.
+constexpr int SEL_NT = 1024, SEL_WAVES = SEL_NT / 64;
+__global__ void k_sigma_zero(float* __restrict__ sigma_out) {
+    if (threadIdx.x == 0) *sigma_out = 0.0f;
+}
+__global__ __launch_bounds__(SEL_NT) void k_noise_select(const uint32_t* __restrict__ vals, size_t count, uint32_t target0, float sqrt20,
                                                          float* __restrict__ sigma_out) {
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t s_red[SEL_NT / 64];
+    __shared__ uint32_t hist[SEL_WAVES][256];
+    __shared__ uint32_t s_wsum[4];
     __shared__ uint32_t s_prefix, s_target;
-    const int tid = threadIdx.x;
-    uint32_t nv = 0;
-    for (size_t k = tid; k < count; k += SEL_NT) nv += vals[k] != kInfBits ? 1u : 0u;
-    for (int off = 32; off > 0; off >>= 1) nv += __shfl_down(nv, off, 64);
-    if ((tid & 63) == 0) s_red[tid >> 6] = nv;
-    __syncthreads();
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) {
-        uint32_t t = 0;
-        for (int i = 0; i < SEL_NT / 64; i++) t += s_red[i];
-        s_target = t / 2;
         s_prefix = 0;
-        if (t == 0) *sigma_out = 0.0f;  // enhance.rs:25-27
-        s_red[0] = t;
+        s_target = target0;
     }
-    __syncthreads();
-    if (s_red[0] == 0) return;
     uint32_t mask = 0;
     for (int shift = 24; shift >= 0; shift -= 8) {
-        for (int i = tid; i < 256; i += SEL_NT) hist[i] = 0;
+        for (int i = tid; i < SEL_WAVES * 256; i += SEL_NT) (&hist[0][0])[i] = 0;
         __syncthreads();
-        const uint32_t prefix = s_prefix;
-        for (size_t k = tid; k < count; k += SEL_NT) {
-            uint32_t v = vals[k];
-            if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255u], 1u);
+        // read before the barriers below: one thread rewrites both at the end
+        const uint32_t prefix = s_prefix, target = s_target;
+        // eight loads in flight per thread before their atomics
+        constexpr int SEL_B = 8;
+        for (size_t base = 0; base < count; base += (size_t)SEL_B * SEL_NT) {
+            uint32_t v[SEL_B];
+#pragma unroll
+            for (int q = 0; q < SEL_B; q++) {
+                const size_t k = base + (size_t)q * SEL_NT + tid;
+                v[q] = k < count ? vals[k] : ~0u;
+            }
+#pragma unroll
+            for (int q = 0; q < SEL_B; q++)
+                if (base + (size_t)q * SEL_NT + tid < count && (v[q] & mask) == prefix) atomicAdd(&hist[wave][(v[q] >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t target = s_target, cum = 0;
-            int b = 0;
-            for (; b < 256; b++) {
-                if (cum + hist[b] > target) break;
-                cum += hist[b];
+        // bins 0..255 on threads 0..255 (waves 0..3): totals, inclusive scan
+        // within each wave, then across the four waves
+        uint32_t tot = 0, incl = 0;
+        if (tid < 256) {
+#pragma unroll
+            for (int w = 0; w < SEL_WAVES; w++) tot += hist[w][tid];
+            incl = tot;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t u = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += u;
             }
-            s_target = target - cum;
-            s_prefix = prefix | ((uint32_t)b << shift);
+            if (lane == 63) s_wsum[wave] = incl;
+        }
+        __syncthreads();
+        if (tid < 256) {
+            uint32_t before = 0;
+            for (int w = 0; w < wave; w++) before += s_wsum[w];
+            const uint32_t excl = before + incl - tot;
+            // the one bin whose cumulative range holds the rank
+            if (excl <= target && target < excl + tot) {
+                s_prefix = prefix | ((uint32_t)tid << shift);
+                s_target = target - excl;
+            }
         }
         mask |= 255u << shift;
         __syncthreads();
@@ -177,18 +198,31 @@ size_t noise_sample_count(size_t w, size_t h) {
     return (n + step - 1) / step;
 }
 
+// Samples of enhance.rs:12-22 off the 1-pixel border: for each inner row y,
+// the multiples of step in [y w + 1, y w + w - 2].
+size_t noise_valid_samples(size_t w, size_t h, size_t step) {
+    if (w < 3 || h < 3) return 0;
+    size_t v = 0;
+    for (size_t y = 1; y + 1 < h; y++) {
+        const size_t lo = y * w + 1, hi = y * w + w - 2;
+        v += hi / step - (lo + step - 1) / step + 1;
+    }
+    return v;
+}
+
 hipError_t launch_estimate_noise(const float* plane, size_t w, size_t h, uint32_t* scratch, float* sigma_out, hipStream_t st) {
-    size_t n = w * h;
-    if (n == 0) {
-        hipLaunchKernelGGL(k_noise_select, dim3(1), dim3(SEL_NT), 0, st, scratch, (size_t)0, 1.0f, sigma_out);
+    const size_t n = w * h;
+    const size_t step = n / 200000 > 1 ? n / 200000 : 1;
+    const size_t valid = noise_valid_samples(w, h, step);
+    if (valid == 0) {  // enhance.rs:25-27
+        hipLaunchKernelGGL(k_sigma_zero, dim3(1), dim3(64), 0, st, sigma_out);
         return hipGetLastError();
     }
-    size_t step = n / 200000 > 1 ? n / 200000 : 1;
-    size_t count = (n + step - 1) / step;
+    const size_t count = (n + step - 1) / step;
     hipLaunchKernelGGL(k_noise_sample, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, plane, w, h, step, count, scratch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_noise_select, dim3(1), dim3(SEL_NT), 0, st, scratch, count, sqrtf(20.0f), sigma_out);
+    hipLaunchKernelGGL(k_noise_select, dim3(1), dim3(SEL_NT), 0, st, scratch, count, (uint32_t)(valid / 2), sqrtf(20.0f), sigma_out);
     return hipGetLastError();
 }
 
