@@ -43,12 +43,11 @@ HL_MUL = (2.0, 1.0, 1.5)
 # elementwise pass reads and writes its planes once, each separable gaussian
 # pass reads and writes one plane.  Sharpen (masking 0), super_res_refine and
 # output_sharpen: luminance 16, gaussian 2 x 8, ratio 32 (RGB in and out, two
-# planes in).  Noise reduction at the reference's defaults (colour 25, so a
-# fast guided filter at sub 4 per chroma plane): Oklab split 24, per chroma
-# plane 8 to resize L and the plane down and 8 for the coefficients' apply,
-# the two mixes 24, the recomposition 24; the guided filters on the 1/16-size
-# planes are left out.
-DETAIL_BYTES_PER_PX = {"sharpen": 64, "super_res_refine": 64, "output_sharpen": 64, "noise_reduction": 104}
+# planes in).  Noise reduction at the reference's defaults (colour 25, so the
+# chroma fast guided filters at sub 4): Oklab split 24, the downscale of L, a
+# and b 12, the upsampled a I + b mixed into a and b 20 (L, a, b in; a, b out),
+# the recomposition 24; the guided filters on the 1/16-size planes are left out.
+DETAIL_BYTES_PER_PX = {"sharpen": 64, "super_res_refine": 64, "output_sharpen": 64, "noise_reduction": 80}
 
 
 def saturate(raw, seed, discs=12):
