@@ -121,7 +121,9 @@ def luminance_plane(O, seed, w, h, iso=1.0):
     return np.ascontiguousarray(O.oklab_from_prophoto(O.rcd_bayer(cfa))[..., 0])
 
 
-@pytest.mark.parametrize("S,P", [(3, 1), (10, 3), (1, 0), (2, 2), (5, 5), (0, 2), (20, 5)])
+# search + patch <= 13 takes the narrow tile instance, 14..24 the wide one,
+# beyond that the direct kernel
+@pytest.mark.parametrize("S,P", [(3, 1), (10, 3), (1, 0), (2, 2), (5, 5), (0, 2), (20, 5), (11, 3), (10, 4), (15, 4), (19, 5)])
 @pytest.mark.parametrize("w,h", [(96, 64), (130, 70), (7, 5)])
 def test_nlm(ctx, oracle_mod, S, P, w, h):
     O = oracle_mod
