@@ -314,6 +314,48 @@ def highlights_leg(ctx, args, raw, rgb, clip, w, h):
     return res
 
 
+def transfer_leg(ctx, step, raw, out, w, h, reps):
+    """The same step from host memory to host memory: the u16 frame copied in
+    from pinned host memory, the device pipeline, the RGB result copied back,
+    one synchronisation per frame; and each copy on its own."""
+    import darkslide_amd as D
+    L = D.lib()
+    n = w * h
+    hin, hout = ctypes.c_void_p(), ctypes.c_void_p()
+    ctx._check(L.ds_host_alloc(ctx.handle, n * 2, ctypes.byref(hin)))
+    ctx._check(L.ds_host_alloc(ctx.handle, n * 12, ctypes.byref(hout)))
+    p_in, p_out, pinned = hin.value, hout.value, True
+    src, dst = ctypes.c_void_p(p_in), ctypes.c_void_p(p_out)
+    ctx._check(L.ds_memcpy_d2h(ctx.handle, src, raw.ptr, n * 2))
+    ctx.sync()
+
+    def timed(fn):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+            ctx.sync()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    def whole():
+        ctx._check(L.ds_memcpy_h2d(ctx.handle, raw.ptr, src, n * 2))
+        step()
+        ctx._check(L.ds_memcpy_d2h(ctx.handle, dst, out.ptr, n * 12))
+
+    ms = timed(whole)
+    h2d = timed(lambda: ctx._check(L.ds_memcpy_h2d(ctx.handle, raw.ptr, src, n * 2)))
+    d2h = timed(lambda: ctx._check(L.ds_memcpy_d2h(ctx.handle, dst, out.ptr, n * 12)))
+    res = {"ms_per_frame": round(ms, 3), "mp_s": round(n / 1e6 / (ms * 1e-3), 1), "pinned_host_memory": pinned,
+           "h2d_bytes": n * 2, "h2d_ms": round(h2d, 3), "h2d_gbs": round(n * 2 / (h2d * 1e-3) / 1e9, 1),
+           "d2h_bytes": n * 12, "d2h_ms": round(d2h, 3), "d2h_gbs": round(n * 12 / (d2h * 1e-3) / 1e9, 1),
+           "note": "u16 frame in, RGB f32 out, synchronised per frame; not the headline value"}
+    ctx._check(L.ds_host_free(ctx.handle, hin))
+    ctx._check(L.ds_host_free(ctx.handle, hout))
+    log(f"host to host: {res}")
+    return res
+
+
 def detail_leg(ctx, rgb, w, h):
     """The render detail stages (pipeline/detail.rs) and the export post-steps
     (export.rs:244-247), in place on the bench's resident 24 MP image, timed
@@ -541,6 +583,7 @@ def main():
     learned = None
     if not args.no_learned:
         learned = learned_leg(ctx, args, raw, rgb, den, out, w, h, ms_per_step)
+    host_to_host = transfer_leg(ctx, step, raw, out, w, h, 5)
     highlights = highlights_leg(ctx, args, raw, rgb, clip, w, h)
     detail = detail_leg(ctx, rgb, w, h)
     batched = None
@@ -580,6 +623,7 @@ def main():
         "stages_ms": stages,
         "learned": learned,
         "batched": batched,
+        "host_to_host": host_to_host,
         "highlights": highlights,
         "detail": detail,
         "stage_errors": errors,

@@ -169,6 +169,8 @@ _SIGS = {
     "ds_luminance": [_P, _P, _SZ, _P],
     "ds_dev_alloc": [_P, _SZ, ctypes.POINTER(_P)],
     "ds_dev_free": [_P, _P],
+    "ds_host_alloc": [_P, _SZ, ctypes.POINTER(_P)],
+    "ds_host_free": [_P, _P],
     "ds_memcpy_h2d": [_P, _P, _P, _SZ],
     "ds_memcpy_d2h": [_P, _P, _P, _SZ],
     "ds_sync": [_P],

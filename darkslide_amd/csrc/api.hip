@@ -244,6 +244,7 @@ struct ds_ctx {
     std::string err;
     Buf slots[S_COUNT];
     std::vector<void*> user_allocs;
+    std::vector<void*> host_allocs;  // pinned host buffers from ds_host_alloc
     OklabM ok;
     bool prof = false;
     std::vector<Pending> pending;
@@ -1505,6 +1506,7 @@ void ds_ctx_destroy(ds_ctx* c) {
         for (auto& b : c->slots)
             if (b.p) hipFree(b.p);
         for (void* p : c->user_allocs) hipFree(p);
+        for (void* p : c->host_allocs) hipHostFree(p);
         hipStreamDestroy(c->stream);
     }
     delete c;
@@ -1560,6 +1562,26 @@ int ds_dev_free(ds_ctx* c, void* p) {
             return DS_OK;
         }
     return fail(c, DS_ERR_INVALID, "pointer was not allocated by ds_dev_alloc on this context");
+}
+
+int ds_host_alloc(ds_ctx* c, size_t bytes, void** hptr) {
+    LOCK(c);
+    if (!hptr) return fail(c, DS_ERR_INVALID, "hptr is null");
+    HIPCHK(c, hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+    c->host_allocs.push_back(*hptr);
+    return DS_OK;
+}
+
+int ds_host_free(ds_ctx* c, void* p) {
+    LOCK(c);
+    for (size_t i = 0; i < c->host_allocs.size(); i++)
+        if (c->host_allocs[i] == p) {
+            HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            HIPCHK(c, hipHostFree(p), "hipHostFree");
+            c->host_allocs.erase(c->host_allocs.begin() + (long)i);
+            return DS_OK;
+        }
+    return fail(c, DS_ERR_INVALID, "pointer was not allocated by ds_host_alloc on this context");
 }
 
 int ds_memcpy_h2d(ds_ctx* c, void* dst, const void* src, size_t bytes) {

@@ -203,6 +203,12 @@ int ds_luminance(ds_ctx* ctx, const float* rgb, size_t n, float* out);
 
 int ds_dev_alloc(ds_ctx* ctx, size_t bytes, void** dptr);
 int ds_dev_free(ds_ctx* ctx, void* dptr);
+/* Pinned (page-locked) host memory for staging frames: copies from and to it
+ * run at the link's rate, where pageable memory is copied through a bounce
+ * buffer.  No counterpart in the reference (its buffers are Vec<f32>); freed
+ * with the context if not before. */
+int ds_host_alloc(ds_ctx* ctx, size_t bytes, void** hptr);
+int ds_host_free(ds_ctx* ctx, void* hptr);
 int ds_memcpy_h2d(ds_ctx* ctx, void* dst, const void* src, size_t bytes); /* synchronous */
 int ds_memcpy_d2h(ds_ctx* ctx, void* dst, const void* src, size_t bytes); /* synchronous */
 int ds_sync(ds_ctx* ctx);

@@ -381,3 +381,32 @@ def test_full_frame_nlm_and_rl(ctx, oracle_mod, frame24):
     y_in = rgb @ np.array([0.2880402, 0.7118741, 0.0000857], np.float32)
     y_out = out @ np.array([0.2880402, 0.7118741, 0.0000857], np.float32)
     assert abs(float(y_out.mean()) / float(y_in.mean()) - 1) < 2e-3
+
+
+def test_pinned_host_buffers_round_trip(ctx):
+    """ds_host_alloc: a frame through pinned host memory and back, unchanged."""
+    import ctypes
+    import darkslide_amd as D
+    L = D.lib()
+    n = 1 << 20
+    data = np.random.default_rng(3).random(n).astype(np.float32)
+    dev = ctx.alloc(n * 4)
+    dev.upload(data)
+    hp = ctypes.c_void_p()
+    ctx._check(L.ds_host_alloc(ctx.handle, n * 4, ctypes.byref(hp)))
+    try:
+        ctx._check(L.ds_memcpy_d2h(ctx.handle, hp, dev.ptr, n * 4))
+        ctx.sync()
+        host = np.ctypeslib.as_array((ctypes.c_float * n).from_address(hp.value))
+        assert np.array_equal(host, data)
+        host[:] = data * 2.0
+        dev2 = ctx.alloc(n * 4)
+        ctx._check(L.ds_memcpy_h2d(ctx.handle, dev2.ptr, hp, n * 4))
+        ctx.sync()
+        assert np.array_equal(dev2.download((n,), np.float32), data * 2.0)
+        dev2.free()
+    finally:
+        ctx._check(L.ds_host_free(ctx.handle, hp))
+    with pytest.raises(D.DarkslideError):
+        ctx._check(L.ds_host_free(ctx.handle, hp))
+    dev.free()
