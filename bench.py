@@ -254,6 +254,37 @@ def batched_leg(ctx, args):
             entry["regions"] = last["regions"]
         res["legs"][name] = entry
         log(f"batched {name}: {entry['value']} MP/s ({t:.3f} s per batch, {info['branches']} branches)")
+    # the first frames again from pinned host memory: the graph copies each
+    # frame in and its result out, overlapping other branches' kernels; the
+    # same frames device-resident beside it
+    nh = min(args.batch_host_frames, len(frames))
+    if nh:
+        sub, hbufs, mp = [], [], 0
+        for f in frames[:nh]:
+            hin, hout = ctx.host_alloc(f.w * f.h * 2), ctx.host_alloc(f.w * f.h * 12)
+            ctx._check(L.ds_memcpy_d2h(ctx.handle, hin.ptr, ctypes.c_void_p(f.raw), f.w * f.h * 2))
+            g = D.Frame.from_buffer_copy(f)
+            g.raw, g.rgb_out = hin.ptr.value, hout.ptr.value
+            sub.append(g)
+            hbufs += [hin, hout]
+            mp += f.w * f.h
+        ctx.sync()
+        kw = dict(denoise=D.DENOISE_NLM, amount=0.0, search=10, patch=3, raw_details=True, rl_sigma=1.0, rl_iters=20)
+        out = {"frames": nh, "megapixels": round(mp / 1e6, 1)}
+        for name, fr, host in (("device_resident", frames[:nh], False), ("pinned_host", sub, True)):
+            info = ctx.pipeline_batch(fr, streams=0, autotune=True, host_frames=host, **kw)
+            times = []
+            for _ in range(args.batch_reps):
+                t0 = time.perf_counter()
+                ctx.pipeline_batch(fr, streams=0, autotune=True, host_frames=host, info=False, **kw)
+                ctx.sync()
+                times.append(time.perf_counter() - t0)
+            t = statistics.median(times)
+            out[name] = {"value": round(mp / 1e6 / t, 2), "unit": "MP/s", "s_per_batch": round(t, 4), "branches": info["branches"]}
+            log(f"batched subset {name}: {out[name]['value']} MP/s")
+        res["legs"]["plain_host_frames"] = out
+        for b in hbufs:
+            b.free()
     tuning = {}
     for fw, fh in BATCH_SIZES:
         tuning[f"{fw}x{fh}"] = ctx.tuning(fw, fh)
@@ -490,6 +521,7 @@ def main():
     ap.add_argument("--learned-sizes", default="6000x4000,8256x5504")
     ap.add_argument("--batch-frames", type=int, default=48, help="frames of the batched-export leg (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--batch-host-frames", type=int, default=10, help="frames of the batch also run from pinned host memory")
     ap.add_argument("--no-errors", action="store_true", help="skip the per-stage error report against the oracle")
     ap.add_argument("--error-size", type=int, default=384)
     args = ap.parse_args()

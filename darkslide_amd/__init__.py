@@ -84,7 +84,7 @@ class PipelineOpts(ctypes.Structure):
                 ("patch", ctypes.c_size_t), ("raw_details", ctypes.c_int), ("rl_sigma", ctypes.c_float), ("rl_iters", ctypes.c_size_t),
                 ("reference_gating", ctypes.c_int), ("net", ctypes.c_void_p), ("threshold_db", ctypes.c_float),
                 ("streams", ctypes.c_int), ("autotune", ctypes.c_int), ("highlights", ctypes.c_int),
-                ("quality", ctypes.c_int)]
+                ("quality", ctypes.c_int), ("host_frames", ctypes.c_int)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -241,6 +241,31 @@ def _f3(v):
     return (ctypes.c_float * 3)(*[float(x) for x in v])
 
 
+class HostBuffer:
+    """Pinned host allocation owned by a Context (ds_host_alloc; freed with it or by .free())."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        ctx._check(lib().ds_host_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    def array(self, shape, dtype):
+        """A numpy view of the buffer (valid until .free())."""
+        dt = np.dtype(dtype)
+        count = int(np.prod(shape))
+        if count * dt.itemsize > self.nbytes:
+            raise ValueError("view larger than buffer")
+        raw = (ctypes.c_uint8 * (count * dt.itemsize)).from_address(self.ptr.value)
+        return np.frombuffer(raw, dtype=dt, count=count).reshape(shape)
+
+    def free(self):
+        if self.ptr is not None:
+            self.ctx._check(lib().ds_host_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+
 class DeviceBuffer:
     """Device allocation owned by a Context (freed with it or by .free())."""
 
@@ -309,6 +334,9 @@ class Context:
     # ---- device memory helpers
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)
+
+    def host_alloc(self, nbytes):
+        return HostBuffer(self, nbytes)
 
     def sync(self):
         self._check(lib().ds_sync(self.handle))
@@ -533,12 +561,13 @@ class Context:
 
     def pipeline_batch(self, frames, algo=ALGO_RCD, denoise=DENOISE_NLM, amount=0.0, search=10, patch=3, raw_details=True,
                        rl_sigma=1.0, rl_iters=20, reference_gating=False, net=None, threshold_db=0.3, streams=0, autotune=False,
-                       info=True, highlights=False, quality=QUALITY_FULL):
-        """Runs the batch graph over `frames` (a list of Frame); returns the batch info (or None with info=False)."""
+                       info=True, highlights=False, quality=QUALITY_FULL, host_frames=False):
+        """Runs the batch graph over `frames` (a list of Frame); returns the batch info (or None with info=False).
+        host_frames: the frames' raw and rgb_out are pinned host buffers (HostBuf)."""
         arr = (Frame * len(frames))(*frames)
         o = PipelineOpts(algo, denoise, amount, search, patch, int(raw_details), rl_sigma, rl_iters, int(reference_gating),
                          net.handle.value if net is not None else None, threshold_db, streams, int(autotune), int(highlights),
-                         int(quality))
+                         int(quality), int(host_frames))
         bi = BatchInfo()
         self._check(lib().ds_pipeline_batch(self.handle, arr, len(frames), ctypes.byref(o), ctypes.byref(bi) if info else None))
         return bi.as_dict() if info else None

@@ -205,7 +205,7 @@ struct Buf {
 
 enum Slot {
     S_IN = 0, S_IN2, S_OUT, S_CLIP, S_WS, S_L, S_A, S_B, S_LF, S_SIGMA, S_NOISE, S_SCENE, S_X, S_T, S_REGIONS, S_ACT,
-    S_PIPE_RGB, S_PIPE_DEN, S_COUNTS, S_STATS, S_HL, S_PIPE_CLIP, S_COUNT
+    S_PIPE_RGB, S_PIPE_DEN, S_COUNTS, S_STATS, S_HL, S_PIPE_CLIP, S_PIPE_RAW, S_PIPE_OUT, S_COUNT
 };
 
 // S_HL: clip levels (3 floats) at 0, cfa_clip_levels partials from HL_PART_OFF.
@@ -990,7 +990,7 @@ int autotune_shape(ds_ctx* c, size_t w, size_t h, ds_tuning* out) {
 
 struct FramePlan {
     size_t idx;
-    bool highlights, denoise, learned, rl, draft;
+    bool highlights, denoise, learned, rl, draft, host;
     size_t w, h;  // size of the demosaiced image
     float amount;
     float rl_sigma;
@@ -1004,6 +1004,7 @@ FramePlan plan_frame(const ds_frame& f, size_t idx, const ds_pipeline_opts& o) {
     p.w = p.draft ? f.w / 2 : f.w;
     p.h = p.draft ? f.h / 2 : f.h;
     p.highlights = o.highlights != 0;
+    p.host = o.host_frames != 0;
     p.denoise = o.denoise != DS_DENOISE_NONE;
     p.learned = o.denoise == DS_DENOISE_LEARNED;
     p.amount = o.amount;
@@ -1025,6 +1026,10 @@ FramePlan plan_frame(const ds_frame& f, size_t idx, const ds_pipeline_opts& o) {
 void frame_needs(const ds_frame& f, const FramePlan& p, size_t need[S_COUNT]) {
     const size_t n = p.w * p.h;
     auto up = [&](Slot s, size_t b) { need[s] = std::max(need[s], b); };
+    if (p.host) {
+        up(S_PIPE_RAW, f.stride * f.h * 2);
+        up(S_PIPE_OUT, n * 12);
+    }
     if (p.denoise || p.rl) up(S_PIPE_RGB, n * 12);
     if (p.highlights) {
         up(S_PIPE_CLIP, n * 4);
@@ -1087,8 +1092,26 @@ void retire_stale_graphs(ds_ctx* c) {
     c->graphs.swap(keep);
 }
 
-// One frame's stages on branch context b.
-int enqueue_frame(ds_ctx* b, const ds_frame& f, const FramePlan& p, const ds_pipeline_opts& o, int* count_dev) {
+int enqueue_stages(ds_ctx* b, const ds_frame& f, const FramePlan& p, const ds_pipeline_opts& o, int* count_dev);
+
+// One frame's stages on branch context b.  Host frames are copied into the
+// branch's staging buffers first and their result copied back last, on the
+// branch's stream, so one branch's copies overlap the others' kernels.
+int enqueue_frame(ds_ctx* b, const ds_frame& hf, const FramePlan& p, const ds_pipeline_opts& o, int* count_dev) {
+    ds_frame f = hf;
+    if (p.host) {
+        SLOT(b, draw, uint16_t*, S_PIPE_RAW, hf.stride * hf.h * 2);
+        SLOT(b, dout, float*, S_PIPE_OUT, p.w * p.h * 12);
+        HIPCHK(b, hipMemcpyAsync(draw, hf.raw, hf.stride * hf.h * 2, hipMemcpyHostToDevice, b->stream), "H2D (host frame)");
+        f.raw = draw;
+        f.rgb_out = dout;
+    }
+    if (int rc = enqueue_stages(b, f, p, o, count_dev)) return rc;
+    if (p.host) HIPCHK(b, hipMemcpyAsync(hf.rgb_out, f.rgb_out, p.w * p.h * 12, hipMemcpyDeviceToHost, b->stream), "D2H (host frame)");
+    return DS_OK;
+}
+
+int enqueue_stages(ds_ctx* b, const ds_frame& f, const FramePlan& p, const ds_pipeline_opts& o, int* count_dev) {
     const bool direct = !p.denoise && !p.rl;
     const size_t w = p.w, h = p.h;
     const float* bs = f.per_site ? f.black_site : nullptr;
@@ -1128,6 +1151,17 @@ int enqueue_frame(ds_ctx* b, const ds_frame& f, const FramePlan& p, const ds_pip
     return rc;
 }
 
+// Page-locked host memory (hipHostMalloc / ds_host_alloc): copies from and to
+// it are asynchronous, so they can sit in the captured graph.
+bool pinned_host(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 int validate_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_opts& o) {
     if (!frames && n) return fail(c, DS_ERR_INVALID, "frames is null");
     if (o.algo != DS_ALGO_RCD && o.algo != DS_ALGO_MHC) return fail(c, DS_ERR_INVALID, "unknown demosaic algorithm %d", o.algo);
@@ -1146,7 +1180,10 @@ int validate_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipelin
         if (f.stride < f.w) return fail(c, DS_ERR_INVALID, "frame %zu: stride < width", i);
         if (!valid_bayer(f.pattern)) return fail(c, DS_ERR_INVALID, "frame %zu: not a Bayer pattern", i);
         if (o.quality == DS_QUALITY_DRAFT && (f.w < 2 || f.h < 2)) return fail(c, DS_ERR_INVALID, "frame %zu: too small for superpixels", i);
+        if (o.host_frames && (!pinned_host(f.raw) || !pinned_host(f.rgb_out)))
+            return fail(c, DS_ERR_INVALID, "frame %zu: host frames must be in pinned host memory (ds_host_alloc)", i);
     }
+    if (o.host_frames != 0 && o.host_frames != 1) return fail(c, DS_ERR_INVALID, "host_frames must be 0 or 1");
     return DS_OK;
 }
 
@@ -1172,6 +1209,7 @@ std::vector<uint8_t> batch_key(ds_ctx* c, const ds_frame* frames, size_t n, cons
     put(&o.threshold_db, sizeof o.threshold_db);
     put(&o.highlights, sizeof o.highlights);
     put(&o.quality, sizeof o.quality);
+    put(&o.host_frames, sizeof o.host_frames);
     for (size_t i = 0; i < n; i++) {
         const ds_frame& f = frames[i];
         put(&f.raw, sizeof f.raw);
@@ -1288,6 +1326,21 @@ int capture_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline
 
 constexpr size_t kMaxGraphs = 4;
 
+// Waits for the batch and fills the learned path's region counts.
+int batch_counts(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_opts& o, const int* counts, ds_batch_info* info) {
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (o.denoise == DS_DENOISE_LEARNED) {
+        std::vector<int> cnt(n);
+        HIPCHK(c, hipMemcpy(cnt.data(), counts, n * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        for (size_t i = 0; i < n; i++) {
+            info->fallback_regions += (size_t)cnt[i];
+            const FramePlan p = plan_frame(frames[i], i, o);
+            info->regions += nlm_regions(p.w, p.h);
+        }
+    }
+    return DS_OK;
+}
+
 int pipeline_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_opts& o, ds_batch_info* info) {
     if (info) std::memset(info, 0, sizeof(*info));
     if (int rc = validate_batch(c, frames, n, o)) return rc;
@@ -1386,17 +1439,7 @@ int pipeline_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipelin
     info->frames = n;
     info->branches = (size_t)hit->branches;
     info->capture_ms = capture_ms;
-    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    if (o.denoise == DS_DENOISE_LEARNED) {
-        std::vector<int> cnt(n);
-        HIPCHK(c, hipMemcpy(cnt.data(), counts, n * sizeof(int), hipMemcpyDeviceToHost), "D2H");
-        for (size_t i = 0; i < n; i++) {
-            info->fallback_regions += (size_t)cnt[i];
-            const FramePlan p = plan_frame(frames[i], i, o);
-            info->regions += nlm_regions(p.w, p.h);
-        }
-    }
-    return DS_OK;
+    return batch_counts(c, frames, n, o, counts, info);
 }
 
 int wb_stats_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, const float* n_as, ds_wb_stats* out) {

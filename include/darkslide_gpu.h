@@ -387,6 +387,8 @@ typedef struct ds_pipeline_opts {
     int autotune;
     int highlights;       /* 1: reconstruct_highlights after demosaic, as Engine::base does (pipeline/mod.rs:186-187) */
     int quality;          /* DS_QUALITY_FULL (RCD / MHC) or DS_QUALITY_DRAFT (superpixel, half size; no RL under reference_gating) */
+    int host_frames;      /* 1: every frame's raw and rgb_out are pinned host buffers (ds_host_alloc); the graph copies each
+                             frame in and its result out on its branch, overlapping other branches' kernels */
 } ds_pipeline_opts;
 
 typedef struct ds_batch_info {

@@ -121,6 +121,40 @@ def test_batch_graph_replay(ctx, batch, expected_plain):
             assert np.array_equal(got, ref), (rep, k)
 
 
+@pytest.mark.parametrize("streams", [1, 3])
+def test_batch_host_frames(ctx, batch, expected_plain, streams):
+    """host_frames: the same frames in pinned host memory, copied in and out
+    inside the graph, give the device-frame results bit for bit, on replay
+    too; pageable host memory is rejected."""
+    import darkslide_amd as D
+    say("batch host frames, streams", streams)
+    hb, frames = [], []
+    try:
+        for f, (raw, _, w, h, *_rest) in zip(batch.frames, batch.bufs):
+            hin, hout = ctx.host_alloc(w * h * 2), ctx.host_alloc(w * h * 12)
+            hin.array((h, w), np.uint16)[:] = raw.download((h, w), np.uint16)
+            g = D.Frame.from_buffer_copy(f)
+            g.raw, g.rgb_out = hin.ptr.value, hout.ptr.value
+            frames.append(g)
+            hb.append((hin, hout, w, h))
+        for rep in range(2):
+            for (_, hout, w, h) in hb:
+                hout.array((h, w, 3), np.float32)[:] = np.nan
+            info = ctx.pipeline_batch(frames, streams=streams, host_frames=True)
+            assert info["frames"] == len(frames) and info["branches"] == streams and info["graph_reused"] == (1 if rep else 0)
+            for k, ((_, hout, w, h), ref) in enumerate(zip(hb, expected_plain)):
+                assert np.array_equal(hout.array((h, w, 3), np.float32), ref), (rep, k)
+        pageable = np.zeros(FRAMES[0][0] * FRAMES[0][1], np.uint16)
+        g = D.Frame.from_buffer_copy(frames[0])
+        g.raw = pageable.ctypes.data
+        with pytest.raises(D.DarkslideError):
+            ctx.pipeline_batch([g], streams=1, host_frames=True)
+    finally:
+        for (hin, hout, *_r) in hb:
+            hin.free()
+            hout.free()
+
+
 def test_batch_after_workspace_growth(ctx, batch, expected_plain):
     """A larger batch grows the branch workspaces; the first batch's graph is captured again, not replayed stale."""
     say("batch growth")
