@@ -555,8 +555,7 @@ int highlights_dev(ds_ctx* c, float* rgb, const float* clip, size_t w, size_t h,
 }
 
 int noise_dev(ds_ctx* c, const float* plane, size_t w, size_t h, float* sigma_dev) {
-    size_t cnt = noise_sample_count(w, h);
-    SLOT(c, scratch, uint32_t*, S_NOISE, (cnt ? cnt : 1) * sizeof(uint32_t));
+    SLOT(c, scratch, uint32_t*, S_NOISE, noise_scratch_u32(w, h) * sizeof(uint32_t));
     return run(c, "estimate_noise", [&] { return launch_estimate_noise(plane, w, h, scratch, sigma_dev, c->stream); });
 }
 
@@ -1044,8 +1043,7 @@ void frame_needs(const ds_frame& f, const FramePlan& p, size_t need[S_COUNT]) {
         up(S_LF, n * 4);
         up(S_SIGMA, 64);
         up(S_WS, 12 * n * 4);
-        const size_t cnt = noise_sample_count(p.w, p.h);
-        up(S_NOISE, (cnt ? cnt : 1) * sizeof(uint32_t));
+        up(S_NOISE, noise_scratch_u32(p.w, p.h) * sizeof(uint32_t));
         if (p.learned) {
             const size_t nreg = nlm_regions(p.w, p.h);
             up(S_REGIONS, (2 * nreg + 64 + nlm_spot_scratch_floats(p.w, p.h)) * 4);

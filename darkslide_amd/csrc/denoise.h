@@ -10,12 +10,14 @@ namespace ds {
 
 hipError_t launch_oklab_fwd(const float* rgb, size_t n, const OklabM& ok, float* L, float* A, float* B, hipStream_t st);
 
-// Samples estimate_noise reads from a w x h plane (the scratch size in u32).
+// Samples estimate_noise reads from a w x h plane.
 size_t noise_sample_count(size_t w, size_t h);
-// estimate_noise(plane) into *sigma_out (device).  scratch: noise_sample_count u32.
+// Scratch of launch_estimate_noise in u32: a first-level histogram and the samples.
+size_t noise_scratch_u32(size_t w, size_t h);
 // Number of samples estimate_noise keeps (off the 1-pixel border) for a
 // w x h plane sampled every step pixels.
 size_t noise_valid_samples(size_t w, size_t h, size_t step);
+// estimate_noise(plane) into *sigma_out (device).  scratch: noise_scratch_u32 u32.
 hipError_t launch_estimate_noise(const float* plane, size_t w, size_t h, uint32_t* scratch, float* sigma_out, hipStream_t st);
 
 // The two guided filters of denoise (enhance.rs:86-91) and the inverse Oklab:

@@ -38,89 +38,197 @@ __global__ void k_oklab_fwd(const float* __restrict__ rgb, size_t n, OklabM ok, 
 
 // ---- noise estimate: every step-th pixel off the 1-px border, |laplacian|
 // as ordered u32 bits; non-samples get +inf and sort last.
+//
+// The element of rank len/2 (select_nth_unstable, enhance.rs:23) is found by
+// an exact MSB-first radix select over the bits in three levels: 12, 10 and
+// 10 bits.  The first level (sign, exponent, 3 mantissa bits) is counted by
+// the sampling kernel itself into a global histogram, so its LDS atomics --
+// which pile onto the few bins the exponents of real data fall in -- are
+// spread over the whole grid; the select kernel then only scans that
+// histogram and counts the two lower levels over the samples in the chosen
+// bin.
 constexpr uint32_t kInfBits = 0x7f800000u;
+constexpr int NOISE_L1_SHIFT = 20, NOISE_L1_BINS = 1 << (32 - NOISE_L1_SHIFT);
+constexpr int NS_NT = 256, NS_PER = 4;  // sampling block: 1024 samples
 
-__global__ void k_noise_sample(const float* __restrict__ p, size_t w, size_t h, size_t step, size_t count, uint32_t* __restrict__ vals) {
-    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    size_t i = k * step;
-    size_t x = i % w, y = i / w;
-    uint32_t bits = kInfBits;
-    if (x >= 1 && y >= 1 && x + 1 < w && y + 1 < h) {
-        float c = p[i];
-        float lap = 4.0f * c - p[i - 1] - p[i + 1] - p[i - w] - p[i + w];
-        bits = __float_as_uint(fabsf(lap));
+// One histogram increment per active lane of the wave; the lanes sharing
+// the first active lane's bin (a hot bin, most of the time) add their count
+// with one atomic instead of serialising on its address.
+__device__ inline void wave_count(uint32_t* hist, uint32_t bin, bool active) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long act = __ballot(active);
+    if (!act) return;
+    const int leader = __ffsll(act) - 1;
+    const uint32_t lb = __shfl(bin, leader, 64);
+    const bool same = active && bin == lb;
+    const unsigned long long sm = __ballot(same);
+    if (same) {
+        if (lane == leader) atomicAdd(&hist[lb], (uint32_t)__popcll(sm));
+    } else if (active) {
+        atomicAdd(&hist[bin], 1u);
     }
-    vals[k] = bits;
 }
 
-// Exact element of rank `target` (select_nth_unstable at len / 2, the valid
-// count coming from the sampling geometry) by a 4 x 8-bit MSB-first radix
-// select in one workgroup, then sigma = v / 0.6745 / sqrt(20).  Border
-// markers (+inf) sort after every finite sample, so they never reach the
-// selected rank.  Each wave counts into its own histogram (the top bytes of
-// the samples cluster in a few bins, which a shared histogram serialises),
-// and the bucket holding the rank is found with a wave prefix scan.
+// Appends the active lanes' values to buf (capacity cap) in lane order.
+__device__ inline void wave_append(uint32_t* buf, uint32_t* n, uint32_t cap, uint32_t v, bool active) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long act = __ballot(active);
+    if (!act) return;
+    const int leader = __ffsll(act) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(n, (uint32_t)__popcll(act));
+    base = __shfl(base, leader, 64);
+    const uint32_t at = base + (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+    if (active && at < cap) buf[at] = v;
+}
+
+__global__ __launch_bounds__(NS_NT) void k_noise_hist_clear(uint32_t* __restrict__ hist) {
+    hist[blockIdx.x * NS_NT + threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(NS_NT) void k_noise_sample(const float* __restrict__ p, size_t w, size_t h, size_t step, size_t count,
+                                                        uint32_t* __restrict__ vals, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t lh[NOISE_L1_BINS];
+    for (int i = threadIdx.x; i < NOISE_L1_BINS; i += NS_NT) lh[i] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * (NS_NT * NS_PER);
+#pragma unroll
+    for (int q = 0; q < NS_PER; q++) {
+        const size_t k = base + (size_t)q * NS_NT + threadIdx.x;
+        const bool in = k < count;
+        uint32_t bits = kInfBits;
+        if (in) {
+            const size_t i = k * step;
+            const size_t x = i % w, y = i / w;
+            if (x >= 1 && y >= 1 && x + 1 < w && y + 1 < h) {
+                const float c = p[i];
+                const float lap = 4.0f * c - p[i - 1] - p[i + 1] - p[i - w] - p[i + w];
+                bits = __float_as_uint(fabsf(lap));
+            }
+            vals[k] = bits;
+        }
+        wave_count(lh, bits >> NOISE_L1_SHIFT, in);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NOISE_L1_BINS; i += NS_NT)
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+}
+
 constexpr int SEL_NT = 1024, SEL_WAVES = SEL_NT / 64;
+constexpr uint32_t SEL_CAP = 16384;  // samples of the chosen first-level bin kept in LDS
+static_assert(NOISE_L1_BINS == 4 * SEL_NT, "first level: four bins per select thread");
+static_assert(SEL_NT == 1 << 10, "lower levels: one 10-bit bin per select thread");
+
 __global__ void k_sigma_zero(float* __restrict__ sigma_out) {
     if (threadIdx.x == 0) *sigma_out = 0.0f;
 }
-__global__ __launch_bounds__(SEL_NT) void k_noise_select(const uint32_t* __restrict__ vals, size_t count, uint32_t target0, float sqrt20,
-                                                         float* __restrict__ sigma_out) {
-    __shared__ uint32_t hist[SEL_WAVES][256];
-    __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_prefix, s_target;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) {
-        s_prefix = 0;
-        s_target = target0;
+
+// Exclusive prefix of v over the workgroup (thread order).  Ends with the
+// totals of every wave published; the caller syncs before s_wsum is reused.
+__device__ inline uint32_t sel_excl_scan(uint32_t v, uint32_t* s_wsum) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
     }
-    uint32_t mask = 0;
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        for (int i = tid; i < SEL_WAVES * 256; i += SEL_NT) (&hist[0][0])[i] = 0;
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int i = 0; i < wave; i++) before += s_wsum[i];
+    return before + incl - v;
+}
+
+// Rank target0 of the samples, then sigma = v / 0.6745 / sqrt(20)
+// (enhance.rs:23-29).  hist1: the first-level counts of every sample
+// (border markers included: +inf sorts after every finite sample, and the
+// rank is below the number of finite ones, so they never reach it).
+__global__ __launch_bounds__(SEL_NT) void k_noise_select(const uint32_t* __restrict__ hist1, const uint32_t* __restrict__ vals, size_t count,
+                                                         uint32_t target0, float sqrt20, float* __restrict__ sigma_out) {
+    __shared__ uint32_t hist[SEL_NT];
+    __shared__ uint32_t buf[SEL_CAP];
+    __shared__ uint32_t s_wsum[SEL_WAVES];
+    __shared__ uint32_t s_prefix, s_target, s_binn, s_n;
+    const int tid = threadIdx.x;
+
+    // level 1: bins 4 tid .. 4 tid + 3 of the global histogram
+    {
+        const uint4 c4 = reinterpret_cast<const uint4*>(hist1)[tid];
+        const uint32_t tot = c4.x + c4.y + c4.z + c4.w;
+        const uint32_t excl = sel_excl_scan(tot, s_wsum);
+        if (excl <= target0 && target0 < excl + tot) {
+            const uint32_t c[4] = {c4.x, c4.y, c4.z, c4.w};
+            uint32_t t = target0 - excl;
+            int j = 0;
+            while (t >= c[j]) t -= c[j++];
+            s_prefix = (uint32_t)(4 * tid + j) << NOISE_L1_SHIFT;
+            s_target = t;
+            s_binn = c[j];
+        }
+        hist[tid] = 0;
+        if (tid == 0) s_n = 0;
         __syncthreads();
-        // read before the barriers below: one thread rewrites both at the end
-        const uint32_t prefix = s_prefix, target = s_target;
-        // eight loads in flight per thread before their atomics
-        constexpr int SEL_B = 8;
+    }
+
+    // level 2: bits 19..10 of the samples in the chosen bin, which are also
+    // gathered into LDS when they fit
+    const uint32_t p1 = s_prefix, t1 = s_target;
+    const bool compact = s_binn <= SEL_CAP;
+    constexpr int SEL_B = 8;  // loads in flight per thread
+    for (size_t base = 0; base < count; base += (size_t)SEL_B * SEL_NT) {
+        uint32_t v[SEL_B];
+#pragma unroll
+        for (int q = 0; q < SEL_B; q++) {
+            const size_t k = base + (size_t)q * SEL_NT + tid;
+            v[q] = k < count ? vals[k] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < SEL_B; q++) {
+            const bool m = base + (size_t)q * SEL_NT + tid < count && (v[q] >> NOISE_L1_SHIFT) == (p1 >> NOISE_L1_SHIFT);
+            wave_count(hist, (v[q] >> 10) & 1023u, m);
+            if (compact) wave_append(buf, &s_n, SEL_CAP, v[q], m);
+        }
+    }
+    __syncthreads();
+    {
+        const uint32_t tot = hist[tid];
+        const uint32_t excl = sel_excl_scan(tot, s_wsum);
+        if (excl <= t1 && t1 < excl + tot) {
+            s_prefix = p1 | ((uint32_t)tid << 10);
+            s_target = t1 - excl;
+        }
+        hist[tid] = 0;
+        __syncthreads();
+    }
+
+    // level 3: bits 9..0 of the samples matching the 22-bit prefix
+    const uint32_t p2 = s_prefix, t2 = s_target;
+    if (compact) {
+        const uint32_t n = s_n < SEL_CAP ? s_n : SEL_CAP;
+        for (uint32_t base = 0; base < n; base += SEL_NT) {
+            const uint32_t i = base + tid;
+            const uint32_t v = i < n ? buf[i] : 0u;
+            wave_count(hist, v & 1023u, i < n && (v >> 10) == (p2 >> 10));
+        }
+    } else {
         for (size_t base = 0; base < count; base += (size_t)SEL_B * SEL_NT) {
             uint32_t v[SEL_B];
 #pragma unroll
             for (int q = 0; q < SEL_B; q++) {
                 const size_t k = base + (size_t)q * SEL_NT + tid;
-                v[q] = k < count ? vals[k] : ~0u;
+                v[q] = k < count ? vals[k] : 0u;
             }
 #pragma unroll
             for (int q = 0; q < SEL_B; q++)
-                if (base + (size_t)q * SEL_NT + tid < count && (v[q] & mask) == prefix) atomicAdd(&hist[wave][(v[q] >> shift) & 255u], 1u);
+                wave_count(hist, v[q] & 1023u, base + (size_t)q * SEL_NT + tid < count && (v[q] >> 10) == (p2 >> 10));
         }
-        __syncthreads();
-        // bins 0..255 on threads 0..255 (waves 0..3): totals, inclusive scan
-        // within each wave, then across the four waves
-        uint32_t tot = 0, incl = 0;
-        if (tid < 256) {
-#pragma unroll
-            for (int w = 0; w < SEL_WAVES; w++) tot += hist[w][tid];
-            incl = tot;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const uint32_t u = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += u;
-            }
-            if (lane == 63) s_wsum[wave] = incl;
-        }
-        __syncthreads();
-        if (tid < 256) {
-            uint32_t before = 0;
-            for (int w = 0; w < wave; w++) before += s_wsum[w];
-            const uint32_t excl = before + incl - tot;
-            // the one bin whose cumulative range holds the rank
-            if (excl <= target && target < excl + tot) {
-                s_prefix = prefix | ((uint32_t)tid << shift);
-                s_target = target - excl;
-            }
-        }
-        mask |= 255u << shift;
+    }
+    __syncthreads();
+    {
+        const uint32_t tot = hist[tid];
+        const uint32_t excl = sel_excl_scan(tot, s_wsum);
+        if (excl <= t2 && t2 < excl + tot) s_prefix = p2 | (uint32_t)tid;
         __syncthreads();
     }
     if (tid == 0) *sigma_out = __uint_as_float(s_prefix) / 0.6745f / sqrt20;
@@ -198,6 +306,8 @@ size_t noise_sample_count(size_t w, size_t h) {
     return (n + step - 1) / step;
 }
 
+size_t noise_scratch_u32(size_t w, size_t h) { return NOISE_L1_BINS + noise_sample_count(w, h); }
+
 // Samples of enhance.rs:12-22 off the 1-pixel border: for each inner row y,
 // the multiples of step in [y w + 1, y w + w - 2].
 size_t noise_valid_samples(size_t w, size_t h, size_t step) {
@@ -219,10 +329,15 @@ hipError_t launch_estimate_noise(const float* plane, size_t w, size_t h, uint32_
         return hipGetLastError();
     }
     const size_t count = (n + step - 1) / step;
-    hipLaunchKernelGGL(k_noise_sample, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, plane, w, h, step, count, scratch);
+    // scratch: the first-level histogram, then the samples
+    uint32_t* hist = scratch;
+    uint32_t* vals = scratch + NOISE_L1_BINS;
+    hipLaunchKernelGGL(k_noise_hist_clear, dim3(NOISE_L1_BINS / NS_NT), dim3(NS_NT), 0, st, hist);
+    const size_t per = (size_t)NS_NT * NS_PER;
+    hipLaunchKernelGGL(k_noise_sample, dim3((unsigned)((count + per - 1) / per)), dim3(NS_NT), 0, st, plane, w, h, step, count, vals, hist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_noise_select, dim3(1), dim3(SEL_NT), 0, st, scratch, count, (uint32_t)(valid / 2), sqrtf(20.0f), sigma_out);
+    hipLaunchKernelGGL(k_noise_select, dim3(1), dim3(SEL_NT), 0, st, hist, vals, count, (uint32_t)(valid / 2), sqrtf(20.0f), sigma_out);
     return hipGetLastError();
 }
 

@@ -115,6 +115,27 @@ def test_estimate_noise_exact(ctx, oracle_mod, w, h):
     assert ctx.estimate_noise(p) == O.estimate_noise(p)
 
 
+@pytest.mark.parametrize("kind", ["constant", "quantized", "octaves", "smooth"])
+def test_estimate_noise_concentrated(ctx, oracle_mod, kind):
+    """Sample sets whose rank falls in a crowded bin: all-equal laplacians
+    (constant), a few repeated values (quantized: the chosen bins hold far
+    more samples than fit in LDS), values over 30 octaves, and a smooth
+    gradient with tiny noise at a large size."""
+    O = oracle_mod
+    rng = np.random.default_rng(11)
+    w, h = (1024, 768) if kind != "smooth" else (3000, 2000)
+    if kind == "constant":
+        p = np.full((h, w), 0.37, np.float32)
+    elif kind == "quantized":
+        p = (rng.integers(0, 4, (h, w)) / np.float32(4)).astype(np.float32)
+    elif kind == "octaves":
+        p = (10.0 ** rng.uniform(-9, 0, (h, w))).astype(np.float32)
+    else:
+        yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+        p = (0.2 + 1e-4 * xx + 5e-5 * yy + 1e-6 * rng.standard_normal((h, w))).astype(np.float32)
+    assert ctx.estimate_noise(p) == O.estimate_noise(p)
+
+
 def luminance_plane(O, seed, w, h, iso=1.0):
     raw = synth(seed, w, h, 14, iso)
     cfa = O.cfa_normalize_u16(raw, RGGB, 512.0, list(WHITE))
