@@ -20,6 +20,8 @@
 // ~(h/64) * planes waves for the horizontal pass, ~(w/64) * planes for the
 // vertical one.
 
+#include <atomic>
+
 #include "box.h"
 #include "common.h"
 
@@ -149,7 +151,22 @@ __device__ __forceinline__ void raw_barrier() {
 // fixed offset given by RM = r % 4 (a template parameter), so 9 + 9 unit
 // reads serve 32 columns.  The output tile holds unit (group m, row t) at
 // m * 64 + (t ^ m).  Needs w % 4 == 0 and 16-byte aligned planes.
+// A block owns rb = 16 nq <= 64 rows (the band is chosen by the host to
+// spread the rows x planes evenly over the CUs): only the first nq 16-row
+// groups are loaded and stored; walker lanes past the band walk stale ring
+// slots and are never stored.
 constexpr int HS = 32;
+
+// s_waitcnt until at most 2 nq loads are in flight (nq uniform, 1..4): the
+// loaders issue nq loads per step and keep D - 1 = 2 steps in flight.
+__device__ __forceinline__ void wait_two_steps(int nq) {
+    switch (nq) {
+        case 1: __builtin_amdgcn_s_waitcnt(waitcnt_vm(2)); break;
+        case 2: __builtin_amdgcn_s_waitcnt(waitcnt_vm(4)); break;
+        case 3: __builtin_amdgcn_s_waitcnt(waitcnt_vm(6)); break;
+        default: __builtin_amdgcn_s_waitcnt(waitcnt_vm(8)); break;
+    }
+}
 
 template <int I>
 __device__ __forceinline__ float comp(const float4& v) {
@@ -165,13 +182,15 @@ __device__ __forceinline__ float pick(const float4& lo, const float4& hi) {
 }
 
 template <int RC, int D, int RM>
-__global__ __launch_bounds__(MW_THREADS) void k_boxh_ring(CBoxPlanes in, BoxPlanes out, int w, int h, int r, float norm) {
+__global__ __launch_bounds__(MW_THREADS) void k_boxh_ring(CBoxPlanes in, BoxPlanes out, int w, int h, int r, float norm, int nq) {
+    static_assert(D == 3, "wait_two_steps keeps two steps of loads in flight");
     if (out.gate && *out.gate == 0) return;
     constexpr int NB = RC / 16;
     __shared__ float ring[RC * BX];
     __shared__ float4 otile[2 * (HS / 4) * BX];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int y0 = blockIdx.x * BX;
+    const int rb = 16 * nq;
+    const int y0 = blockIdx.x * rb;
     const int k = blockIdx.y;
     const float* __restrict__ src = in.p[k];
     float* __restrict__ dst = out.p[k];
@@ -189,7 +208,8 @@ __global__ __launch_bounds__(MW_THREADS) void k_boxh_ring(CBoxPlanes in, BoxPlan
     const int wbase = (lane >> 4) * 256 + 16 * (lane & 15), wx = (lane >> 1) & 3;
     auto get = [&](int c) { return ring[((c >> 4) & (NB - 1)) * 1024 + wbase + 4 * (((c >> 2) & 3) ^ wx) + (c & 3)]; };
     const int A0 = (HS * D + r + 1 + 15) & ~15;
-    for (int it = wave; it < (A0 >> 4) * 4; it += MOVERS + 1) load_group(16 * (it >> 2), it & 3);
+    for (int it = wave; it < (A0 >> 4) * 4; it += MOVERS + 1)
+        if ((it & 3) < nq) load_group(16 * (it >> 2), it & 3);
     __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     __syncthreads();
     float acc = 0.0f;
@@ -222,7 +242,7 @@ __global__ __launch_bounds__(MW_THREADS) void k_boxh_ring(CBoxPlanes in, BoxPlan
 #pragma unroll
         for (int i = 0; i < BX / 8; i++) {
             const int row = (lane >> 3) + 8 * i;
-            if (c < w && y0 + row < h) *reinterpret_cast<float4*>(dst + (uint32_t)(y0 + row) * uw + (uint32_t)c) = v[i];
+            if (c < w && row < rb && y0 + row < h) *reinterpret_cast<float4*>(dst + (uint32_t)(y0 + row) * uw + (uint32_t)c) = v[i];
         }
     };
     const int steps = (w + HS - 1) / HS;
@@ -270,8 +290,9 @@ __global__ __launch_bounds__(MW_THREADS) void k_boxh_ring(CBoxPlanes in, BoxPlan
         } else if (wave <= 2) {
             const int L = A0 + HS * s + 16 * (wave - 1);
 #pragma unroll
-            for (int q = 0; q < 4; q++) load_group(L, q);
-            __builtin_amdgcn_s_waitcnt(waitcnt_vm(4 * (D - 1)));
+            for (int q = 0; q < 4; q++)
+                if (q < nq) load_group(L, q);
+            wait_two_steps(nq);
         } else if (s > 0) {
             store_step(s - 1);
         }
@@ -302,14 +323,18 @@ __device__ __forceinline__ void lds_load16(const float* g, float* lds) { __built
 // one load covering 4 rows x 64 columns, lane l taking row l / 16 and
 // columns 4 (l % 16) .. + 3, so the units land in the ring's row-major slots
 // as they are; the prefetch start is rounded up to a whole 4-row group.
+// A block owns cb <= 64 columns (cb % 4 == 0 with V16; the band is chosen
+// by the host to spread the columns x planes evenly over the CUs); lanes
+// past the band shadow its last column and are never stored.
 template <int RING, int D, bool V16>
-__global__ __launch_bounds__(MW_THREADS) void k_boxv_ring(CBoxPlanes in, BoxPlanes out, int w, int h, int r, float norm) {
+__global__ __launch_bounds__(MW_THREADS) void k_boxv_ring(CBoxPlanes in, BoxPlanes out, int w, int h, int r, float norm, int cb) {
     if (out.gate && *out.gate == 0) return;
     __shared__ float ring[RING * BX];
     __shared__ float otile[2 * VS * BX];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int x = min(blockIdx.x * BX + lane, w - 1);  // lanes past the edge shadow the last column
-    const bool live = blockIdx.x * BX + lane < w;
+    const int x0 = blockIdx.x * cb;
+    const int x = min(x0 + min(lane, cb - 1), w - 1);  // lanes past the band or the edge shadow its last column
+    const bool live = lane < cb && x0 + lane < w;
     const int k = blockIdx.y;
     const float* __restrict__ src = in.p[k];
     float* __restrict__ dst = out.p[k];
@@ -319,7 +344,8 @@ __global__ __launch_bounds__(MW_THREADS) void k_boxv_ring(CBoxPlanes in, BoxPlan
     const int A0 = V16 ? (VS * D + r + 1 + 3) & ~3 : VS * D + r + 1;
     // 16-byte loader lane: row within its 4-row group, first column of its unit
     const int qrow = lane >> 4;
-    const uint32_t qcol = (uint32_t)min((int)blockIdx.x * BX + 4 * (lane & 15), w - 4);
+    const int qu = min(lane & 15, cb / 4 - 1);  // 16-byte unit of the band
+    const uint32_t qcol = (uint32_t)min(x0 + 4 * qu, w - 4);
     auto load_quad = [&](int c) {  // rows c .. c + 3 (c % 4 == 0), clamped per lane
         lds_load16(row_in(min(c + qrow, h - 1)) + qcol, ring + (c & (RING - 1)) * BX);
     };
@@ -341,7 +367,7 @@ __global__ __launch_bounds__(MW_THREADS) void k_boxv_ring(CBoxPlanes in, BoxPlan
             float4 u[VS / 4];
 #pragma unroll
             for (int i = 0; i < VS / 4; i++) u[i] = *reinterpret_cast<const float4*>(ot + (qrow + 4 * i) * BX + 4 * (lane & 15));
-            if ((int)blockIdx.x * BX + 4 * (lane & 15) >= w) return;
+            if (4 * (lane & 15) >= cb || x0 + 4 * (lane & 15) >= w) return;
 #pragma unroll
             for (int i = 0; i < VS / 4; i++)
                 if (Y0 + qrow + 4 * i < h) *reinterpret_cast<float4*>(dst + (size_t)(Y0 + qrow + 4 * i) * (size_t)w + qcol) = u[i];
@@ -406,6 +432,39 @@ __global__ __launch_bounds__(MW_THREADS) void k_boxv_ring(CBoxPlanes in, BoxPlan
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
+// Compute units of the current device (cached per device).
+int device_cus() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cache[dev].load(std::memory_order_relaxed);
+    if (n == 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cache[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+// Band (rows or columns per block, a multiple of `step`, 32..64) of a
+// ring pass over `extent` rows or columns of `planes` planes.  A CU moves
+// data at about the same rate with one block or two, so a pass lasts as
+// long as its busiest CU: ceil(blocks / CUs) bands.  The smallest such
+// load wins, ties going to the wider band (fewer blocks).
+int ring_band(size_t extent, int planes, int step) {
+    const size_t cus = (size_t)device_cus();
+    int best = BX;
+    size_t best_load = ~(size_t)0;
+    for (int b = BX; b >= 32; b -= step) {
+        const size_t blocks = (extent + b - 1) / b * (size_t)planes;
+        const size_t load = (blocks + cus - 1) / cus * (size_t)b;
+        if (load < best_load) {
+            best_load = load;
+            best = b;
+        }
+    }
+    return best;
+}
+
 // Any radius: one thread per row / column reading global memory directly.
 template <class In>
 __global__ void k_boxx_h_any(In in, BoxPlanes out, int w, int h, int r, float norm, int planes) {
@@ -457,9 +516,10 @@ hipError_t box_h_planes(CBoxPlanes in, BoxPlanes out, int planes, size_t w, size
     if (!ring) return run_h(BoxInPlanes{in}, out, planes, w, h, r, st);
     const float norm = 1.0f / (float)(2 * r + 1);  // image.rs:274
     // ring sizes per the bound RC > 32 D + 2 r + 47 with D = 3
-    dim3 grid((unsigned)((h + BX - 1) / BX), (unsigned)planes);
-    const int ri = (int)r;
-#define DS_BOXH(RC, RM) hipLaunchKernelGGL((k_boxh_ring<RC, 3, RM>), grid, dim3(MW_THREADS), 0, st, in, out, (int)w, (int)h, ri, norm)
+    const int rb = ring_band(h, planes, 16);
+    dim3 grid((unsigned)((h + rb - 1) / rb), (unsigned)planes);
+    const int ri = (int)r, nq = rb / 16;
+#define DS_BOXH(RC, RM) hipLaunchKernelGGL((k_boxh_ring<RC, 3, RM>), grid, dim3(MW_THREADS), 0, st, in, out, (int)w, (int)h, ri, norm, nq)
 #define DS_BOXH_RM(RC)              \
     switch (ri & 3) {               \
         case 0: DS_BOXH(RC, 0); break; \
@@ -479,12 +539,13 @@ hipError_t box_h_planes(CBoxPlanes in, BoxPlanes out, int planes, size_t w, size
 
 hipError_t box_v_planes(CBoxPlanes in, BoxPlanes out, int planes, size_t w, size_t h, size_t r, hipStream_t st) {
     const float norm = 1.0f / (float)(2 * r + 1);
-    const dim3 grid((unsigned)((w + BX - 1) / BX), (unsigned)planes);
     bool v16 = w % 4 == 0;
     for (int k = 0; k < planes && v16; k++) v16 = aligned16(in.p[k]) && aligned16(out.p[k]);
+    const int cb = r <= 191 ? ring_band(w, planes, 4) : BX;
+    const dim3 grid((unsigned)((w + cb - 1) / cb), (unsigned)planes);
     // ring sizes from the schedule's bound RING > 32 D + 2 r + 32 with D = 3
     // (+ 3 when the prefetch start is rounded to a 4-row group)
-#define DS_BOXV(RING, V) hipLaunchKernelGGL((k_boxv_ring<RING, 3, V>), grid, dim3(MW_THREADS), 0, st, in, out, (int)w, (int)h, (int)r, norm)
+#define DS_BOXV(RING, V) hipLaunchKernelGGL((k_boxv_ring<RING, 3, V>), grid, dim3(MW_THREADS), 0, st, in, out, (int)w, (int)h, (int)r, norm, cb)
     if (v16 && r <= 60)
         DS_BOXV(256, true);
     else if (v16 && r <= 188)

@@ -280,6 +280,28 @@ def test_box_blur_ring_limits(ctx, oracle_mod, w, h, r):
     assert np.array_equal(got, want), f"bit-equal {np.mean(got == want):.6f}"
 
 
+# Ring passes split the rows (columns) x planes into bands of 32..64 rows
+# (columns) per block so that the busiest CU's share is smallest.  On a
+# 256-CU device these shapes select row bands 64, 48 and 32 and column bands
+# 64, 56, 52, 48, 44 and 40 (16-byte and 4-byte column loads); the result
+# must not depend on the band.
+@pytest.mark.parametrize("w,h,r", [(64, 16384, 3), (68, 12288, 17), (16384, 40, 5), (14336, 36, 21), (13312, 20, 2),
+                                   (12288, 33, 9), (10240, 24, 60), (11264, 50, 7), (14334, 30, 6)])
+def test_box_blur_bands(ctx, oracle_mod, w, h, r):
+    rng = np.random.default_rng(w + h + r)
+    p = rng.random((h, w)).astype(np.float32)
+    got, want = ctx.box_blur(p, r), oracle_mod.box_blur(p, r)
+    assert np.array_equal(got, want), f"bit-equal {np.mean(got == want):.6f}"
+
+
+def test_guided_filter_bands(ctx, oracle_mod):
+    # four-plane row passes over 3072 rows take 48-row bands
+    rng = np.random.default_rng(5)
+    I = rng.random((3072, 96)).astype(np.float32)
+    p = (0.5 * I + 0.3 * rng.random((3072, 96))).astype(np.float32)
+    assert np.array_equal(ctx.guided_filter(I, p, 6, 1e-3), oracle_mod.guided_filter(I, p, 6, 1e-3))
+
+
 @pytest.mark.parametrize("sigma", [0.2, 0.5, 0.75, 1.0, 1.9, 2.5, 7.0])
 def test_gaussian_blur_exact(ctx, oracle_mod, sigma):
     rng = np.random.default_rng(int(sigma * 10))
