@@ -40,16 +40,25 @@ __global__ void k_oklab_fwd(const float* __restrict__ rgb, size_t n, OklabM ok, 
 // as ordered u32 bits; non-samples get +inf and sort last.
 //
 // The element of rank len/2 (select_nth_unstable, enhance.rs:23) is found by
-// an exact MSB-first radix select over the bits in three levels: 12, 10 and
-// 10 bits.  The first level (sign, exponent, 3 mantissa bits) is counted by
-// the sampling kernel itself into a global histogram, so its LDS atomics --
-// which pile onto the few bins the exponents of real data fall in -- are
-// spread over the whole grid; the select kernel then only scans that
-// histogram and counts the two lower levels over the samples in the chosen
-// bin.
+// an exact MSB-first radix select over the bits in three levels of 12, 10
+// and 10 bits, each level spread over the whole grid where it touches every
+// sample:
+//   k_noise_sample  writes the samples and counts level 1 (sign, exponent,
+//                   3 mantissa bits) into a global histogram;
+//   k_noise_l2      ranks level 1 (every block, redundantly), counts level 2
+//                   of the samples in the chosen bin and appends those
+//                   samples to a list (one global atomic per block);
+//   k_noise_final   ranks levels 1 and 2 and counts level 3 over the list,
+//                   which holds only the samples of the chosen level-1 bin.
+// Border markers (+inf) sort after every finite sample, and the rank is
+// below the number of finite ones, so they never reach it.
 constexpr uint32_t kInfBits = 0x7f800000u;
 constexpr int NOISE_L1_SHIFT = 20, NOISE_L1_BINS = 1 << (32 - NOISE_L1_SHIFT);
-constexpr int NS_NT = 256, NS_PER = 4;  // sampling block: 1024 samples
+constexpr int NOISE_L2_BINS = 1024;
+constexpr int NOISE_CTRL = 64;  // [0]: list length
+constexpr int NOISE_HDR = NOISE_L1_BINS + NOISE_L2_BINS + NOISE_CTRL;
+static_assert(NOISE_HDR % 4 == 0, "16-byte aligned list and samples");
+constexpr int NS_NT = 256, NS_PER = 4;  // sample / level-2 block: 1024 samples
 
 // One histogram increment per active lane of the wave; the lanes sharing
 // the first active lane's bin (a hot bin, most of the time) add their count
@@ -82,12 +91,54 @@ __device__ inline void wave_append(uint32_t* buf, uint32_t* n, uint32_t cap, uin
     if (active && at < cap) buf[at] = v;
 }
 
-__global__ __launch_bounds__(NS_NT) void k_noise_hist_clear(uint32_t* __restrict__ hist) {
-    hist[blockIdx.x * NS_NT + threadIdx.x] = 0;
+// Exclusive prefix of v over the workgroup (thread order).  Ends with the
+// totals of every wave published; the caller syncs before s_wsum is reused.
+__device__ inline uint32_t sel_excl_scan(uint32_t v, uint32_t* s_wsum) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int i = 0; i < wave; i++) before += s_wsum[i];
+    return before + incl - v;
+}
+
+// The bin of a NT * PER-bin histogram holding rank `target`, and the rank
+// within it: s_sel = {bin, rank in bin}.  Thread t reads bins PER t ..
+// PER t + PER - 1.  Ends with a barrier.
+template <int NT, int PER>
+__device__ inline void rank_bin(const uint32_t* bins, uint32_t target, uint32_t* s_wsum, uint32_t* s_sel) {
+    const int tid = threadIdx.x;
+    uint32_t c[PER];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        c[j] = bins[PER * tid + j];
+        tot += c[j];
+    }
+    const uint32_t excl = sel_excl_scan(tot, s_wsum);
+    if (excl <= target && target < excl + tot) {
+        uint32_t t = target - excl;
+        int j = 0;
+        while (t >= c[j]) t -= c[j++];
+        s_sel[0] = (uint32_t)(PER * tid + j);
+        s_sel[1] = t;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(NS_NT) void k_noise_hist_clear(uint32_t* __restrict__ hdr) {
+    const int i = blockIdx.x * NS_NT + threadIdx.x;
+    if (i < NOISE_HDR) hdr[i] = 0;
 }
 
 __global__ __launch_bounds__(NS_NT) void k_noise_sample(const float* __restrict__ p, size_t w, size_t h, size_t step, size_t count,
-                                                        uint32_t* __restrict__ vals, uint32_t* __restrict__ hist) {
+                                                        uint32_t* __restrict__ vals, uint32_t* __restrict__ hist1) {
     __shared__ uint32_t lh[NOISE_L1_BINS];
     for (int i = threadIdx.x; i < NOISE_L1_BINS; i += NS_NT) lh[i] = 0;
     __syncthreads();
@@ -111,127 +162,80 @@ __global__ __launch_bounds__(NS_NT) void k_noise_sample(const float* __restrict_
     }
     __syncthreads();
     for (int i = threadIdx.x; i < NOISE_L1_BINS; i += NS_NT)
-        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+        if (lh[i]) atomicAdd(&hist1[i], lh[i]);
 }
 
-constexpr int SEL_NT = 1024, SEL_WAVES = SEL_NT / 64;
-constexpr uint32_t SEL_CAP = 16384;  // samples of the chosen first-level bin kept in LDS
-static_assert(NOISE_L1_BINS == 4 * SEL_NT, "first level: four bins per select thread");
-static_assert(SEL_NT == 1 << 10, "lower levels: one 10-bit bin per select thread");
+__global__ __launch_bounds__(NS_NT) void k_noise_l2(const uint32_t* __restrict__ vals, size_t count, uint32_t target0,
+                                                    const uint32_t* __restrict__ hist1, uint32_t* __restrict__ hist2,
+                                                    uint32_t* __restrict__ ctrl, uint32_t* __restrict__ list) {
+    __shared__ uint32_t h2[NOISE_L2_BINS];
+    __shared__ uint32_t keep[NS_NT * NS_PER];
+    __shared__ uint32_t s_wsum[NS_NT / 64];
+    __shared__ uint32_t s_sel[2], s_n, s_base;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NOISE_L2_BINS; i += NS_NT) h2[i] = 0;
+    if (tid == 0) s_n = 0;
+    // the block's samples are loaded before the level-1 rank is known
+    const size_t base = (size_t)blockIdx.x * (NS_NT * NS_PER);
+    uint32_t v[NS_PER];
+#pragma unroll
+    for (int q = 0; q < NS_PER; q++) {
+        const size_t k = base + (size_t)q * NS_NT + tid;
+        v[q] = k < count ? vals[k] : 0u;
+    }
+    rank_bin<NS_NT, NOISE_L1_BINS / NS_NT>(hist1, target0, s_wsum, s_sel);
+    const uint32_t b1 = s_sel[0];
+#pragma unroll
+    for (int q = 0; q < NS_PER; q++) {
+        const bool m = base + (size_t)q * NS_NT + tid < count && (v[q] >> NOISE_L1_SHIFT) == b1;
+        wave_count(h2, (v[q] >> 10) & 1023u, m);
+        wave_append(keep, &s_n, NS_NT * NS_PER, v[q], m);
+    }
+    __syncthreads();
+    const uint32_t n = s_n;
+    if (tid == 0 && n) s_base = atomicAdd(&ctrl[0], n);
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += NS_NT) list[s_base + i] = keep[i];
+    for (int i = tid; i < NOISE_L2_BINS; i += NS_NT)
+        if (h2[i]) atomicAdd(&hist2[i], h2[i]);
+}
+
+constexpr int SEL_NT = 1024;
+static_assert(NOISE_L1_BINS == 4 * SEL_NT && NOISE_L2_BINS == SEL_NT, "final ranks: four level-1 bins, one level-2 / level-3 bin per thread");
 
 __global__ void k_sigma_zero(float* __restrict__ sigma_out) {
     if (threadIdx.x == 0) *sigma_out = 0.0f;
 }
 
-// Exclusive prefix of v over the workgroup (thread order).  Ends with the
-// totals of every wave published; the caller syncs before s_wsum is reused.
-__device__ inline uint32_t sel_excl_scan(uint32_t v, uint32_t* s_wsum) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t incl = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t u = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += u;
-    }
-    if (lane == 63) s_wsum[wave] = incl;
-    __syncthreads();
-    uint32_t before = 0;
-    for (int i = 0; i < wave; i++) before += s_wsum[i];
-    return before + incl - v;
-}
-
-// Rank target0 of the samples, then sigma = v / 0.6745 / sqrt(20)
-// (enhance.rs:23-29).  hist1: the first-level counts of every sample
-// (border markers included: +inf sorts after every finite sample, and the
-// rank is below the number of finite ones, so they never reach it).
-__global__ __launch_bounds__(SEL_NT) void k_noise_select(const uint32_t* __restrict__ hist1, const uint32_t* __restrict__ vals, size_t count,
-                                                         uint32_t target0, float sqrt20, float* __restrict__ sigma_out) {
-    __shared__ uint32_t hist[SEL_NT];
-    __shared__ uint32_t buf[SEL_CAP];
-    __shared__ uint32_t s_wsum[SEL_WAVES];
-    __shared__ uint32_t s_prefix, s_target, s_binn, s_n;
+// sigma = v / 0.6745 / sqrt(20) of the sample of rank target0 (enhance.rs:23-29).
+__global__ __launch_bounds__(SEL_NT) void k_noise_final(const uint32_t* __restrict__ hist1, const uint32_t* __restrict__ hist2,
+                                                        const uint32_t* __restrict__ ctrl, const uint32_t* __restrict__ list,
+                                                        uint32_t target0, float sqrt20, float* __restrict__ sigma_out) {
+    __shared__ uint32_t h3[SEL_NT];
+    __shared__ uint32_t s_wsum[SEL_NT / 64];
+    __shared__ uint32_t s_sel[2];
     const int tid = threadIdx.x;
-
-    // level 1: bins 4 tid .. 4 tid + 3 of the global histogram
-    {
-        const uint4 c4 = reinterpret_cast<const uint4*>(hist1)[tid];
-        const uint32_t tot = c4.x + c4.y + c4.z + c4.w;
-        const uint32_t excl = sel_excl_scan(tot, s_wsum);
-        if (excl <= target0 && target0 < excl + tot) {
-            const uint32_t c[4] = {c4.x, c4.y, c4.z, c4.w};
-            uint32_t t = target0 - excl;
-            int j = 0;
-            while (t >= c[j]) t -= c[j++];
-            s_prefix = (uint32_t)(4 * tid + j) << NOISE_L1_SHIFT;
-            s_target = t;
-            s_binn = c[j];
-        }
-        hist[tid] = 0;
-        if (tid == 0) s_n = 0;
-        __syncthreads();
-    }
-
-    // level 2: bits 19..10 of the samples in the chosen bin, which are also
-    // gathered into LDS when they fit
-    const uint32_t p1 = s_prefix, t1 = s_target;
-    const bool compact = s_binn <= SEL_CAP;
-    constexpr int SEL_B = 8;  // loads in flight per thread
-    for (size_t base = 0; base < count; base += (size_t)SEL_B * SEL_NT) {
-        uint32_t v[SEL_B];
+    h3[tid] = 0;
+    rank_bin<SEL_NT, NOISE_L1_BINS / SEL_NT>(hist1, target0, s_wsum, s_sel);
+    const uint32_t b1 = s_sel[0], t1 = s_sel[1];
+    rank_bin<SEL_NT, 1>(hist2, t1, s_wsum, s_sel);
+    const uint32_t p2 = (b1 << 10) | s_sel[0], t2 = s_sel[1];  // bits 31..10 of the answer
+    // level 3 over the list: the samples of bin b1, in no particular order
+    const uint32_t n = ctrl[0];
+    constexpr int B = 8;  // loads in flight per thread
+    for (uint32_t base = 0; base < n; base += (uint32_t)B * SEL_NT) {
+        uint32_t v[B];
 #pragma unroll
-        for (int q = 0; q < SEL_B; q++) {
-            const size_t k = base + (size_t)q * SEL_NT + tid;
-            v[q] = k < count ? vals[k] : 0u;
+        for (int q = 0; q < B; q++) {
+            const uint32_t i = base + (uint32_t)q * SEL_NT + tid;
+            v[q] = i < n ? list[i] : 0u;
         }
 #pragma unroll
-        for (int q = 0; q < SEL_B; q++) {
-            const bool m = base + (size_t)q * SEL_NT + tid < count && (v[q] >> NOISE_L1_SHIFT) == (p1 >> NOISE_L1_SHIFT);
-            wave_count(hist, (v[q] >> 10) & 1023u, m);
-            if (compact) wave_append(buf, &s_n, SEL_CAP, v[q], m);
-        }
+        for (int q = 0; q < B; q++) wave_count(h3, v[q] & 1023u, base + (uint32_t)q * SEL_NT + tid < n && (v[q] >> 10) == p2);
     }
     __syncthreads();
-    {
-        const uint32_t tot = hist[tid];
-        const uint32_t excl = sel_excl_scan(tot, s_wsum);
-        if (excl <= t1 && t1 < excl + tot) {
-            s_prefix = p1 | ((uint32_t)tid << 10);
-            s_target = t1 - excl;
-        }
-        hist[tid] = 0;
-        __syncthreads();
-    }
-
-    // level 3: bits 9..0 of the samples matching the 22-bit prefix
-    const uint32_t p2 = s_prefix, t2 = s_target;
-    if (compact) {
-        const uint32_t n = s_n < SEL_CAP ? s_n : SEL_CAP;
-        for (uint32_t base = 0; base < n; base += SEL_NT) {
-            const uint32_t i = base + tid;
-            const uint32_t v = i < n ? buf[i] : 0u;
-            wave_count(hist, v & 1023u, i < n && (v >> 10) == (p2 >> 10));
-        }
-    } else {
-        for (size_t base = 0; base < count; base += (size_t)SEL_B * SEL_NT) {
-            uint32_t v[SEL_B];
-#pragma unroll
-            for (int q = 0; q < SEL_B; q++) {
-                const size_t k = base + (size_t)q * SEL_NT + tid;
-                v[q] = k < count ? vals[k] : 0u;
-            }
-#pragma unroll
-            for (int q = 0; q < SEL_B; q++)
-                wave_count(hist, v[q] & 1023u, base + (size_t)q * SEL_NT + tid < count && (v[q] >> 10) == (p2 >> 10));
-        }
-    }
-    __syncthreads();
-    {
-        const uint32_t tot = hist[tid];
-        const uint32_t excl = sel_excl_scan(tot, s_wsum);
-        if (excl <= t2 && t2 < excl + tot) s_prefix = p2 | (uint32_t)tid;
-        __syncthreads();
-    }
-    if (tid == 0) *sigma_out = __uint_as_float(s_prefix) / 0.6745f / sqrt20;
+    rank_bin<SEL_NT, 1>(h3, t2, s_wsum, s_sel);
+    if (tid == 0) *sigma_out = __uint_as_float((p2 << 10) | s_sel[0]) / 0.6745f / sqrt20;
 }
 
 // The guide I = L + (lf - L) m (enhance.rs:86) and the products the guided
@@ -306,7 +310,7 @@ size_t noise_sample_count(size_t w, size_t h) {
     return (n + step - 1) / step;
 }
 
-size_t noise_scratch_u32(size_t w, size_t h) { return NOISE_L1_BINS + noise_sample_count(w, h); }
+size_t noise_scratch_u32(size_t w, size_t h) { return NOISE_HDR + 2 * noise_sample_count(w, h); }
 
 // Samples of enhance.rs:12-22 off the 1-pixel border: for each inner row y,
 // the multiples of step in [y w + 1, y w + w - 2].
@@ -329,15 +333,22 @@ hipError_t launch_estimate_noise(const float* plane, size_t w, size_t h, uint32_
         return hipGetLastError();
     }
     const size_t count = (n + step - 1) / step;
-    // scratch: the first-level histogram, then the samples
-    uint32_t* hist = scratch;
-    uint32_t* vals = scratch + NOISE_L1_BINS;
-    hipLaunchKernelGGL(k_noise_hist_clear, dim3(NOISE_L1_BINS / NS_NT), dim3(NS_NT), 0, st, hist);
+    // scratch: level-1 and level-2 histograms, control words, the list of
+    // samples in the chosen level-1 bin, the samples
+    uint32_t* hist1 = scratch;
+    uint32_t* hist2 = hist1 + NOISE_L1_BINS;
+    uint32_t* ctrl = hist2 + NOISE_L2_BINS;
+    uint32_t* list = scratch + NOISE_HDR;
+    uint32_t* vals = list + count;
+    const uint32_t target = (uint32_t)(valid / 2);
     const size_t per = (size_t)NS_NT * NS_PER;
-    hipLaunchKernelGGL(k_noise_sample, dim3((unsigned)((count + per - 1) / per)), dim3(NS_NT), 0, st, plane, w, h, step, count, vals, hist);
+    const unsigned blocks = (unsigned)((count + per - 1) / per);
+    hipLaunchKernelGGL(k_noise_hist_clear, dim3((NOISE_HDR + NS_NT - 1) / NS_NT), dim3(NS_NT), 0, st, scratch);
+    hipLaunchKernelGGL(k_noise_sample, dim3(blocks), dim3(NS_NT), 0, st, plane, w, h, step, count, vals, hist1);
+    hipLaunchKernelGGL(k_noise_l2, dim3(blocks), dim3(NS_NT), 0, st, vals, count, target, hist1, hist2, ctrl, list);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_noise_select, dim3(1), dim3(SEL_NT), 0, st, hist, vals, count, (uint32_t)(valid / 2), sqrtf(20.0f), sigma_out);
+    hipLaunchKernelGGL(k_noise_final, dim3(1), dim3(SEL_NT), 0, st, hist1, hist2, ctrl, list, target, sqrtf(20.0f), sigma_out);
     return hipGetLastError();
 }
 
