@@ -277,13 +277,11 @@ __global__ void k_gf_coef(const float* __restrict__ mi_, const float* __restrict
 
 // image.rs:404 a*I + b for both planes, then Oklab -> ProPhoto and max(0)
 // (enhance.rs:91)
-__global__ void k_gf_final(const float* __restrict__ L, const float* __restrict__ LF, float m, const float* __restrict__ ma_a,
-                           const float* __restrict__ mb_a, const float* __restrict__ ma_b, const float* __restrict__ mb_b, size_t n, OklabM ok,
-                           float* __restrict__ rgb) {
+__global__ void k_gf_final(const float* __restrict__ I_, const float* __restrict__ ma_a, const float* __restrict__ mb_a,
+                           const float* __restrict__ ma_b, const float* __restrict__ mb_b, size_t n, OklabM ok, float* __restrict__ rgb) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float o = L[i];
-    const float I = o + (LF[i] - o) * m;
+    const float I = I_[i];
     float lab[3] = {I, ma_a[i] * I + mb_a[i], ma_b[i] * I + mb_b[i]};
     float l[3], cube[3], p[3];
     mul_mv(ok.m2_inv, lab, l);
@@ -353,8 +351,9 @@ hipError_t launch_estimate_noise(const float* plane, size_t w, size_t h, uint32_
 }
 
 // Guided filters of a and b by l2 = L + (lf - L) m, then Oklab^-1 into rgb.
-// ws: 12 planes of w*h floats.
-hipError_t launch_guided_pair(const float* L, const float* LF, const float* A, const float* B, float m, size_t r, const float* sigma_dev,
+// ws: 12 planes of w*h floats.  LF is dead once the guide is formed and
+// serves as a spare plane.
+hipError_t launch_guided_pair(const float* L, float* LF, const float* A, const float* B, float m, size_t r, const float* sigma_dev,
                               const OklabM& ok, float* ws, float* rgb, size_t w, size_t h, hipStream_t st) {
     const size_t n = w * h;
     if (n == 0) return hipSuccess;
@@ -371,11 +370,12 @@ hipError_t launch_guided_pair(const float* L, const float* LF, const float* A, c
         return o;
     };
     const unsigned nb = (unsigned)((n + 255) / 256);
-    // the guide I = l2 and its products (image.rs:386-391) -> t0..t3
+    // the guide I = l2 and its products (image.rs:386-391) -> t0..t3; I
+    // stays in t0 for the final step
     hipLaunchKernelGGL(k_gf_inputs, dim3(nb), dim3(256), 0, st, L, LF, A, B, m, n, t[0], t[1], t[2], t[3]);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // means of I, a, b, I a, I b, I I -> t6..t11 (horizontal pass into t4, t5, t8..t11 first)
+    // means of I, a, b, I a, I b, I I (horizontal pass into t4, t5, t8..t11 first)
     CBoxPlanes hin{};
     hin.p[0] = t[0];
     hin.p[1] = A;
@@ -397,21 +397,21 @@ hipError_t launch_guided_pair(const float* L, const float* LF, const float* A, c
     BoxPlanes vout{};
     vout.p[0] = t[6];
     vout.p[1] = t[7];
-    vout.p[2] = t[0];
+    vout.p[2] = LF;
     vout.p[3] = t[1];
     vout.p[4] = t[2];
     vout.p[5] = t[3];
     e = box_v_planes(vin, vout, 6, w, h, r, st);
     if (e != hipSuccess) return e;
-    // means now: I t6, a t7, b t0, I a t1, I b t2, I I t3
+    // means now: I t6, a t7, b LF, I a t1, I b t2, I I t3
     // coefficients (image.rs:395-400) -> t8..t11
-    hipLaunchKernelGGL(k_gf_coef, dim3(nb), dim3(256), 0, st, t[6], t[7], t[0], t[1], t[2], t[3], n, sigma_dev, t[8], t[9], t[10], t[11]);
+    hipLaunchKernelGGL(k_gf_coef, dim3(nb), dim3(256), 0, st, t[6], t[7], LF, t[1], t[2], t[3], n, sigma_dev, t[8], t[9], t[10], t[11]);
     e = hipGetLastError();
     // their means (image.rs:402-403) -> t8..t11 (via t4..t7)
     if (e == hipSuccess) e = box_h_planes(cplanes(8, 4), planes(4, 4), 4, w, h, r, st);
     if (e == hipSuccess) e = box_v_planes(cplanes(4, 4), planes(8, 4), 4, w, h, r, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gf_final, dim3(nb), dim3(256), 0, st, L, LF, m, t[8], t[9], t[10], t[11], n, ok, rgb);
+    hipLaunchKernelGGL(k_gf_final, dim3(nb), dim3(256), 0, st, t[0], t[8], t[9], t[10], t[11], n, ok, rgb);
     return hipGetLastError();
 }
 

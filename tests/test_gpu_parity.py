@@ -222,6 +222,27 @@ def test_denoise_stages(ctx, oracle_mod, amount, S, P):
     assert maxabs(ctx.oklab_to_prophoto(back), O.oklab_to_prophoto(back)) <= TOL
 
 
+@pytest.mark.parametrize("amount,S,P", [(0.0, 10, 3), (37.0, 3, 1), (100.0, 19, 5), (50.0, 22, 3)])
+def test_denoise_is_its_stages(ctx, oracle_mod, amount, S, P):
+    """denoise equals, bit for bit, the GPU's own stages run one by one:
+    Oklab split, noise estimate, NLM (tile kernel and, for search + patch
+    > 24, the direct one), both guided filters on l2, Oklab back, max 0."""
+    rgb = rgb_frame(oracle_mod, 401, 250, 190, iso=8.0)
+    lab = ctx.oklab_from_prophoto(rgb)
+    L, A, B = (np.ascontiguousarray(lab[..., i]) for i in range(3))
+    sigma = np.float32(max(np.float32(ctx.estimate_noise(L)), np.float32(1e-4)))
+    k = np.float32(amount) / np.float32(100)
+    hp = np.float32(sigma * (np.float32(0.6) + np.float32(1.6) * k))
+    lf = ctx.nlm(L, hp, S, P)
+    l2 = (L + (lf - L) * (np.float32(0.5) + np.float32(0.5) * k)).astype(np.float32)
+    r = int(max((np.float32(250) / np.float32(600)) * (np.float32(1) + np.float32(2) * k), np.float32(2)))
+    eps = np.float32(sigma * np.float32(2)) ** 2
+    a2, b2 = ctx.guided_filter(l2, A, r, eps), ctx.guided_filter(l2, B, r, eps)
+    want = np.maximum(ctx.oklab_to_prophoto(np.stack([l2, a2, b2], -1).astype(np.float32)), np.float32(0))
+    got = ctx.denoise(rgb, amount, S, P)
+    assert np.array_equal(got, want), f"bit-equal {np.mean(got == want):.6f}"
+
+
 @pytest.mark.parametrize("amount,S,P", [(0.0, 10, 3), (37.0, 3, 1), (100.0, 3, 1)])
 def test_denoise_composite(ctx, oracle_mod, amount, S, P):
     """The composite can only be held to the reference's own reproducibility:
