@@ -63,19 +63,80 @@ __device__ __forceinline__ void stage_tile(Src src, Dst dst) {
     }
 }
 
-// Tile of a 2-D grid, renumbered so that each of the 8 XCDs (which take
-// workgroups round-robin by linear id, each with its own L2) walks one
-// contiguous run of tiles in row-major order.  Neighbouring tiles then share
-// an L2, so the halos they read twice come from L2 instead of HBM.  A
-// bijection of the grid: XCD x owns tiles [x q + min(x, m), + q + (x < m)),
-// q = n / 8, m = n % 8.
-__device__ __forceinline__ void xcd_tile(int& tx, int& ty) {
-    const int nx = (int)gridDim.x, n = nx * (int)gridDim.y;
-    const int b = (int)blockIdx.y * nx + (int)blockIdx.x;
-    const int x = b & 7, k = b >> 3, q = n >> 3, m = n & 7;
-    const int t = x * q + (x < m ? x : m) + k;
-    tx = t % nx;
-    ty = t / nx;
+// Tile orders of 1-D grids over an nx x ny tile grid whose frame-edge tiles
+// (halo leaving the image: the kernels' clamped paths, about twice the
+// interior path's cost) lie outside the interior rectangle
+// [ix0, ix1) x [iy0, iy1).  Dispatched last, the edge row at the bottom of
+// the frame stretches a kernel's final round of tiles; these orders put the
+// edge tiles first.
+struct TileOrder {
+    int nx, ny, ix0, ix1, iy0, iy1;  // empty interior: all four zero
+};
+// Interior tiles of TW x TH tiles over a w x h image with halos hx, hy:
+// ox >= hx, oy >= hy, ox + TW + hx <= w, oy + TH + hy <= h.
+inline TileOrder tile_order(int w, int h, int TW, int TH, int hx, int hy) {
+    TileOrder o;
+    o.nx = (w + TW - 1) / TW;
+    o.ny = (h + TH - 1) / TH;
+    o.ix0 = (hx + TW - 1) / TW;
+    o.iy0 = (hy + TH - 1) / TH;
+    o.ix1 = w >= TW + hx ? (w - TW - hx) / TW + 1 : 0;
+    o.iy1 = h >= TH + hy ? (h - TH - hy) / TH + 1 : 0;
+    if (o.ix1 <= o.ix0 || o.iy1 <= o.iy0) o.ix0 = o.ix1 = o.iy0 = o.iy1 = 0;
+    return o;
+}
+// The j-th edge tile in row-major order.
+__device__ __forceinline__ void edge_tile(int j, const TileOrder& o, int& tx, int& ty) {
+    const int inx = o.ix1 - o.ix0, iny = o.iy1 - o.iy0;
+    const int top = o.iy0 * o.nx;
+    if (j < top) {
+        tx = j % o.nx;
+        ty = j / o.nx;
+        return;
+    }
+    j -= top;
+    const int m = o.nx - inx;  // edge tiles of a row crossing the interior
+    if (j < iny * m) {
+        const int c = j % m;
+        ty = o.iy0 + j / m;
+        tx = c < o.ix0 ? c : o.ix1 + (c - o.ix0);
+        return;
+    }
+    j -= iny * m;
+    tx = j % o.nx;
+    ty = o.iy1 + j / o.nx;
+}
+// Block b: the edge tiles in row-major order, then the interior ones.
+__device__ __forceinline__ void tile_edge_first(int b, const TileOrder& o, int& tx, int& ty) {
+    const int inx = o.ix1 - o.ix0, iny = o.iy1 - o.iy0;
+    const int ne = o.nx * o.ny - inx * iny;
+    if (b < ne) {
+        edge_tile(b, o, tx, ty);
+        return;
+    }
+    b -= ne;
+    tx = o.ix0 + b % inx;
+    ty = o.iy0 + b / inx;
+}
+// Block b as the 8 XCDs take it (round-robin by linear id, each XCD with
+// its own L2): XCD x first takes every 8th edge tile (x, x + 8, ...), then
+// one contiguous row-major run of interior tiles, so neighbouring interior
+// tiles share an L2 and the halos they read twice come from it rather than
+// HBM.  A bijection of the tile grid.
+__device__ __forceinline__ void tile_edge_first_xcd(int b, const TileOrder& o, int& tx, int& ty) {
+    const int inx = o.ix1 - o.ix0, iny = o.iy1 - o.iy0;
+    const int n = o.nx * o.ny, ne = n - inx * iny;
+    const int x = b & 7, k = b >> 3;
+    const int q = n >> 3, m = n & 7, qe = ne >> 3, me = ne & 7;
+    const int ne_x = qe + (x < me ? 1 : 0);
+    if (k < ne_x) {
+        edge_tile(8 * k + x, o, tx, ty);
+        return;
+    }
+    // interior items of XCDs 0 .. x-1: their blocks minus their edge tiles
+    const int i = x * (q - qe) + (x < m ? x : m) - (x < me ? x : me) + (k - ne_x);
+    tx = o.ix0 + i % inx;
+    ty = o.iy0 + i / inx;
 }
 
 // Runtime-sized variant: batches of KB loads per thread.

@@ -51,40 +51,6 @@ constexpr int NLM_HSR = NLM_HR + 4;  // sum rows incl. padding read by the last 
 
 __host__ __device__ constexpr int nlm_th(int P) { return NLM_HR - 2 * P; }
 
-// Tile order of the 1-D grid: frame-edge tiles (whose halo leaves the image:
-// the clamped offset loop, ~1.75x the interior one's instructions) first,
-// row-major, then the interior rectangle [ix0, ix1) x [iy0, iy1).  Run last,
-// the slow edge row would stretch the final round of tiles.
-struct NlmOrder {
-    int nx, ny, ix0, ix1, iy0, iy1;
-};
-__device__ __forceinline__ void nlm_tile_of(int b, const NlmOrder& o, int& tx, int& ty) {
-    const int inx = o.ix1 - o.ix0, iny = o.iy1 - o.iy0;
-    const int ne = o.nx * o.ny - inx * iny;
-    if (b >= ne) {  // interior
-        b -= ne;
-        tx = o.ix0 + b % inx;
-        ty = o.iy0 + b / inx;
-        return;
-    }
-    const int top = o.iy0 * o.nx;
-    if (b < top) {
-        tx = b % o.nx;
-        ty = b / o.nx;
-        return;
-    }
-    b -= top;
-    const int m = o.nx - inx;  // edge tiles of a middle row
-    if (b < iny * m) {
-        const int j = b % m;
-        ty = o.iy0 + b / m;
-        tx = j < o.ix0 ? j : o.ix1 + (j - o.ix0);
-        return;
-    }
-    b -= iny * m;
-    tx = b % o.nx;
-    ty = o.iy1 + b / o.nx;
-}
 // input tile plus two spare rows read (not stored) by the last row segment
 __host__ __device__ inline int nlm_pt_floats(int SP, int P, int PW) { return ((nlm_th(P) + 2 * SP + 2) * PW + 3) & ~3; }
 
@@ -95,7 +61,7 @@ __host__ __device__ inline int nlm_pt_floats(int SP, int P, int PW) { return ((n
 // and a quarter of the tile rows.
 template <int P, int PW, int HSB>
 __global__ __launch_bounds__(NLM_NT, 4 - HSB) void k_nlm_tile(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S,
-                                                              NlmWeight wp, const uint8_t* __restrict__ regions, NlmOrder order) {
+                                                              NlmWeight wp, const uint8_t* __restrict__ regions, TileOrder order) {
     extern __shared__ float lds[];
     constexpr int TW = NLM_TW, TH = nlm_th(P), NT = NLM_NT, SEG = NLM_SEG, HS = NLM_HS;
     constexpr int DW = SEG + 2 * P;           // differences per phase-1 task
@@ -112,8 +78,10 @@ __global__ __launch_bounds__(NLM_NT, 4 - HSB) void k_nlm_tile(const float* __res
     float* pt = lds;                          // input tile, PH x PW
     float* hs0 = lds + nlm_pt_floats(SP, P, PW);  // horizontal sums, HSB x HSR x HS
     const int tid = threadIdx.x;
+    // frame-edge tiles (the clamped offset loop, ~1.75x the interior one's
+    // instructions) first, so they do not stretch the last round
     int tile_x, tile_y;
-    nlm_tile_of(blockIdx.x, order, tile_x, tile_y);
+    tile_edge_first(blockIdx.x, order, tile_x, tile_y);
     const int ox = tile_x * TW, oy = tile_y * TH;
     // region mode: only tiles that touch a flagged region run, and they store
     // only the pixels of flagged regions
@@ -474,17 +442,8 @@ __global__ __launch_bounds__(64) void k_nlm_spot_final(const float* __restrict__
 template <int P, int PW, int HSB>
 hipError_t launch_tile_pw(const float* in, float* out, int w, int h, int S, NlmWeight wp, const uint8_t* regions, hipStream_t stream) {
     const size_t lds = (size_t)(nlm_pt_floats(S + P, P, PW) + HSB * NLM_HSR * NLM_HS) * sizeof(float);
-    // interior tiles: ox >= SP, oy >= SP, ox + TW + SP <= w, oy + TH + SP <= h
-    constexpr int TH = nlm_th(P);
-    const int SP = S + P;
-    NlmOrder order;
-    order.nx = (w + NLM_TW - 1) / NLM_TW;
-    order.ny = (h + TH - 1) / TH;
-    order.ix0 = (SP + NLM_TW - 1) / NLM_TW;
-    order.iy0 = (SP + TH - 1) / TH;
-    order.ix1 = w >= NLM_TW + SP ? (w - NLM_TW - SP) / NLM_TW + 1 : 0;
-    order.iy1 = h >= TH + SP ? (h - TH - SP) / TH + 1 : 0;
-    if (order.ix1 <= order.ix0 || order.iy1 <= order.iy0) order.ix0 = order.ix1 = order.iy0 = order.iy1 = 0;  // no interior
+    // interior tiles: ox >= S + P, oy >= S + P, ox + TW + S + P <= w, oy + TH + S + P <= h
+    const TileOrder order = tile_order(w, h, NLM_TW, nlm_th(P), S + P, S + P);
     dim3 grid((unsigned)(order.nx * order.ny));
     static bool attr_set = false;
     if (!attr_set) {

@@ -84,7 +84,7 @@ __device__ __forceinline__ void stage_rows(const float* __restrict__ g, int w, f
 
 template <int R, int TY>
 __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est, const float* __restrict__ yp, float* __restrict__ est_out,
-                                                   int w, int h, Taps taps) {
+                                                   int w, int h, Taps taps, TileOrder order) {
     static_assert(TY % 8 == 0, "row blocks of 8");
     constexpr int TX = RL_TX, NT = RL_NT, K = 2 * R + 1, B = 8;
     // est: [-2R, T+2R) both ways; y, ratio: [-R, T+R); tmp: cols [-R, TX+R), rows [-2R, TY+2R)
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est
     float* const s_rat = s_est;
     const int tid = threadIdx.x;
     int tx, ty;
-    xcd_tile(tx, ty);
+    tile_edge_first_xcd(blockIdx.x, order, tx, ty);
     const int ox = tx * TX, oy = ty * TY;
     float kk[K];
 #pragma unroll
@@ -294,8 +294,10 @@ __global__ void k_rl_update(float* __restrict__ est, const float* __restrict__ c
 
 template <int R, int TY>
 hipError_t launch_iter_ty(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, hipStream_t st) {
-    dim3 grid((unsigned)((w + RL_TX - 1) / RL_TX), (unsigned)((h + TY - 1) / TY));
-    hipLaunchKernelGGL((k_rl_iter<R, TY>), grid, dim3(RL_NT), 0, st, est, y, est_out, (int)w, (int)h, t);
+    // interior tiles: ox >= 2R, oy >= 2R, ox + TX + 2R <= w, oy + TY + 2R <= h
+    const TileOrder order = tile_order((int)w, (int)h, RL_TX, TY, 2 * R, 2 * R);
+    dim3 grid((unsigned)(order.nx * order.ny));
+    hipLaunchKernelGGL((k_rl_iter<R, TY>), grid, dim3(RL_NT), 0, st, est, y, est_out, (int)w, (int)h, t, order);
     return hipGetLastError();
 }
 
