@@ -36,8 +36,9 @@ namespace {
 
 constexpr int NLM_TW = 64;        // tile columns
 constexpr int NLM_HR = 64;        // rows of horizontal sums per tile (TH + 2P)
-constexpr int NLM_NT = 256;
-constexpr int NLM_SEG = 16;       // columns per phase-1 task
+constexpr int NLM_NT = 512;
+constexpr int NLM_SEG = 8;        // columns per phase-1 task
+constexpr int NLM_NG = NLM_NT / NLM_TW;  // phase-2 row groups (threads per column)
 constexpr int NLM_HS = NLM_TW + 4;  // padded row stride of the horizontal-sum buffer
 constexpr int NLM_MAX_SP = 24;    // halo limit of the tile kernel
 constexpr int NLM_NARROW_SP = 13; // halo limit of the narrow instance (search 10, patch 3)
@@ -47,27 +48,36 @@ constexpr int NLM_NARROW_SP = 13; // halo limit of the narrow instance (search 1
 // second barrier per offset, fits three tiles per CU instead of two.
 constexpr int NLM_PW_WIDE = NLM_TW + 2 * NLM_MAX_SP + 1;
 constexpr int NLM_PW_NARROW = NLM_TW + 2 * NLM_NARROW_SP + 1;
-constexpr int NLM_HSR = NLM_HR + 4;  // sum rows incl. padding read by the last row segment
-
 __host__ __device__ constexpr int nlm_th(int P) { return NLM_HR - 2 * P; }
-
-// input tile plus two spare rows read (not stored) by the last row segment
-__host__ __device__ inline int nlm_pt_floats(int SP, int P, int PW) { return ((nlm_th(P) + 2 * SP + 2) * PW + 3) & ~3; }
+// phase-2 rows per thread: the tile rows split over NLM_NG row groups
+__host__ __device__ constexpr int nlm_rpt(int P) { return (nlm_th(P) + NLM_NG - 1) / NLM_NG; }
+// sum rows incl. padding read by the last row group (which runs past the
+// tile: NLM_NG * rpt >= TH rows, reading 2P more)
+constexpr int NLM_HSR = NLM_HR + NLM_NG;
+// input tile (TH + 2 SP rows) plus spare rows read (not stored) by the last
+// row group: rows up to NG * rpt + 2 SP - P
+__host__ __device__ inline int nlm_pt_rows(int SP, int P) {
+    const int a = nlm_th(P) + 2 * SP + 2, b = NLM_NG * nlm_rpt(P) + 2 * SP - P + 1;
+    return a > b ? a : b;
+}
+__host__ __device__ inline int nlm_pt_floats(int SP, int P, int PW) { return (nlm_pt_rows(SP, P) * PW + 3) & ~3; }
 
 // Tile kernel.  Phase 1: thread t owns hs row (t % 64) and the column
-// segments t / 64 + 4 k, k < NTASK (TW / (4 SEG) tasks per thread).  The
-// centre samples of its segments do not depend on the offset and stay in
-// registers for the whole offset loop.  Phase 2: thread t owns column t % 64
-// and a quarter of the tile rows.
+// segments t / 64 + NSG k, k < NTASK (TW / (NSG SEG) tasks per thread, NSG =
+// NT / 64).  The centre samples of its segments do not depend on the offset
+// and stay in registers for the whole offset loop.  Phase 2: thread t owns
+// column t % 64 and one of NLM_NG groups of tile rows.
 template <int P, int PW, int HSB>
-__global__ __launch_bounds__(NLM_NT, 4 - HSB) void k_nlm_tile(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S,
+__global__ __launch_bounds__(NLM_NT, HSB == 1 ? 6 : 2) void k_nlm_tile(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S,
                                                               NlmWeight wp, const uint8_t* __restrict__ regions, TileOrder order) {
     extern __shared__ float lds[];
     constexpr int TW = NLM_TW, TH = nlm_th(P), NT = NLM_NT, SEG = NLM_SEG, HS = NLM_HS;
     constexpr int DW = SEG + 2 * P;           // differences per phase-1 task
-    constexpr int NTASK = TW / (SEG * (NT / NLM_HR));
-    static_assert(NTASK * SEG * (NT / NLM_HR) == TW && SEG % 4 == 0, "phase-1 tasks must tile the rows");
-    constexpr int RPT = (TH + 3) / 4;         // phase-2 rows per thread
+    constexpr int NSG = NT / NLM_HR;          // phase-1 segment groups
+    constexpr int NTASK = TW / (SEG * NSG);
+    static_assert(NTASK * SEG * NSG == TW && SEG % 4 == 0, "phase-1 tasks must tile the rows");
+    constexpr int RPT = nlm_rpt(P);           // phase-2 rows per thread
+    static_assert(NLM_NG * RPT + 2 * P <= NLM_HSR, "the last row group's sums are inside the buffer");
     // The tile is staged scaled by cs = sqrt(-scale), so a window sum of
     // squared differences is already the exp2 argument; the accumulated
     // samples carry the same factor, removed once per output.
@@ -109,7 +119,7 @@ __global__ __launch_bounds__(NLM_NT, 4 - HSB) void k_nlm_tile(const float* __res
 
     // phase-1 geometry
     const int v1 = tid % NLM_HR - P;  // tile row of this thread's horizontal sums
-    const int u00 = (tid / NLM_HR) * SEG;  // first segment; task k starts at u00 + 4 k SEG
+    const int u00 = (tid / NLM_HR) * SEG;  // first segment; task k starts at u00 + NSG k SEG
     // window positions (tile coords) of the two segments; at the frame edge the
     // difference plane is evaluated at the clamped position (box_blur's clamp)
     auto cu_of = [&](int u) { return interior ? u : clampi(ox + u, 0, w - 1) - ox; };
@@ -118,7 +128,7 @@ __global__ __launch_bounds__(NLM_NT, 4 - HSB) void k_nlm_tile(const float* __res
 #pragma unroll
     for (int task = 0; task < NTASK; task++)
 #pragma unroll
-        for (int i = 0; i < DW; i++) cc[task][i] = P_at(cu_of(u00 + 4 * SEG * task - P + i), cv);
+        for (int i = 0; i < DW; i++) cc[task][i] = P_at(cu_of(u00 + NSG * SEG * task - P + i), cv);
 
     // phase-2 geometry
     const int col = tid % TW;
@@ -144,7 +154,7 @@ __global__ __launch_bounds__(NLM_NT, 4 - HSB) void k_nlm_tile(const float* __res
 #pragma unroll
             for (int task = 0; task < NTASK; task++) {
                 const float* c = cc[task];
-                const int u0 = u00 + 4 * SEG * task;
+                const int u0 = u00 + NSG * SEG * task;
                 float d[DW];
                 if constexpr (IN) {
                     const float* sp = pt + (cv + SP) * PW + (u0 - P + SP) + doff;
