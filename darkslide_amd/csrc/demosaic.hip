@@ -602,7 +602,7 @@ hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream, int varian
         switch (variant) {
             case 1: return launch_rcd<32, 32, 256>(src, a.rgb, a.clip, stream);
             case 2: return launch_rcd<64, 16, 256>(src, a.rgb, a.clip, stream);
-            default: return launch_rcd<64, 32, 512>(src, a.rgb, a.clip, stream);
+            default: return launch_rcd<64, 32, 1024>(src, a.rgb, a.clip, stream);
         }
     }
     switch (a.algo == DS_ALGO_MHC ? variant : 0) {
