@@ -25,7 +25,7 @@ namespace ds {
 
 namespace {
 
-constexpr int RL_TX = 64, RL_NT = 256;
+constexpr int RL_TX = 64;
 
 struct Taps {
     float k[13];
@@ -82,11 +82,11 @@ __device__ __forceinline__ void stage_rows(const float* __restrict__ g, int w, f
         }
 }
 
-template <int R, int TY>
-__global__ __launch_bounds__(RL_NT) void k_rl_iter(const float* __restrict__ est, const float* __restrict__ yp, float* __restrict__ est_out,
+template <int R, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, const float* __restrict__ yp, float* __restrict__ est_out,
                                                    int w, int h, Taps taps, TileOrder order) {
     static_assert(TY % 8 == 0, "row blocks of 8");
-    constexpr int TX = RL_TX, NT = RL_NT, K = 2 * R + 1, B = 8;
+    constexpr int TX = RL_TX, K = 2 * R + 1, B = 8;
     // est: [-2R, T+2R) both ways; y, ratio: [-R, T+R); tmp: cols [-R, TX+R), rows [-2R, TY+2R)
     constexpr int EW = TX + 4 * R, EH = TY + 4 * R;
     constexpr int YW = TX + 2 * R, YH = TY + 2 * R;
@@ -292,23 +292,25 @@ __global__ void k_rl_update(float* __restrict__ est, const float* __restrict__ c
     if (i < n) est[i] = clampr(est[i] * corr[i], 0.0f, 64.0f);
 }
 
-template <int R, int TY>
+template <int R, int TY, int NT>
 hipError_t launch_iter_ty(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, hipStream_t st) {
     // interior tiles: ox >= 2R, oy >= 2R, ox + TX + 2R <= w, oy + TY + 2R <= h
     const TileOrder order = tile_order((int)w, (int)h, RL_TX, TY, 2 * R, 2 * R);
     dim3 grid((unsigned)(order.nx * order.ny));
-    hipLaunchKernelGGL((k_rl_iter<R, TY>), grid, dim3(RL_NT), 0, st, est, y, est_out, (int)w, (int)h, t, order);
+    hipLaunchKernelGGL((k_rl_iter<R, TY, NT>), grid, dim3(NT), 0, st, est, y, est_out, (int)w, (int)h, t, order);
     return hipGetLastError();
 }
 
 // Tile-shape variants of the iteration kernel (64 x TY, same results):
-// 0 = 64x32, 1 = 64x16, 2 = 64x64.
+// 0 = 64x64 in 512 threads (45 KB, three tiles and 6 waves per SIMD per CU;
+// 8 % less halo work than 64x32), 1 = 64x16, 2 = 64x32 (26 KB, six tiles of
+// 256 threads per CU).
 template <int R>
 hipError_t launch_iter(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, int variant, hipStream_t st) {
     switch (variant) {
-        case 1: return launch_iter_ty<R, 16>(est, y, est_out, w, h, t, st);
-        case 2: return launch_iter_ty<R, 64>(est, y, est_out, w, h, t, st);
-        default: return launch_iter_ty<R, 32>(est, y, est_out, w, h, t, st);
+        case 1: return launch_iter_ty<R, 16, 256>(est, y, est_out, w, h, t, st);
+        case 2: return launch_iter_ty<R, 32, 256>(est, y, est_out, w, h, t, st);
+        default: return launch_iter_ty<R, 64, 512>(est, y, est_out, w, h, t, st);
     }
 }
 
