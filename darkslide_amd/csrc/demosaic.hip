@@ -608,7 +608,7 @@ hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream, int varian
     switch (a.algo == DS_ALGO_MHC ? variant : 0) {
         case 1: return launch_mhc<128, 8, 256>(src, a.rgb, a.clip, stream);
         case 2: return launch_mhc<32, 32, 256>(src, a.rgb, a.clip, stream);
-        default: return launch_mhc<64, 16, 256>(src, a.rgb, a.clip, stream);
+        default: return launch_mhc<128, 32, 1024>(src, a.rgb, a.clip, stream);
     }
 }
 
