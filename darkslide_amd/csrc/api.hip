@@ -296,10 +296,10 @@ void* slot(ds_ctx* c, Slot s, size_t bytes, hipError_t* err) {
         // captured batches run on the root stream and may hold this buffer:
         // wait for them and retire every graph captured before the change
         ds_ctx* root = c->parent ? c->parent : c;
-        hipStreamSynchronize(root->stream);
+        if ((*err = hipStreamSynchronize(root->stream)) != hipSuccess) return nullptr;
         root->slot_gen++;
-        hipStreamSynchronize(c->stream);
-        hipFree(b.p);
+        if ((*err = hipStreamSynchronize(c->stream)) != hipSuccess) return nullptr;
+        (void)hipFree(b.p);
         b.p = nullptr;
         b.bytes = 0;
     }
@@ -325,15 +325,16 @@ void* slot(ds_ctx* c, Slot s, size_t bytes, hipError_t* err) {
 template <class F>
 int run(ds_ctx* c, const char* name, F&& f) {
     hipEvent_t a = nullptr, b = nullptr;
-    if (c->prof) {
-        hipEventCreate(&a);
-        hipEventCreate(&b);
-        hipEventRecord(a, c->stream);
-    }
+    // a launch whose events cannot be made or recorded is not timed
+    bool timed = c->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess &&
+                 hipEventRecord(a, c->stream) == hipSuccess;
     hipError_t e = f();
-    if (c->prof) {
-        hipEventRecord(b, c->stream);
+    if (timed) timed = hipEventRecord(b, c->stream) == hipSuccess;
+    if (timed) {
         c->pending.push_back({name, a, b});
+    } else {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
     }
     if (e != hipSuccess) return fail(c, DS_ERR_LAUNCH, "%s: %s", name, hipGetErrorString(e));
     return DS_OK;
@@ -341,15 +342,16 @@ int run(ds_ctx* c, const char* name, F&& f) {
 
 void collect(ds_ctx* c) {
     if (c->pending.empty()) return;
-    hipStreamSynchronize(c->stream);
+    const bool synced = hipStreamSynchronize(c->stream) == hipSuccess;
     for (auto& p : c->pending) {
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, p.a, p.b);
-        auto& t = c->totals[p.name];
-        t.first += ms;
-        t.second += 1;
-        hipEventDestroy(p.a);
-        hipEventDestroy(p.b);
+        if (synced && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto& t = c->totals[p.name];
+            t.first += ms;
+            t.second += 1;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
     }
     c->pending.clear();
 }
@@ -667,8 +669,8 @@ int cnn_ensure_train(ds_cnn* net, int n, int H, int W) {
     const size_t fl = tr_floats(npx, n, H, W);
     if (net->tr_px >= npx && net->tr_fcap >= fl) return DS_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    if (net->tr_act) hipFree(net->tr_act);
-    if (net->tr_f) hipFree(net->tr_f);
+    if (net->tr_act) (void)hipFree(net->tr_act);
+    if (net->tr_f) (void)hipFree(net->tr_f);
     net->tr_act = nullptr;
     net->tr_f = nullptr;
     const size_t px = std::max(npx, net->tr_px), cap = std::max(fl, net->tr_fcap);
@@ -884,7 +886,7 @@ float time_ms(ds_ctx* c, int reps, F&& f) {
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess) return -1.0f;
     if (hipEventCreate(&b) != hipSuccess) {
-        hipEventDestroy(a);
+        (void)hipEventDestroy(a);
         return -1.0f;
     }
     std::vector<float> t;
@@ -896,8 +898,8 @@ float time_ms(ds_ctx* c, int reps, F&& f) {
         if (ok) ok = hipEventElapsedTime(&ms, a, b) == hipSuccess;
         t.push_back(ms);
     }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     if (!ok) return -1.0f;
     std::sort(t.begin(), t.end());
     return t[t.size() / 2];
@@ -921,11 +923,11 @@ int autotune_shape(ds_ctx* c, size_t w, size_t h, ds_tuning* out) {
     if (e == hipSuccess) e = hipMalloc(&out_rgb, n * 12);
     if (e == hipSuccess) e = hipMalloc(&ws, rl_workspace_floats(w, h) * 4);
     auto release = [&] {
-        hipStreamSynchronize(c->stream);
-        if (raw) hipFree(raw);
-        if (rgb) hipFree(rgb);
-        if (out_rgb) hipFree(out_rgb);
-        if (ws) hipFree(ws);
+        (void)hipStreamSynchronize(c->stream);
+        if (raw) (void)hipFree(raw);
+        if (rgb) (void)hipFree(rgb);
+        if (out_rgb) (void)hipFree(out_rgb);
+        if (ws) (void)hipFree(ws);
     };
     if (e != hipSuccess) {
         release();
@@ -1072,8 +1074,8 @@ int ensure_branches(ds_ctx* c, int k) {
 }
 
 void destroy_graph(GraphEntry& g) {
-    if (g.exec) hipGraphExecDestroy(g.exec);
-    if (g.graph) hipGraphDestroy(g.graph);
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
     g.exec = nullptr;
     g.graph = nullptr;
 }
@@ -1266,7 +1268,7 @@ int capture_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline
     for (auto& e : ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     auto cleanup = [&] {
         for (auto& e : ev)
-            if (e) hipEventDestroy(e);
+            if (e) (void)hipEventDestroy(e);
         c->capturing = false;
         for (ds_ctx* b : c->branches) b->capturing = false;
     };
@@ -1304,14 +1306,14 @@ int capture_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline
     c->prof = prof;
     cleanup();
     if (rc) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc;
     }
     if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
     hipGraphExec_t exec = nullptr;
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         return hip_fail(c, e, "hipGraphInstantiate");
     }
     g.graph = graph;
@@ -1528,27 +1530,27 @@ void ds_ctx_destroy(ds_ctx* c) {
     if (!c) return;
     {
         std::lock_guard<std::recursive_mutex> g(c->mu);
-        hipSetDevice(c->device);
-        hipStreamSynchronize(c->stream);
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
         for (auto& p : c->pending) {
-            hipEventDestroy(p.a);
-            hipEventDestroy(p.b);
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
         }
         for (auto& g : c->graphs) destroy_graph(g);
         c->graphs.clear();
         for (ds_ctx* b : c->branches) {
-            hipStreamSynchronize(b->stream);
+            (void)hipStreamSynchronize(b->stream);
             for (auto& sb : b->slots)
-                if (sb.p) hipFree(sb.p);
-            hipStreamDestroy(b->stream);
+                if (sb.p) (void)hipFree(sb.p);
+            (void)hipStreamDestroy(b->stream);
             delete b;
         }
         c->branches.clear();
         for (auto& b : c->slots)
-            if (b.p) hipFree(b.p);
-        for (void* p : c->user_allocs) hipFree(p);
-        for (void* p : c->host_allocs) hipHostFree(p);
-        hipStreamDestroy(c->stream);
+            if (b.p) (void)hipFree(b.p);
+        for (void* p : c->user_allocs) (void)hipFree(p);
+        for (void* p : c->host_allocs) (void)hipHostFree(p);
+        (void)hipStreamDestroy(c->stream);
     }
     delete c;
 }
@@ -2217,7 +2219,7 @@ int ds_cnn_create(ds_ctx* c, uint64_t seed, ds_cnn** out) {
     hipError_t e = hipMalloc(&net->state, 4 * CNN_NPARAM * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&net->wpk, CNN_WPK * sizeof(uint16_t));
     if (e != hipSuccess) {
-        if (net->state) hipFree(net->state);
+        if (net->state) (void)hipFree(net->state);
         delete net;
         return hip_fail(c, e, "hipMalloc (CNN state)");
     }
@@ -2228,8 +2230,8 @@ int ds_cnn_create(ds_ctx* c, uint64_t seed, ds_cnn** out) {
     if (e == hipSuccess) e = hipMemcpyAsync(net->state + CNN_NPARAM, zero.data(), 3 * CNN_NPARAM * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
-        hipFree(net->state);
-        hipFree(net->wpk);
+        (void)hipFree(net->state);
+        (void)hipFree(net->wpk);
         delete net;
         return hip_fail(c, e, "CNN initialisation");
     }
@@ -2242,12 +2244,12 @@ void ds_cnn_destroy(ds_cnn* net) {
     ds_ctx* c = net->ctx;
     {
         std::lock_guard<std::recursive_mutex> g(c->mu);
-        hipSetDevice(c->device);
-        hipStreamSynchronize(c->stream);
-        hipFree(net->state);
-        hipFree(net->wpk);
-        if (net->tr_act) hipFree(net->tr_act);
-        if (net->tr_f) hipFree(net->tr_f);
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(net->state);
+        (void)hipFree(net->wpk);
+        if (net->tr_act) (void)hipFree(net->tr_act);
+        if (net->tr_f) (void)hipFree(net->tr_f);
     }
     delete net;
 }
