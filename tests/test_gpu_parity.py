@@ -447,6 +447,28 @@ def test_full_frame_nlm_and_rl(ctx, oracle_mod, frame24):
     assert abs(float(y_out.mean()) / float(y_in.mean()) - 1) < 2e-3
 
 
+def test_full_frame_denoise_is_its_stages(ctx, frame24):
+    """At 6000x4000 the guided pair's six- and four-plane box passes run in
+    48-row / 48-column bands, the single guided filter's two- and four-plane
+    passes in other bands; the running sums, and so denoise, must not depend
+    on the band: bit-equal to the GPU's stages run one by one."""
+    h, w = frame24.shape
+    rgb = ctx.demosaic_u16(frame24, clip=False)
+    lab = ctx.oklab_from_prophoto(rgb)
+    L, A, B = (np.ascontiguousarray(lab[..., i]) for i in range(3))
+    sigma = np.float32(max(np.float32(ctx.estimate_noise(L)), np.float32(1e-4)))
+    k = np.float32(0)
+    hp = np.float32(sigma * (np.float32(0.6) + np.float32(1.6) * k))
+    lf = ctx.nlm(L, hp, 10, 3)
+    l2 = (L + (lf - L) * (np.float32(0.5) + np.float32(0.5) * k)).astype(np.float32)
+    r = int(max((np.float32(w) / np.float32(600)) * (np.float32(1) + np.float32(2) * k), np.float32(2)))
+    eps = np.float32(sigma * np.float32(2)) ** 2
+    a2, b2 = ctx.guided_filter(l2, A, r, eps), ctx.guided_filter(l2, B, r, eps)
+    want = np.maximum(ctx.oklab_to_prophoto(np.stack([l2, a2, b2], -1).astype(np.float32)), np.float32(0))
+    got = ctx.denoise(rgb, 0.0, 10, 3)
+    assert np.array_equal(got, want), f"bit-equal {np.mean(got == want):.7f}"
+
+
 def test_pinned_host_buffers_round_trip(ctx):
     """ds_host_alloc: a frame through pinned host memory and back, unchanged."""
     import ctypes
