@@ -160,6 +160,10 @@ _SIGS = {
     "ds_nlm": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _P],
     "ds_denoise": [_P, _P, _SZ, _SZ, _F, _P],
     "ds_denoise_ex": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _P],
+    "ds_nlm_reference_order": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _I, _P],
+    "ds_denoise_reference_order": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _I, _P],
+    "ds_nlm_reference_order_dev": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _I, _P],
+    "ds_denoise_reference_order_dev": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _I, _P],
     "ds_raw_details": [_P, _P, _SZ, _SZ, _F, _SZ, _P],
     "ds_box_blur": [_P, _P, _SZ, _SZ, _SZ, _P],
     "ds_gaussian_blur": [_P, _P, _SZ, _SZ, _F, _P],
@@ -167,6 +171,7 @@ _SIGS = {
     "ds_oklab_from_prophoto": [_P, _P, _SZ, _P],
     "ds_oklab_to_prophoto": [_P, _P, _SZ, _P],
     "ds_luminance": [_P, _P, _SZ, _P],
+    "ds_libm_hash": [_P, _I, ctypes.POINTER(ctypes.c_uint64)],
     "ds_dev_alloc": [_P, _SZ, ctypes.POINTER(_P)],
     "ds_dev_free": [_P, _P],
     "ds_host_alloc": [_P, _SZ, ctypes.POINTER(_P)],
@@ -231,6 +236,44 @@ def lib():
 
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _plane(a, name, hw=None, dtype=np.float32):
+    """a as a C-contiguous (h, w) array; with hw, its shape must be hw."""
+    a = np.ascontiguousarray(a, dtype)
+    if a.ndim != 2:
+        raise ValueError(f"{name} must be an (h, w) array, got shape {a.shape}")
+    if hw is not None and a.shape != tuple(hw):
+        raise ValueError(f"{name} must have shape {tuple(hw)}, got {a.shape}")
+    return a
+
+
+def _rgb(a, name, hw=None, copy=False):
+    """a as a C-contiguous (h, w, 3) float32 array (a fresh copy with copy=True)."""
+    a = np.array(a, np.float32, order="C", copy=True) if copy else np.ascontiguousarray(a, np.float32)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError(f"{name} must be an (h, w, 3) array, got shape {a.shape}")
+    if hw is not None and a.shape[:2] != tuple(hw):
+        raise ValueError(f"{name} must have shape {tuple(hw) + (3,)}, got {a.shape}")
+    return a
+
+
+def _triples(a, name):
+    """a as a C-contiguous float32 array of interleaved triples (last axis 3)."""
+    a = np.ascontiguousarray(a, np.float32)
+    if a.ndim < 1 or a.shape[-1] != 3:
+        raise ValueError(f"{name} must have a last axis of 3, got shape {a.shape}")
+    return a
+
+
+def _batch(a, name, shape=None):
+    """a as a C-contiguous (n, h, w) float32 batch; with shape, its shape must be shape."""
+    a = np.ascontiguousarray(a, np.float32)
+    if a.ndim != 3:
+        raise ValueError(f"{name} must be an (n, h, w) array, got shape {a.shape}")
+    if shape is not None and a.shape != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {a.shape}")
+    return a
 
 
 def _pat(pattern):
@@ -358,7 +401,7 @@ class Context:
 
     # ---- stage functions (host arrays in, host arrays out)
     def demosaic(self, cfa, pattern=RGGB, mul=(1.0, 1.0, 1.0), algo=ALGO_RCD, clip=True):
-        cfa = np.ascontiguousarray(cfa, np.float32)
+        cfa = _plane(cfa, "cfa")
         h, w = cfa.shape
         rgb = np.empty((h, w, 3), np.float32)
         cl = np.empty((h, w), np.float32) if clip else None
@@ -366,7 +409,7 @@ class Context:
         return (rgb, cl) if clip else rgb
 
     def demosaic_u16(self, raw, pattern=RGGB, black=512.0, white=(16383.0,) * 3, mul=(1.0, 1.0, 1.0), algo=ALGO_RCD, clip=True):
-        raw = np.ascontiguousarray(raw, np.uint16)
+        raw = _plane(raw, "raw", dtype=np.uint16)
         h, w = raw.shape
         rgb = np.empty((h, w, 3), np.float32)
         cl = np.empty((h, w), np.float32) if clip else None
@@ -376,7 +419,7 @@ class Context:
 
     @staticmethod
     def _raw_cfa(raw, crop, pattern, black, white, black_dims):
-        raw = np.ascontiguousarray(raw, np.uint16)
+        raw = _plane(raw, "raw", dtype=np.uint16)
         H, W = raw.shape
         cx, cy, cw, ch = crop if crop is not None else (0, 0, W, H)
         b = np.ascontiguousarray(np.asarray(black, np.float32).ravel())
@@ -412,7 +455,7 @@ class Context:
 
     def superpixel(self, cfa, pattern=RGGB, mul=(1.0, 1.0, 1.0)):
         """demosaic.rs:44-87 on a normalised CFA -> (rgb (h/2, w/2, 3), clip (h/2, w/2))."""
-        cfa = np.ascontiguousarray(cfa, np.float32)
+        cfa = _plane(cfa, "cfa")
         h, w = cfa.shape
         rgb = np.empty((h // 2, w // 2, 3), np.float32)
         cl = np.empty((h // 2, w // 2), np.float32)
@@ -422,44 +465,44 @@ class Context:
     def noise_reduction(self, rgb, nr, field=None, scale=1.0):
         """pipeline/detail.rs:13 noise_reduction -> new (h, w, 3) array; nr = (luminance, luminance_detail,
         luminance_contrast, color, color_detail, color_smoothness)."""
-        out = np.array(rgb, np.float32, order="C", copy=True)
+        out = _rgb(rgb, "rgb", copy=True)
         h, w, _ = out.shape
-        f = None if field is None else np.ascontiguousarray(field, np.float32)
+        f = None if field is None else _plane(field, "field", (h, w))
         self._check(lib().ds_noise_reduction(self.handle, _ptr(out), w, h, ctypes.byref(NrParams(*nr)), _ptr(f), float(scale)))
         return out
 
     def sharpen(self, rgb, sh, field=None, scale=1.0):
         """pipeline/detail.rs:96 sharpen -> new array; sh = (amount, radius, detail, masking)."""
-        out = np.array(rgb, np.float32, order="C", copy=True)
+        out = _rgb(rgb, "rgb", copy=True)
         h, w, _ = out.shape
-        f = None if field is None else np.ascontiguousarray(field, np.float32)
+        f = None if field is None else _plane(field, "field", (h, w))
         self._check(lib().ds_sharpen(self.handle, _ptr(out), w, h, ctypes.byref(SharpenParams(*sh)), _ptr(f), float(scale)))
         return out
 
     def super_res_refine(self, rgb):
         """enhance.rs:122 super_res_refine -> new array."""
-        out = np.array(rgb, np.float32, order="C", copy=True)
+        out = _rgb(rgb, "rgb", copy=True)
         h, w, _ = out.shape
         self._check(lib().ds_super_res_refine(self.handle, _ptr(out), w, h))
         return out
 
     def output_sharpen(self, rgb, mode, amount):
         """export.rs:199 output_sharpen -> new array; mode SHARPEN_FOR_*, amount SHARPEN_AMOUNT_*."""
-        out = np.array(rgb, np.float32, order="C", copy=True)
+        out = _rgb(rgb, "rgb", copy=True)
         h, w, _ = out.shape
         self._check(lib().ds_output_sharpen(self.handle, _ptr(out), w, h, int(mode), int(amount)))
         return out
 
     def fast_guided_filter(self, I, p, r, eps, sub):
-        I = np.ascontiguousarray(I, np.float32)
-        p = np.ascontiguousarray(p, np.float32)
+        I = _plane(I, "I")
         h, w = I.shape
+        p = _plane(p, "p", (h, w))
         out = np.empty_like(I)
         self._check(lib().ds_fast_guided_filter(self.handle, _ptr(I), _ptr(p), w, h, int(r), float(eps), int(sub), _ptr(out)))
         return out
 
     def plane_resize(self, p, w2, h2):
-        p = np.ascontiguousarray(p, np.float32)
+        p = _plane(p, "p")
         h, w = p.shape
         out = np.empty((h2, w2), np.float32)
         self._check(lib().ds_plane_resize(self.handle, _ptr(p), w, h, int(w2), int(h2), _ptr(out)))
@@ -467,7 +510,7 @@ class Context:
 
     def cfa_clip_levels(self, cfa, pattern=RGGB):
         """demosaic.rs:419 cfa_clip_levels on a normalised CFA -> [3] float32."""
-        cfa = np.ascontiguousarray(cfa, np.float32)
+        cfa = _plane(cfa, "cfa")
         h, w = cfa.shape
         out = np.zeros(3, np.float32)
         self._check(lib().ds_cfa_clip_levels_f32(self.handle, _ptr(cfa), w, h, _pat(pattern), _ptr(out)))
@@ -475,7 +518,7 @@ class Context:
 
     def cfa_clip_levels_u16(self, raw, pattern=RGGB, black=512.0, white=(16383.0,) * 3):
         """cfa_clip_levels of raw DN normalised as decode.rs:291-306 -> [3] float32."""
-        raw = np.ascontiguousarray(raw, np.uint16)
+        raw = _plane(raw, "raw", dtype=np.uint16)
         h, w = raw.shape
         out = np.zeros(3, np.float32)
         self._check(lib().ds_cfa_clip_levels_u16(self.handle, _ptr(raw), w, h, w, _pat(pattern), float(black), _f3(white), _ptr(out)))
@@ -483,77 +526,99 @@ class Context:
 
     def reconstruct_highlights(self, rgb, clip, mul, raw_clip):
         """demosaic.rs:435 reconstruct_highlights -> new (h, w, 3) array (the input is not changed)."""
-        out = np.array(rgb, np.float32, order="C", copy=True)
-        clip = np.ascontiguousarray(clip, np.float32)
+        out = _rgb(rgb, "rgb", copy=True)
         h, w, _ = out.shape
-        if clip.shape != (h, w):
-            raise ValueError("clip plane shape must be (h, w)")
+        clip = _plane(clip, "clip", (h, w))
         self._check(lib().ds_reconstruct_highlights(self.handle, _ptr(out), _ptr(clip), w, h, _f3(mul), _f3(raw_clip)))
         return out
 
     def estimate_noise(self, plane):
-        plane = np.ascontiguousarray(plane, np.float32)
+        plane = _plane(plane, "plane")
         h, w = plane.shape
         s = np.zeros(1, np.float32)
         self._check(lib().ds_estimate_noise(self.handle, _ptr(plane), w, h, _ptr(s)))
         return float(s[0])
 
     def nlm(self, plane, h_param, search, patch):
-        plane = np.ascontiguousarray(plane, np.float32)
+        plane = _plane(plane, "plane")
         h, w = plane.shape
         out = np.empty_like(plane)
         self._check(lib().ds_nlm(self.handle, _ptr(plane), w, h, float(h_param), int(search), int(patch), _ptr(out)))
         return out
 
     def denoise(self, rgb, amount, search=3, patch=1):
-        rgb = np.ascontiguousarray(rgb, np.float32)
+        rgb = _rgb(rgb, "rgb")
         h, w, _ = rgb.shape
         out = np.empty_like(rgb)
         self._check(lib().ds_denoise_ex(self.handle, _ptr(rgb), w, h, float(amount), int(search), int(patch), _ptr(out)))
         return out
 
+    def nlm_reference_order(self, plane, h_param, search, patch, threads):
+        """nlm in the reference's summation order for a rayon pool of `threads` threads (bit for bit)."""
+        plane = _plane(plane, "plane")
+        h, w = plane.shape
+        out = np.empty_like(plane)
+        self._check(lib().ds_nlm_reference_order(self.handle, _ptr(plane), w, h, float(h_param), int(search), int(patch), int(threads),
+                                                 _ptr(out)))
+        return out
+
+    def denoise_reference_order(self, rgb, amount, search, patch, threads):
+        """denoise with its NLM in the reference's summation order for `threads` threads (bit for bit)."""
+        rgb = _rgb(rgb, "rgb")
+        h, w, _ = rgb.shape
+        out = np.empty_like(rgb)
+        self._check(lib().ds_denoise_reference_order(self.handle, _ptr(rgb), w, h, float(amount), int(search), int(patch), int(threads),
+                                                     _ptr(out)))
+        return out
+
     def box_blur(self, plane, r):
-        plane = np.ascontiguousarray(plane, np.float32)
+        plane = _plane(plane, "plane")
         h, w = plane.shape
         out = np.empty_like(plane)
         self._check(lib().ds_box_blur(self.handle, _ptr(plane), w, h, int(r), _ptr(out)))
         return out
 
     def gaussian_blur(self, plane, sigma):
-        plane = np.ascontiguousarray(plane, np.float32)
+        plane = _plane(plane, "plane")
         h, w = plane.shape
         out = np.empty_like(plane)
         self._check(lib().ds_gaussian_blur(self.handle, _ptr(plane), w, h, float(sigma), _ptr(out)))
         return out
 
     def guided_filter(self, guide, src, r, eps):
-        guide = np.ascontiguousarray(guide, np.float32)
-        src = np.ascontiguousarray(src, np.float32)
+        guide = _plane(guide, "guide")
         h, w = guide.shape
+        src = _plane(src, "src", (h, w))
         out = np.empty_like(guide)
         self._check(lib().ds_guided_filter(self.handle, _ptr(guide), _ptr(src), w, h, int(r), float(eps), _ptr(out)))
         return out
 
     def oklab_from_prophoto(self, rgb):
-        rgb = np.ascontiguousarray(rgb, np.float32)
+        rgb = _triples(rgb, "rgb")
         out = np.empty_like(rgb)
         self._check(lib().ds_oklab_from_prophoto(self.handle, _ptr(rgb), rgb.size // 3, _ptr(out)))
         return out
 
     def oklab_to_prophoto(self, lab):
-        lab = np.ascontiguousarray(lab, np.float32)
+        lab = _triples(lab, "lab")
         out = np.empty_like(lab)
         self._check(lib().ds_oklab_to_prophoto(self.handle, _ptr(lab), lab.size // 3, _ptr(out)))
         return out
 
+    def libm_hash(self, fn):
+        """ds_libm_hash: order-free hash of ds_cbrtf (fn 0) or ds_expf (fn 1) over all 2^32 inputs, on the GPU."""
+        v = ctypes.c_uint64()
+        self._check(lib().ds_libm_hash(self.handle, int(fn), ctypes.byref(v)))
+        return int(v.value)
+
     def luminance(self, rgb):
-        rgb = np.ascontiguousarray(rgb, np.float32)
+        rgb = _triples(rgb, "rgb")
         out = np.empty(rgb.shape[:-1], np.float32)
         self._check(lib().ds_luminance(self.handle, _ptr(rgb), rgb.size // 3, _ptr(out)))
         return out
 
     def raw_details(self, rgb, sigma, iterations):
-        rgb = np.ascontiguousarray(rgb, np.float32)
+        rgb = _rgb(rgb, "rgb")
         h, w, _ = rgb.shape
         out = np.empty_like(rgb)
         self._check(lib().ds_raw_details(self.handle, _ptr(rgb), w, h, float(sigma), int(iterations), _ptr(out)))
@@ -588,7 +653,7 @@ class Context:
 
     def auto_wb_stats(self, rgb, as_shot_neutral):
         """pipeline/mod.rs:421-441 statistics in FP64 -> dict(acc, n, neutral, valid)."""
-        rgb = np.ascontiguousarray(rgb, np.float32)
+        rgb = _rgb(rgb, "rgb")
         h, w, _ = rgb.shape
         st = WbStats()
         self._check(lib().ds_auto_wb_stats(self.handle, _ptr(rgb), w, h, _f3(as_shot_neutral), ctypes.byref(st)))
@@ -596,7 +661,7 @@ class Context:
 
     def denoise_learned(self, net, rgb, amount, search=10, patch=3, threshold_db=0.3):
         """denoise with the learned luminance and the per-region NLM fallback -> (rgb, stats)."""
-        rgb = np.ascontiguousarray(rgb, np.float32)
+        rgb = _rgb(rgb, "rgb")
         h, w, _ = rgb.shape
         out = np.empty_like(rgb)
         st = LearnedStats()
@@ -665,7 +730,7 @@ class CNN:
         self.ctx._check(lib().ds_cnn_load(self.handle, os.fsencode(path)))
 
     def forward(self, x):
-        x = np.ascontiguousarray(x, np.float32)
+        x = _batch(x, "x")
         n, h, w = x.shape
         y = np.empty_like(x)
         self.ctx._check(lib().ds_cnn_forward(self.handle, _ptr(x), n, h, w, _ptr(y)))
@@ -673,15 +738,15 @@ class CNN:
 
     def infer(self, x):
         """Fused inference forward (same result as forward)."""
-        x = np.ascontiguousarray(x, np.float32)
+        x = _batch(x, "x")
         n, h, w = x.shape
         y = np.empty_like(x)
         self.ctx._check(lib().ds_cnn_infer(self.handle, _ptr(x), n, h, w, _ptr(y)))
         return y
 
     def loss_grad(self, x, t, margin):
-        x = np.ascontiguousarray(x, np.float32)
-        t = np.ascontiguousarray(t, np.float32)
+        x = _batch(x, "x")
+        t = _batch(t, "t", x.shape)
         n, h, w = x.shape
         g = np.empty(self.param_count(), np.float32)
         loss = ctypes.c_double()
@@ -695,7 +760,7 @@ class CNN:
         return loss.value if want_loss else None
 
     def denoise_plane(self, L):
-        L = np.ascontiguousarray(L, np.float32)
+        L = _plane(L, "L")
         h, w = L.shape
         out = np.empty_like(L)
         self.ctx._check(lib().ds_cnn_denoise_plane(self.handle, _ptr(L), w, h, _ptr(out)))

@@ -67,5 +67,17 @@ def build_mirror_smoke():
     return out
 
 
+def build_libm_exhaustive():
+    """Builds tests/cpp/libm_exhaustive: the host glibc against the kernels'
+    libm restatements (csrc/libm_ref.h compiled for the CPU), no GPU code."""
+    src = os.path.join(ROOT, "tests", "cpp", "libm_exhaustive.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "libm_exhaustive")
+    cmd = ["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-I" + CSRC, src, "-o", out, "-pthread", "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("libm_exhaustive build failed:\n" + r.stderr)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

@@ -20,6 +20,7 @@
 #include "detail.h"
 #include "highlights.h"
 #include "nlm.h"
+#include "nlm_ref.h"
 #include "box.h"
 #include "rl.h"
 #include "stages.h"
@@ -205,7 +206,7 @@ struct Buf {
 
 enum Slot {
     S_IN = 0, S_IN2, S_OUT, S_CLIP, S_WS, S_L, S_A, S_B, S_LF, S_SIGMA, S_NOISE, S_SCENE, S_X, S_T, S_REGIONS, S_ACT,
-    S_PIPE_RGB, S_PIPE_DEN, S_COUNTS, S_STATS, S_HL, S_PIPE_CLIP, S_PIPE_RAW, S_PIPE_OUT, S_COUNT
+    S_PIPE_RGB, S_PIPE_DEN, S_COUNTS, S_STATS, S_HL, S_PIPE_CLIP, S_PIPE_RAW, S_PIPE_OUT, S_NLMREF, S_COUNT
 };
 
 // S_HL: clip levels (3 floats) at 0, cfa_clip_levels partials from HL_PART_OFF.
@@ -577,7 +578,22 @@ int nlm_dev(ds_ctx* c, const float* in, size_t w, size_t h, NlmH hp, long search
     return run(c, "nlm", [&] { return launch_nlm(in, out, w, h, hp, search, patch, c->stream); });
 }
 
-int denoise_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* out) {
+// NLM in the reference's summation order for a pool of `threads` threads.
+int nlm_ref_dev(ds_ctx* c, const float* in, size_t w, size_t h, NlmH hp, long search, size_t patch, int threads, float* out) {
+    if (w == 0 || h == 0) return DS_OK;
+    if (search < 0) return zero_fill(c, out, w * h);
+    if (threads < 1 || threads > 4096) return fail(c, DS_ERR_INVALID, "thread count %d out of range", threads);
+    if (patch > 64) return fail(c, DS_ERR_INVALID, "patch radius %zu too large", patch);
+    if (search > 1024) return fail(c, DS_ERR_INVALID, "search radius %ld too large", search);
+    SLOT(c, ws, float*, S_NLMREF, nlm_ref_workspace_floats(w, h, search, patch, threads) * 4);
+    return run(c, "nlm_reference_order",
+               [&] { return launch_nlm_reference_order(in, out, w, h, hp, search, patch, threads, ws, c->stream); });
+}
+
+// denoise (enhance.rs:74-92); ref_threads > 0 runs its NLM in the
+// reference's summation order for a pool of that many threads.
+int denoise_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* out,
+                int ref_threads = 0) {
     const size_t n = w * h;
     if (n == 0) return DS_OK;
     SLOT(c, L, float*, S_L, n * 4);
@@ -592,7 +608,8 @@ int denoise_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, l
     // enhance.rs:82-88
     const float k = amount / 100.0f;
     NlmH hp{0.0f, sig, 0.6f + 1.6f * k};
-    if ((rc = nlm_dev(c, L, w, h, hp, search, patch, LF))) return rc;
+    rc = ref_threads > 0 ? nlm_ref_dev(c, L, w, h, hp, search, patch, ref_threads, LF) : nlm_dev(c, L, w, h, hp, search, patch, LF);
+    if (rc) return rc;
     const float m = 0.5f + 0.5f * k;
     float rf = std::fmax(((float)(w > h ? w : h) / 600.0f) * (1.0f + 2.0f * k), 2.0f);
     size_t r = (size_t)rf;
@@ -1786,6 +1803,23 @@ int ds_denoise_ex_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float amo
     return denoise_dev(c, rgb, w, h, amount, search, patch, rgb_out);
 }
 
+int ds_nlm_reference_order_dev(ds_ctx* c, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, int threads,
+                               float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if ((!in || !out) && w * h) return fail(c, DS_ERR_INVALID, "null pointer");
+    return nlm_ref_dev(c, in, w, h, NlmH{h_param, nullptr, 1.0f}, search, patch, threads, out);
+}
+
+int ds_denoise_reference_order_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, int threads,
+                                   float* rgb_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if ((!rgb || !rgb_out) && w * h) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (threads < 1) return fail(c, DS_ERR_INVALID, "thread count %d out of range", threads);
+    return denoise_dev(c, rgb, w, h, amount, search, patch, rgb_out, threads);
+}
+
 int ds_raw_details_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float sigma, size_t iterations, float* rgb_out) {
     LOCK(c);
     if (int rc = check_dims(c, w, h)) return rc;
@@ -2074,6 +2108,39 @@ int ds_nlm(ds_ctx* c, const float* in, size_t w, size_t h, float h_param, long s
     return DS_OK;
 }
 
+int ds_nlm_reference_order(ds_ctx* c, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, int threads,
+                           float* out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    if (!in || !out) return fail(c, DS_ERR_INVALID, "null pointer");
+    SLOT(c, d_in, float*, S_IN, n * 4);
+    SLOT(c, d_out, float*, S_OUT, n * 4);
+    HIPCHK(c, hipMemcpyAsync(d_in, in, n * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = nlm_ref_dev(c, d_in, w, h, NlmH{h_param, nullptr, 1.0f}, search, patch, threads, d_out)) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, d_out, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
+int ds_denoise_reference_order(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, int threads,
+                               float* rgb_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    const size_t n = w * h;
+    if (n == 0) return DS_OK;
+    if (!rgb || !rgb_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    if (threads < 1) return fail(c, DS_ERR_INVALID, "thread count %d out of range", threads);
+    SLOT(c, d_in, float*, S_IN, n * 12);
+    SLOT(c, d_out, float*, S_OUT, n * 12);
+    HIPCHK(c, hipMemcpyAsync(d_in, rgb, n * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+    if (int rc = denoise_dev(c, d_in, w, h, amount, search, patch, d_out, threads)) return rc;
+    HIPCHK(c, hipMemcpyAsync(rgb_out, d_out, n * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return DS_OK;
+}
+
 int ds_denoise_ex(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* rgb_out) {
     LOCK(c);
     if (int rc = check_dims(c, w, h)) return rc;
@@ -2153,6 +2220,19 @@ int ds_luminance(ds_ctx* c, const float* rgb, size_t n, float* out) {
     LOCK(c);
     return host_op(c, "luminance", rgb, nullptr, 3 * n, out, n,
                    [&](float* a, float*, float* o) { return launch_luminance(a, n, o, c->stream); });
+}
+
+int ds_libm_hash(ds_ctx* c, int fn, uint64_t* hash_out) {
+    LOCK(c);
+    if (fn != DS_LIBM_CBRTF && fn != DS_LIBM_EXPF) return fail(c, DS_ERR_INVALID, "unknown function %d", fn);
+    if (!hash_out) return fail(c, DS_ERR_INVALID, "null pointer");
+    SLOT(c, d_acc, unsigned long long*, S_STATS, 64);
+    if (int rc = run(c, "libm_hash", [&] { return launch_libm_hash(fn, d_acc, c->stream); })) return rc;
+    unsigned long long v = 0;
+    HIPCHK(c, hipMemcpyAsync(&v, d_acc, 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    *hash_out = (uint64_t)v;
+    return DS_OK;
 }
 
 // ---- profiling

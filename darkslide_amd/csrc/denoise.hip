@@ -10,6 +10,7 @@
 
 #include "box.h"
 #include "common.h"
+#include "libm_ref.h"
 #include "denoise.h"
 #include "nlm.h"
 
@@ -27,9 +28,9 @@ __global__ void k_oklab_fwd(const float* __restrict__ rgb, size_t n, OklabM ok, 
     float p[3] = {rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]};
     float lms[3], c[3], lab[3];
     mul_mv(ok.to_lms, p, lms);
-    c[0] = cbrtf(lms[0]);
-    c[1] = cbrtf(lms[1]);
-    c[2] = cbrtf(lms[2]);
+    c[0] = ds_cbrtf(lms[0]);
+    c[1] = ds_cbrtf(lms[1]);
+    c[2] = ds_cbrtf(lms[2]);
     mul_mv(ok.m2, c, lab);
     L[i] = lab[0];
     A[i] = lab[1];

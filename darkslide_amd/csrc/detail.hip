@@ -6,9 +6,9 @@
 //
 // Per-pixel arithmetic keeps the reference's operation order and is
 // compiled without FMA contraction; the guided filters and box passes are
-// the exact running-sum kernels of the denoise path.  The transcendental
-// calls (powf in the sRGB curves, expf in the halo damping) are the
-// device's, within an ulp or two of glibc's.
+// the exact running-sum kernels of the denoise path.  The halo damping's
+// expf is ds_expf (glibc's bits); powf in the sRGB curves is the device's,
+// within an ulp or two of glibc's.
 
 #include <cmath>
 
@@ -16,6 +16,7 @@
 #include "common.h"
 #include "denoise.h"
 #include "detail.h"
+#include "libm_ref.h"
 #include "stages.h"
 
 #pragma clang fp contract(off)
@@ -215,7 +216,7 @@ __global__ void k_sharp_apply(float* __restrict__ rgb, const float* __restrict__
     if (amt == 0.0f) return;
     const float ev = e[i];
     const float d = ev - blur[i];
-    const float damp = detail + (1.0f - detail) * expf(-fabsf(d) / 0.04f);
+    const float damp = detail + (1.0f - detail) * ds_expf(-fabsf(d) / 0.04f);
     float dv = d * damp;
     if (s) {
         auto at = [&](int u, int v) { return s[(size_t)clampi(v, 0, h - 1) * w + clampi(u, 0, w - 1)]; };

@@ -19,5 +19,8 @@ hipError_t launch_guided_filter(const float* I, const float* P, size_t w, size_t
 hipError_t launch_oklab_fwd_interleaved(const float* rgb, size_t n, const OklabM& ok, float* lab, hipStream_t st);
 hipError_t launch_oklab_inv(const float* lab, size_t n, const OklabM& ok, float* rgb, hipStream_t st);
 hipError_t launch_luminance(const float* rgb, size_t n, float* out, hipStream_t st);
+// The order-free hash (libm_hash_term) of ds_cbrtf (fn 0) or ds_expf (fn 1)
+// over all 2^32 inputs, added into *acc (zeroed here).
+hipError_t launch_libm_hash(int fn, unsigned long long* acc, hipStream_t st);
 
 }  // namespace ds

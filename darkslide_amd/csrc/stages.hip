@@ -6,6 +6,7 @@
 
 #include "box.h"
 #include "common.h"
+#include "libm_ref.h"
 #include "rl.h"
 #include "stages.h"
 
@@ -90,9 +91,9 @@ __global__ void k_oklab_fwd_il(const float* __restrict__ rgb, size_t n, OklabM o
     float p[3] = {rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]};
     float lms[3], c[3], o[3];
     mul_mv(ok.to_lms, p, lms);
-    c[0] = cbrtf(lms[0]);
-    c[1] = cbrtf(lms[1]);
-    c[2] = cbrtf(lms[2]);
+    c[0] = ds_cbrtf(lms[0]);
+    c[1] = ds_cbrtf(lms[1]);
+    c[2] = ds_cbrtf(lms[2]);
     mul_mv(ok.m2, c, o);
     lab[3 * i] = o[0];
     lab[3 * i + 1] = o[1];
@@ -105,6 +106,24 @@ __global__ void k_luminance(const float* __restrict__ rgb, size_t n, float* __re
 }
 
 unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+__global__ void k_u64_zero(unsigned long long* p) { *p = 0; }
+
+// 2^32 inputs: 2^18 blocks of 256 threads, 64 inputs per thread at a
+// stride of 256; one atomic per wave.
+constexpr int LH_PER = 64;
+__global__ __launch_bounds__(256) void k_libm_hash(int fn, unsigned long long* __restrict__ acc) {
+    const uint32_t base = blockIdx.x * (256u * LH_PER) + threadIdx.x;
+    unsigned long long sum = 0;
+    for (int k = 0; k < LH_PER; k++) {
+        const uint32_t i = base + 256u * (uint32_t)k;
+        float x;
+        memcpy(&x, &i, 4);
+        sum += libm_hash_term(i, fn == 0 ? ds_cbrtf(x) : ds_expf(x));
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(acc, sum);
+}
 
 }  // namespace
 
@@ -219,6 +238,12 @@ hipError_t launch_oklab_inv(const float* lab, size_t n, const OklabM& ok, float*
 hipError_t launch_luminance(const float* rgb, size_t n, float* out, hipStream_t st) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_luminance, dim3(nblk(n)), dim3(256), 0, st, rgb, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_libm_hash(int fn, unsigned long long* acc, hipStream_t st) {
+    hipLaunchKernelGGL(k_u64_zero, dim3(1), dim3(1), 0, st, acc);
+    hipLaunchKernelGGL(k_libm_hash, dim3((unsigned)((1ull << 32) / (256 * LH_PER))), dim3(256), 0, st, fn, acc);
     return hipGetLastError();
 }
 

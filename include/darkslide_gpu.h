@@ -169,12 +169,29 @@ int ds_estimate_noise(ds_ctx* ctx, const float* plane, size_t w, size_t h, float
 /* enhance.rs:36 nlm(p, h_param, search, patch) */
 int ds_nlm(ds_ctx* ctx, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, float* out);
 
+/* enhance.rs:36 nlm(p, h_param, search, patch) in the reference's own
+ * summation order for a rayon pool of `threads` threads
+ * (rayon::current_num_threads(), enhance.rs:43-44): offsets in chunks of
+ * ceil((2S+1)^2 / threads), each chunk's sums offset by offset, running
+ * box sums along whole rows then whole columns (image.rs:280-309), glibc's
+ * expf, chunk partials added in order.  Bit for bit the CPU result; it
+ * walks every row and column once per offset, so it is a parity mode, far
+ * slower than ds_nlm. */
+int ds_nlm_reference_order(ds_ctx* ctx, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, int threads,
+                           float* out);
+
 /* enhance.rs:75 denoise(img, amount): NLM with search 3, patch 1. */
 int ds_denoise(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float amount, float* rgb_out);
 
 /* enhance.rs:75 denoise with the NLM search / patch radii as parameters
  * (the reference hard-codes 3, 1 at enhance.rs:84). */
 int ds_denoise_ex(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* rgb_out);
+
+/* enhance.rs:75 denoise with its NLM in the reference's summation order for
+ * a pool of `threads` threads (as ds_nlm_reference_order); every other stage
+ * is the same as ds_denoise_ex, so the result is the CPU path's, bit for bit. */
+int ds_denoise_reference_order(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, int threads,
+                               float* rgb_out);
 
 /* enhance.rs:96 raw_details(img, sigma, iterations) */
 int ds_raw_details(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float sigma, size_t iterations, float* rgb_out);
@@ -198,6 +215,16 @@ int ds_oklab_to_prophoto(ds_ctx* ctx, const float* lab, size_t n, float* rgb_out
 
 /* image.rs:174 Image3::luminance (color.rs:196 luma_prophoto) of n pixels. */
 int ds_luminance(ds_ctx* ctx, const float* rgb, size_t n, float* out);
+
+/* Self-check of the kernels' libm restatements: the cube root behind
+ * Oklab::from_prophoto (color.rs:408-412, f32::cbrt -> glibc cbrtf) and the
+ * exponential (f32::exp -> glibc expf).  Evaluates the device function on
+ * all 2^32 float inputs and returns sum_i mix(i << 32 | bits(f(x_i)))
+ * mod 2^64 (SplitMix64 finaliser, NaN results as 0x7fc00000); the host
+ * libm's outputs give the same hash when the two agree bit for bit. */
+#define DS_LIBM_CBRTF 0
+#define DS_LIBM_EXPF 1
+int ds_libm_hash(ds_ctx* ctx, int fn, uint64_t* hash_out);
 
 /* ---- device-resident variants (enqueue only) ---------------------------- */
 
@@ -246,6 +273,10 @@ int ds_reconstruct_highlights_dev(ds_ctx* ctx, float* rgb, const float* clip, si
 int ds_estimate_noise_dev(ds_ctx* ctx, const float* plane, size_t w, size_t h, float* sigma_out);
 int ds_nlm_dev(ds_ctx* ctx, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, float* out);
 int ds_denoise_ex_dev(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* rgb_out);
+int ds_nlm_reference_order_dev(ds_ctx* ctx, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, int threads,
+                               float* out);
+int ds_denoise_reference_order_dev(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch,
+                                   int threads, float* rgb_out);
 int ds_raw_details_dev(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float sigma, size_t iterations, float* rgb_out);
 
 /* ---- stage timing ------------------------------------------------------ */
