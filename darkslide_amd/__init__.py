@@ -183,6 +183,7 @@ _SIGS = {
     "ds_demosaic_u16_dev": [_P, _P, _SZ, _SZ, _SZ, _P, _F, _P, _P, _I, _P, _P],
     "ds_demosaic_f32_dev": [_P, _P, _SZ, _SZ, _P, _P, _I, _P, _P],
     "ds_estimate_noise_dev": [_P, _P, _SZ, _SZ, _P],
+    "ds_estimate_noise_noalloc": [_P, _P, _SZ, _SZ, _P],
     "ds_nlm_dev": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _P],
     "ds_denoise_ex_dev": [_P, _P, _SZ, _SZ, _F, _L, _SZ, _P],
     "ds_raw_details_dev": [_P, _P, _SZ, _SZ, _F, _SZ, _P],

@@ -1522,6 +1522,39 @@ extern "C" {
 
 const char* ds_version(void) { return "darkslide-mi355x 0.1 (gfx950)"; }
 
+// ---- device memory of the allocation-free entry point
+
+namespace {
+
+// The radix select's histograms and sample buffers for
+// ds_estimate_noise_noalloc: NOISE_SCRATCH_MAX_U32 words (3.2 MB), enough
+// for every plane.  Each GPU the library runs on holds its own copy until the
+// process ends.  In host code the name is a host-side stand-in: kernels get
+// the device address from noalloc_scratch_ptr instead.
+static __device__ __align__(256) uint32_t noalloc_scratch[NOISE_SCRATCH_MAX_U32];
+
+std::mutex g_noalloc_mu;
+std::map<int, uint32_t*> g_noalloc_ptr;  // device -> address of noalloc_scratch there
+
+// The address of noalloc_scratch on `device`, which must be current; looked
+// up once per device.
+hipError_t noalloc_scratch_ptr(int device, uint32_t** out) {
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    auto it = g_noalloc_ptr.find(device);
+    if (it != g_noalloc_ptr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess) return e;
+    g_noalloc_ptr[device] = (uint32_t*)p;
+    *out = (uint32_t*)p;
+    return hipSuccess;
+}
+
+}  // namespace
+
 int ds_ctx_create(int device, ds_ctx** out) {
     if (!out) return fail(nullptr, DS_ERR_INVALID, "out is null");
     *out = nullptr;
@@ -1539,6 +1572,10 @@ int ds_ctx_create(int device, ds_ctx** out) {
         return fail(nullptr, DS_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     c->ok = build_oklab();
+    // set-up of the allocation-free entry point on this device; a failed
+    // lookup is retried (and reported) by that entry point's first call
+    uint32_t* scr = nullptr;
+    (void)noalloc_scratch_ptr(device, &scr);
     *out = c;
     return DS_OK;
 }
@@ -1787,6 +1824,22 @@ int ds_estimate_noise_dev(ds_ctx* c, const float* plane, size_t w, size_t h, flo
     if (int rc = check_dims(c, w, h)) return rc;
     if (!sigma_out || (!plane && w * h)) return fail(c, DS_ERR_INVALID, "null pointer");
     return noise_dev(c, plane, w, h, sigma_out);
+}
+
+int ds_estimate_noise_noalloc(ds_ctx* c, const float* plane, size_t w, size_t h, float* sigma_out) {
+    LOCK(c);
+    if (int rc = check_dims(c, w, h)) return rc;
+    if (!sigma_out || (!plane && w * h)) return fail(c, DS_ERR_INVALID, "null pointer");
+    const size_t need = noise_scratch_u32(w, h);
+    if (need > NOISE_SCRATCH_MAX_U32)
+        return fail(c, DS_ERR_INVALID, "plane %zux%zu needs %zu scratch words, the static scratch holds %zu", w, h, need,
+                    NOISE_SCRATCH_MAX_U32);
+    uint32_t* scr = nullptr;
+    const hipError_t e = noalloc_scratch_ptr(c->device, &scr);
+    if (e != hipSuccess) return hip_fail(c, e, "hipGetSymbolAddress (noise scratch)");
+    const hipError_t le = launch_estimate_noise(plane, w, h, scr, sigma_out, c->stream);
+    if (le != hipSuccess) return fail(c, DS_ERR_LAUNCH, "estimate_noise: %s", hipGetErrorString(le));
+    return DS_OK;
 }
 
 int ds_nlm_dev(ds_ctx* c, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, float* out) {

@@ -14,6 +14,13 @@ hipError_t launch_oklab_fwd(const float* rgb, size_t n, const OklabM& ok, float*
 size_t noise_sample_count(size_t w, size_t h);
 // Scratch of launch_estimate_noise in u32: a first-level histogram and the samples.
 size_t noise_scratch_u32(size_t w, size_t h);
+// The largest scratch any plane needs: the histograms and control words
+// (NOISE_HDR_U32) plus twice the sample count ceil(n / max(n / 200000, 1)),
+// which peaks at 399,999 (n = 399,999, step 1; above 400,000 pixels the
+// step grows and the count stays below 300,000).
+constexpr size_t NOISE_HDR_U32 = 4096 + 1024 + 64;
+constexpr size_t NOISE_MAX_SAMPLES = 399999;
+constexpr size_t NOISE_SCRATCH_MAX_U32 = NOISE_HDR_U32 + 2 * NOISE_MAX_SAMPLES;
 // Number of samples estimate_noise keeps (off the 1-pixel border) for a
 // w x h plane sampled every step pixels.
 size_t noise_valid_samples(size_t w, size_t h, size_t step);

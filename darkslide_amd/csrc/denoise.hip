@@ -59,6 +59,7 @@ constexpr int NOISE_L2_BINS = 1024;
 constexpr int NOISE_CTRL = 64;  // [0]: list length
 constexpr int NOISE_HDR = NOISE_L1_BINS + NOISE_L2_BINS + NOISE_CTRL;
 static_assert(NOISE_HDR % 4 == 0, "16-byte aligned list and samples");
+static_assert((size_t)NOISE_HDR == NOISE_HDR_U32, "scratch header size published in denoise.h");
 constexpr int NS_NT = 256, NS_PER = 4;  // sample / level-2 block: 1024 samples
 
 // One histogram increment per active lane of the wave; the lanes sharing

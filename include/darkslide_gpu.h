@@ -271,6 +271,17 @@ int ds_reconstruct_highlights_dev(ds_ctx* ctx, float* rgb, const float* clip, si
                                   const float* raw_clip);
 /* sigma_out is a device pointer to one float */
 int ds_estimate_noise_dev(ds_ctx* ctx, const float* plane, size_t w, size_t h, float* sigma_out);
+/* No counterpart in the reference: ds_estimate_noise_dev for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no allocating routine:
+ * the select's scratch (histograms and samples, 3.2 MB) is a static device
+ * array of the library, one per GPU, sized for every plane size; a plane
+ * needing more would be refused with DS_ERR_INVALID.  Every call on a GPU
+ * shares that array and leaves its contents there (the call clears what must
+ * start at zero), so calls that overlap on one GPU (from two contexts, or
+ * streams running at once) race; calls on one stream are ordered.  plane and
+ * sigma_out are device pointers; enqueue only. */
+int ds_estimate_noise_noalloc(ds_ctx* ctx, const float* plane, size_t w, size_t h, float* sigma_out);
 int ds_nlm_dev(ds_ctx* ctx, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, float* out);
 int ds_denoise_ex_dev(ds_ctx* ctx, const float* rgb, size_t w, size_t h, float amount, long search, size_t patch, float* rgb_out);
 int ds_nlm_reference_order_dev(ds_ctx* ctx, const float* in, size_t w, size_t h, float h_param, long search, size_t patch, int threads,
