@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -227,6 +228,7 @@ struct GraphEntry {
     int branches = 1;
     size_t n_frames = 0;
     unsigned long gen = 0;  // workspace generation the graph was captured against
+    uint64_t net_uid = 0;   // the learned denoiser's network (ds_cnn::uid), 0 for none
 };
 
 struct Pending {
@@ -366,10 +368,12 @@ bool valid_bayer(const uint8_t* p) {
     return diag03 || diag12;
 }
 
+// Images below 2^32 pixels: the box passes index a plane with 32-bit
+// element offsets (row * width).
 int check_dims(ds_ctx* c, size_t w, size_t h) {
     if (!c) return DS_ERR_INVALID;
-    if (w > (size_t)INT32_MAX / 4 || h > (size_t)INT32_MAX / 4 || (w && h && w * h > ((size_t)1 << 34)))
-        return fail(c, DS_ERR_INVALID, "image %zux%zu too large", w, h);
+    if (w > (size_t)INT32_MAX / 4 || h > (size_t)INT32_MAX / 4 || (w && h && w * h >= ((size_t)1 << 32)))
+        return fail(c, DS_ERR_INVALID, "image %zux%zu too large (2^32 pixels or more)", w, h);
     return DS_OK;
 }
 
@@ -650,6 +654,7 @@ int synth_dev(ds_ctx* c, uint64_t seed, size_t w, size_t h, int bits, float iso,
 
 struct ds_cnn {
     ds_ctx* ctx = nullptr;
+    uint64_t uid = 0;          // never reused, unlike the address: batch graphs are keyed on it
     float* state = nullptr;    // params | grad | adam m | adam v, CNN_NPARAM floats each
     uint16_t* wpk = nullptr;   // bf16 weights of the MFMA convolutions (CNN_WPK values)
     long step = 0;
@@ -1199,6 +1204,13 @@ int validate_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipelin
         if (o.quality == DS_QUALITY_DRAFT && (f.w < 2 || f.h < 2)) return fail(c, DS_ERR_INVALID, "frame %zu: too small for superpixels", i);
         if (o.host_frames && (!pinned_host(f.raw) || !pinned_host(f.rgb_out)))
             return fail(c, DS_ERR_INVALID, "frame %zu: host frames must be in pinned host memory (ds_host_alloc)", i);
+        if (o.denoise == DS_DENOISE_LEARNED) {  // the network's own limits, checked before any capture starts
+            const FramePlan p = plan_frame(f, i, o);
+            if (int rc = cnn_check_shape(c, 1, p.h, p.w)) {
+                c->err = "frame " + std::to_string(i) + ": " + c->err;
+                return rc;
+            }
+        }
     }
     if (o.host_frames != 0 && o.host_frames != 1) return fail(c, DS_ERR_INVALID, "host_frames must be 0 or 1");
     return DS_OK;
@@ -1222,7 +1234,9 @@ std::vector<uint8_t> batch_key(ds_ctx* c, const ds_frame* frames, size_t n, cons
     put(&o.rl_sigma, sizeof o.rl_sigma);
     put(&o.rl_iters, sizeof o.rl_iters);
     put(&o.reference_gating, sizeof o.reference_gating);
-    put(&o.net, sizeof o.net);
+    // the network by its unique id (a new network may reuse a freed one's address)
+    const uint64_t net_uid = o.denoise == DS_DENOISE_LEARNED && o.net ? o.net->uid : 0;
+    put(&net_uid, sizeof net_uid);
     put(&o.threshold_db, sizeof o.threshold_db);
     put(&o.highlights, sizeof o.highlights);
     put(&o.quality, sizeof o.quality);
@@ -1299,6 +1313,8 @@ int capture_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline
     }
     c->capturing = true;
     int rc = DS_OK;
+    int forked = 0;                  // branches that waited on the fork event
+    std::vector<char> joined(k, 0);  // branches joined back into the context stream
     if (hipEventRecord(ev[k], c->stream) != hipSuccess) rc = fail(c, DS_ERR_DEVICE, "fork event");
     for (int j = 0; j < k && !rc; j++) {
         ds_ctx* b = c->branches[j];
@@ -1307,6 +1323,7 @@ int capture_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline
             rc = fail(c, DS_ERR_DEVICE, "fork wait");
             break;
         }
+        forked = j + 1;
         for (size_t i : assign[j]) {
             const FramePlan p = plan_frame(frames[i], i, o);
             rc = enqueue_frame(b, frames[i], p, o, p.learned ? counts + i : nullptr);
@@ -1317,7 +1334,12 @@ int capture_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline
         }
         if (!rc && hipEventRecord(ev[j], b->stream) != hipSuccess) rc = fail(c, DS_ERR_DEVICE, "join event");
         if (!rc && hipStreamWaitEvent(c->stream, ev[j], 0) != hipSuccess) rc = fail(c, DS_ERR_DEVICE, "join wait");
+        if (!rc) joined[j] = 1;
     }
+    // a failed capture still joins every branch it forked, so that ending
+    // the capture leaves no stream in capture mode; the graph is discarded
+    for (int j = 0; j < forked; j++)
+        if (!joined[j] && hipEventRecord(ev[j], c->branches[j]->stream) == hipSuccess) (void)hipStreamWaitEvent(c->stream, ev[j], 0);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(c->stream, &graph);
     c->prof = prof;
@@ -1338,6 +1360,7 @@ int capture_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline
     g.branches = k;
     g.n_frames = n;
     g.gen = c->slot_gen;
+    g.net_uid = o.denoise == DS_DENOISE_LEARNED ? o.net->uid : 0;
     return DS_OK;
 }
 
@@ -2346,8 +2369,10 @@ int ds_cnn_create(ds_ctx* c, uint64_t seed, ds_cnn** out) {
     LOCK(c);
     if (!out) return fail(c, DS_ERR_INVALID, "out is null");
     *out = nullptr;
+    static std::atomic<uint64_t> next_uid{0};
     ds_cnn* net = new ds_cnn();
     net->ctx = c;
+    net->uid = ++next_uid;
     net->seed = seed;
     hipError_t e = hipMalloc(&net->state, 4 * CNN_NPARAM * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&net->wpk, CNN_WPK * sizeof(uint16_t));
@@ -2379,6 +2404,15 @@ void ds_cnn_destroy(ds_cnn* net) {
         std::lock_guard<std::recursive_mutex> g(c->mu);
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
+        // batch graphs captured with this network read its buffers: retire them
+        std::vector<GraphEntry> keep;
+        for (auto& ge : c->graphs) {
+            if (ge.net_uid == net->uid)
+                destroy_graph(ge);
+            else
+                keep.push_back(ge);
+        }
+        c->graphs.swap(keep);
         (void)hipFree(net->state);
         (void)hipFree(net->wpk);
         if (net->tr_act) (void)hipFree(net->tr_act);

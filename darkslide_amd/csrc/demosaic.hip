@@ -590,8 +590,8 @@ hipError_t launch_cfa_clip_levels(const DemosaicArgs& a, float* partial, float* 
 
 // Tile-shape variants (TX x TY, threads), all with the same per-pixel
 // arithmetic, so every variant gives the same result bit for bit.
-//   RCD: 0 = 64x32/512, 1 = 32x32/256, 2 = 64x16/256
-//   MHC: 0 = 64x16/256, 1 = 128x8/256, 2 = 32x32/256
+//   RCD: 0 = 64x32/1024, 1 = 32x32/256, 2 = 64x16/256
+//   MHC: 0 = 128x32/1024, 1 = 128x8/256, 2 = 32x32/256
 int demosaic_variants(int algo) { return algo == DS_ALGO_RCD ? 3 : 3; }
 
 hipError_t launch_demosaic(const DemosaicArgs& a, hipStream_t stream, int variant) {

@@ -21,6 +21,7 @@
  *
  * Every call returns DS_OK (0) or a negative DS_ERR_* code;
  * ds_last_error() gives the message of the last failure on that context.
+ * Images of 2^32 pixels or more are refused with DS_ERR_INVALID.
  * A context serialises its calls with an internal mutex, so it may be shared
  * between threads; one context per thread gives concurrency.
  */
