@@ -107,6 +107,86 @@ def cpu_baseline(size, reps):
     }
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_stages(w, h):
+    """Each stage of the pipeline once on the CPU at the full frame size
+    (the oracle's FP32 restatement, OpenMP over all host cores, NLM offsets
+    chunked as rayon chunks them): RCD, MHC, denoise, RL."""
+    import numpy as np
+    import darkslide_amd as D
+    import oracle as O
+    threads = O.threads()
+    raw = D.synth_bayer(0x5EED0001, w, h)
+    out = {}
+    t0 = time.perf_counter()
+    cfa = O.cfa_normalize_u16(raw, D.RGGB, 512.0, [16383.0] * 3)
+    rgb = O.rcd_bayer(cfa)
+    O.clip_plane(cfa)
+    out["demosaic_rcd"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.mhc_bayer(O.cfa_normalize_u16(raw, D.RGGB, 512.0, [16383.0] * 3))
+    out["demosaic_mhc"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dn = O.denoise(rgb, 0.0, 10, 3, nthreads=threads)
+    out["denoise"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.raw_details(dn, 1.0, 20)
+    out["raw_details"] = time.perf_counter() - t0
+    n = w * h
+    return {k: {"s": round(v, 3), "mp_s": round(n / 1e6 / v, 3)} for k, v in out.items()}
+
+
+def rcd_config1(ctx, size=1024, reps=20):
+    """Configuration 1: RCD demosaic of one 1024x1024 frame, GPU kernel time
+    (HIP events, device-resident) beside the CPU restatement's stage time
+    (median of 3 after 1 warm-up)."""
+    import numpy as np
+    import darkslide_amd as D
+    import oracle as O
+    L = D.lib()
+    n = size * size
+    raw_h = D.synth_bayer(0x5EED0000, size, size)
+    raw, rgb, clip = ctx.alloc(n * 2), ctx.alloc(n * 12), ctx.alloc(n * 4)
+    raw.upload(raw_h)
+    pat, white, ones = D._pat(D.RGGB), D._f3((16383.0,) * 3), D._f3((1.0, 1.0, 1.0))
+    go = lambda: ctx._check(L.ds_demosaic_u16_dev(ctx.handle, raw.ptr, size, size, size, pat, 512.0, white, ones, D.ALGO_RCD,
+                                                  rgb.ptr, clip.ptr))
+    for _ in range(3):
+        go()
+    ctx.sync()
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    for _ in range(reps):
+        go()
+    gpu_ms = ctx.prof()["demosaic_rcd"][0] / reps
+    ctx.prof_enable(False)
+    for b in (raw, rgb, clip):
+        b.free()
+    times = []
+    for k in range(4):
+        t0 = time.perf_counter()
+        cfa = O.cfa_normalize_u16(raw_h, D.RGGB, 512.0, [16383.0] * 3)
+        O.rcd_bayer(cfa)
+        O.clip_plane(cfa)
+        if k:
+            times.append(time.perf_counter() - t0)
+    cpu_s = statistics.median(times)
+    return {"workload": f"RCD demosaic + clip plane, {size}x{size} RGGB 14-bit", "gpu_ms": round(gpu_ms, 4),
+            "gpu_mp_s": round(n / 1e6 / (gpu_ms * 1e-3), 1),
+            "gpu_hbm_gbs": round(DEMOSAIC_BYTES_PER_PX * n / (gpu_ms * 1e-3) / 1e9, 1),
+            "cpu_ms": round(cpu_s * 1e3, 3), "cpu_mp_s": round(n / 1e6 / cpu_s, 2), "cpu_threads": O.threads()}
+
+
 def _psnr(a, ref):
     import numpy as np
     d = a.astype(np.float64) - ref.astype(np.float64)
@@ -515,6 +595,7 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=1024)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-stages", action="store_true", help="skip the full-frame per-stage CPU timings")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device copy of this size (known-size reference for HBM counters)")
     ap.add_argument("--checkpoint", default=os.path.join(ROOT, "darkslide_amd", "weights", "cnn_l32x8.dsck"))
@@ -603,9 +684,9 @@ def main():
     dominant = max(prof.items(), key=lambda kv: kv[1][0])[0]
     roofline = {
         "kernel": "k_nlm_tile (nlm)",
-        "bound": "mfma",
-        "bound_note": "compute-bound: FP32 arithmetic on the vector ALUs, priced against MI355X's dense FP32 peak "
-                      "(157.3 TFLOP/s, the published MFMA FP32 figure); NLM issues no MFMA instructions",
+        "bound": "valu",
+        "bound_note": "compute-bound: FP32 arithmetic on the vector ALUs (no MFMA instructions), priced against "
+                      "MI355X's published FP32 peak of 157.3 TFLOP/s",
         "achieved": round(nlm_tflops, 3),
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
@@ -633,7 +714,23 @@ def main():
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline(args.cpu_size, args.cpu_reps)
-        log(f"cpu baseline {cpu['value']} MP/s on {cpu['cores']} threads")
+        cpu["cpu_model"] = cpu_model()
+        cpu["logical_cpus"] = os.cpu_count()
+        log(f"cpu baseline {cpu['value']} MP/s on {cpu['cores']} threads ({cpu['cpu_model']})")
+        if not args.no_cpu_stages:
+            log(f"cpu stages at {w}x{h} ...")
+            cs = cpu_stages(w, h)
+            gpu_of = {"demosaic_rcd": stages["demosaic_rcd"]["avg_ms"], "demosaic_mhc": stages["demosaic_mhc"]["avg_ms"],
+                      "denoise": sum(stages[k]["avg_ms"] for k in ("oklab_fwd", "estimate_noise", "nlm", "guided_filter") if k in stages),
+                      "raw_details": stages["raw_details"]["avg_ms"]}
+            for k, v in cs.items():
+                v["gpu_ms"] = gpu_of[k]
+                v["gpu_speedup"] = round(v["s"] * 1e3 / gpu_of[k], 1) if gpu_of[k] > 0 else None
+            cpu["stages_full_frame"] = {"frame": f"{w}x{h}", "threads": cpu["cores"], "stages": cs,
+                                        "note": "each stage once on the CPU (s); gpu_ms: the same stage's kernels, averaged over the timed steps"}
+            log(f"cpu stages: {cs}")
+        cpu["config1_rcd"] = rcd_config1(ctx)
+        log(f"config 1 RCD: {cpu['config1_rcd']}")
     line = {
         "metric": METRIC,
         "value": round(value, 2),
