@@ -22,6 +22,8 @@
 //   B: lane holds B[k = 8g .. 8g + 7][n = l % 16]
 //   D: lane holds D[m = 4g + i][n = l % 16], i = 0..3
 
+#include <type_traits>
+
 #include "cnn.h"
 #include "common.h"
 
@@ -575,23 +577,46 @@ __device__ __forceinline__ int mod3(int r) { return ((r % 3) + 3) % 3; }
 __device__ __forceinline__ int fs_sw(int pr) { return (pr >> 1) & 3; }
 __device__ __forceinline__ int fs_chunk(int pr, int c) { return pr * 4 + (c ^ fs_sw(pr)); }  // uint4 index in a ring row
 
+// Byte offsets, within a ring row, of the operands a lane touches.  A
+// lane's B fragment for column tile tb and horizontal tap kx is the 16-byte
+// chunk g of pixel 16 tb + ln + kx; the swizzle (pixel >> 1) & 3 depends
+// only on ln + kx (16 tb is a multiple of 8), so the offset is a per-lane
+// part plus 1024 tb bytes.  The epilogue writes pixel 16 tb + ln + 1 the
+// same way.
+struct FsLane {
+    int b[3];  // B fragment, tap kx = 0..2, column tile 0
+    int st;    // epilogue store (8 bytes), column tile 0
+};
+__device__ __forceinline__ FsLane fs_lane(int lane, int mt) {
+    const int g = lane >> 4, ln = lane & 15;
+    FsLane o;
+#pragma unroll
+    for (int kx = 0; kx < 3; kx++) o.b[kx] = fs_chunk(ln + kx, g) * 16;
+    o.st = fs_chunk(ln + 1, 2 * mt + (g >> 1)) * 16 + 8 * (g & 1);
+    return o;
+}
+template <class T>
+__device__ __forceinline__ T lds_at(const void* base, int byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(base) + byte_off);
+}
+
 // One MFMA layer row: dst row (columns 0..FS_W-1) from three source rows.
 // GLOBAL_OUT: store the stored columns to HBM instead of an LDS ring.
 template <bool GLOBAL_OUT>
 __device__ __forceinline__ void fs_layer(const uint16_t* const src[3], uint16_t* dst, const bf16x8 (&wa)[9], const float (&bias)[4], int wave,
-                                         int lane, bool row_in, int gx0, int Wd, int store_lo, int store_hi) {
+                                         int lane, const FsLane& lo, bool row_in, int gx0, int Wd, int store_lo, int store_hi) {
     const int g = lane >> 4, ln = lane & 15, mt = wave & 1;
+    const int tb = wave >> 1;  // column tiles tb, tb + 2, tb + 4
     f32x4 acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
     bf16x8 bcur[3], bnxt[3];
 #pragma unroll
-    for (int q = 0; q < 3; q++) bcur[q] = reinterpret_cast<const bf16x8*>(src[0])[fs_chunk(((wave >> 1) + 2 * q) * 16 + ln, g)];
+    for (int q = 0; q < 3; q++) bcur[q] = lds_at<bf16x8>(src[0], lo.b[0] + (tb + 2 * q) * 1024);
 #pragma unroll
     for (int t = 0; t < 9; t++) {
         if (t < 8) {
             const int ky = (t + 1) / 3, kx = (t + 1) % 3;
 #pragma unroll
-            for (int q = 0; q < 3; q++)
-                bnxt[q] = reinterpret_cast<const bf16x8*>(src[ky])[fs_chunk(((wave >> 1) + 2 * q) * 16 + ln + kx, g)];
+            for (int q = 0; q < 3; q++) bnxt[q] = lds_at<bf16x8>(src[ky], lo.b[kx] + (tb + 2 * q) * 1024);
         }
 #pragma unroll
         for (int q = 0; q < 3; q++) acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t], bcur[q], acc[q], 0, 0, 0);
@@ -602,7 +627,7 @@ __device__ __forceinline__ void fs_layer(const uint16_t* const src[3], uint16_t*
     }
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-        const int j = ((wave >> 1) + 2 * q) * 16 + ln;
+        const int j = (tb + 2 * q) * 16 + ln;
         const int gx = gx0 + j;
         const bool in = row_in && gx >= 0 && gx < Wd;
         const uint32_t keep = in ? 0xffffffffu : 0u;
@@ -612,12 +637,15 @@ __device__ __forceinline__ void fs_layer(const uint16_t* const src[3], uint16_t*
         if (GLOBAL_OUT) {
             if (in && j >= store_lo && j < store_hi) *reinterpret_cast<uint2*>(dst + (size_t)gx * C + mt * 16 + 4 * g) = o;
         } else {
-            *reinterpret_cast<uint2*>(dst + (size_t)fs_chunk(j + 1, 2 * mt + (g >> 1)) * 8 + 4 * (g & 1)) = o;
+            *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + (tb + 2 * q) * 1024) = o;
         }
     }
 }
 
 // PASS 0 (A): x -> a4 (act_out, n*H*W*32 bf16).  PASS 1 (B): a4 (act_in) -> y.
+// The row loop is unrolled by three (y0 and the first row are multiples of
+// three), so every ring slot ((row) mod 3) is a compile-time constant and
+// the LDS addresses are a lane's fixed offsets plus immediates.
 template <int PASS>
 __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict__ x, const uint16_t* __restrict__ act_in,
                                                         const float* __restrict__ params, const uint16_t* __restrict__ wpk,
@@ -628,9 +656,11 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     float* xr = reinterpret_cast<float*>(fs_lds + NR * 3 * FS_ROW_BYTES);  // A: x ring [3][FS_RW]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, ln = lane & 15;
     const int x0 = blockIdx.x * FS_OUT, y0 = blockIdx.y * FS_SH;
+    static_assert(FS_SH % 3 == 0, "ring slots follow the row index mod 3");
     const size_t ioff = (size_t)blockIdx.z * H * Wd;
     const int gx0 = x0 - FS_HALO;  // global column of computed column 0
-    auto RING = [&](int k, int row) { return ring + (size_t)(k * 3 + mod3(row)) * FS_RW * C; };
+    // ring k, slot s (= row mod 3)
+    auto RING = [&](int k, int slot) { return ring + (size_t)(k * 3 + slot) * FS_RW * C; };
 
     for (int i = tid; i < (int)(NR * 3 * FS_ROW_BYTES / 16); i += FS_NT) reinterpret_cast<uint4*>(fs_lds)[i] = make_uint4(0, 0, 0, 0);
     // input / output layer operands (as k_conv_in / k_conv_out)
@@ -660,6 +690,7 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
 #pragma unroll
         for (int i = 0; i < 4; i++) bm[l][i] = params[CNN_OFF_MID + (size_t)(l0 + l) * CNN_MID_STRIDE + C * 9 * C + mt * 16 + 4 * g + i];
     }
+    const FsLane lo = fs_lane(lane, mt);
 
     // input rows: A reads x rows (FS_RW floats), B reads a4 rows (FS_RW pixels of 64 bytes)
     // row i of the input ring holds global column gx0 - 1 + i
@@ -682,43 +713,50 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
             }
         }
     };
-    auto commit = [&](int row) {  // registers -> input ring
+    auto commit = [&](int slot) {  // registers -> input ring slot
 #pragma unroll
         for (int k = 0; k < IN_K; k++) {
             const int u = tid + k * FS_NT;
             if (u < IN_UNITS) {
                 if (PASS == 0)
-                    xr[mod3(row) * FS_RW + u] = xv[k];
+                    xr[slot * FS_RW + u] = xv[k];
                 else
-                    reinterpret_cast<uint4*>(RING(0, row))[fs_chunk(u >> 2, u & 3)] = av[k];
+                    reinterpret_cast<uint4*>(RING(0, slot))[fs_chunk(u >> 2, u & 3)] = av[k];
             }
         }
     };
-    // step r consumes input rows r-1..r+1 and fetches r+2; the first step is r = y0 - 3
+    // step r consumes input rows r-1..r+1 and fetches r+2; the first step is
+    // r = y0 - 3, a multiple of 3
     const int r_first = y0 - 3;
+    const int r_end = y0 + FS_SH + 4;
     fetch(r_first - 1);
     __syncthreads();  // ring zeroing done before any commit
-    commit(r_first - 1);
+    commit(2);        // row r_first - 1, slot 2
     fetch(r_first);
-    commit(r_first);
+    commit(0);
     fetch(r_first + 1);
 
-    for (int r = r_first; r < y0 + FS_SH + 4; r++) {
-        commit(r + 1);
+    // one step at row r with r mod 3 == SUB
+    auto step = [&](auto sub_tag, int r) {
+        constexpr int SUB = decltype(sub_tag)::value;
+        auto slot = [](int c) { return ((SUB + c) % 3 + 3) % 3; };  // constant-folded
+        commit(slot(1));
         fetch(r + 2);
         __syncthreads();
         if (PASS == 0) {
             // ---- a1 row r into ring 0 (one MFMA per tile, K = taps; as k_conv_in)
-            uint16_t* dst = RING(0, r);
+            uint16_t* dst = RING(0, slot(0));
             const bool row_in = r >= 0 && r < H;
+            const int tb = wave >> 1;
 #pragma unroll
             for (int q = 0; q < 3; q++) {
-                const int j = ((wave >> 1) + 2 * q) * 16 + ln;
+                const int j = (tb + 2 * q) * 16 + ln;
                 bf16x8 bv;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
                     const int t = 8 * g + e;
-                    const float v = t < 9 ? xr[mod3(r + t / 3 - 1) * FS_RW + j + t % 3] : 0.0f;
+                    const int ty = t < 9 ? t / 3 : 0, tx = t < 9 ? t % 3 : 0;
+                    const float v = t < 9 ? xr[slot(ty - 1) * FS_RW + j + tx] : 0.0f;
                     bv[e] = (__bf16)v;
                 }
                 const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(win, bv, f32x4{0, 0, 0, 0}, 0, 0, 0);
@@ -727,31 +765,32 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
                 uint2 o;
                 o.x = (f2bf(fmaxf(acc[0] + bin[0], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[1] + bin[1], 0.0f)) << 16)) & keep;
                 o.y = (f2bf(fmaxf(acc[2] + bin[2], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[3] + bin[3], 0.0f)) << 16)) & keep;
-                *reinterpret_cast<uint2*>(dst + (size_t)fs_chunk(j + 1, 2 * (wave & 1) + (g >> 1)) * 8 + 4 * (g & 1)) = o;
+                *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + (tb + 2 * q) * 1024) = o;
             }
             __syncthreads();
         }
         // ---- three MFMA layers: ring k -> ring k+1 (A: rows r-1, r-2, r-3; B: rows r, r-1, r-2)
-        const int base = PASS == 0 ? r - 1 : r;
+        constexpr int BASE = PASS == 0 ? -1 : 0;  // first output row relative to r
 #pragma unroll
         for (int l = 0; l < 3; l++) {
-            const int orow = base - l;
-            const uint16_t* src[3] = {RING(l, orow - 1), RING(l, orow), RING(l, orow + 1)};
+            const int orow = r + BASE - l;
+            const int os = slot(BASE - l);
+            const uint16_t* src[3] = {RING(l, slot(BASE - l - 1)), RING(l, os), RING(l, slot(BASE - l + 1))};
             const bool row_in = orow >= 0 && orow < H;
             if (PASS == 0 && l == 2) {
                 // a4: stored columns and rows of this block go to HBM
                 if (orow >= y0 && orow < y0 + FS_SH)
-                    fs_layer<true>(src, act_out + ioff * C + (size_t)orow * Wd * C, wa[l], bm[l], wave, lane, row_in, gx0, Wd, FS_HALO,
+                    fs_layer<true>(src, act_out + ioff * C + (size_t)orow * Wd * C, wa[l], bm[l], wave, lane, lo, row_in, gx0, Wd, FS_HALO,
                                    FS_HALO + FS_OUT);
             } else {
-                fs_layer<false>(src, RING(l + 1, orow), wa[l], bm[l], wave, lane, row_in, gx0, Wd, 0, 0);
+                fs_layer<false>(src, RING(l + 1, os), wa[l], bm[l], wave, lane, lo, row_in, gx0, Wd, 0, 0);
                 __syncthreads();
             }
         }
         if (PASS == 1) {
             // ---- output row r - 3: W_out in row 0 of the A operand, one MFMA per tap (as k_conv_out)
             const int orow = r - 3;
-            const uint16_t* src[3] = {RING(3, orow - 1), RING(3, orow), RING(3, orow + 1)};
+            const uint16_t* src[3] = {RING(3, slot(-4)), RING(3, slot(-3)), RING(3, slot(-2))};
             const bool store_row = orow >= y0 && orow < y0 + FS_SH && orow < H;
 #pragma unroll
             for (int q = 0; q < 2; q++) {
@@ -760,7 +799,7 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
                     f32x4 acc = {0, 0, 0, 0};
 #pragma unroll
                     for (int t = 0; t < 9; t++) {
-                        const bf16x8 bv = reinterpret_cast<const bf16x8*>(src[t / 3])[fs_chunk(nt * 16 + ln + t % 3, g)];
+                        const bf16x8 bv = lds_at<bf16x8>(src[t / 3], lo.b[t % 3] + nt * 1024);
                         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wout[t], bv, acc, 0, 0, 0);
                     }
                     const int j = nt * 16 + ln, gx = gx0 + j;
@@ -771,6 +810,11 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
                 }
             }
         }
+    };
+    for (int rb = r_first; rb < r_end; rb += 3) {
+        step(std::integral_constant<int, 0>{}, rb);
+        if (rb + 1 < r_end) step(std::integral_constant<int, 1>{}, rb + 1);
+        if (rb + 2 < r_end) step(std::integral_constant<int, 2>{}, rb + 2);
     }
 }
 
