@@ -567,77 +567,88 @@ constexpr size_t FS_ROW_BYTES = (size_t)FS_RW * C * sizeof(uint16_t);
 constexpr size_t FS_LDS_A = 3 * 3 * FS_ROW_BYTES + 3 * FS_RW * sizeof(float);  // rings a1..a3, x ring
 constexpr size_t FS_LDS_B = 4 * 3 * FS_ROW_BYTES;                              // rings a4..a7
 
-__device__ __forceinline__ int mod3(int r) { return ((r % 3) + 3) % 3; }
+// Ring rows are stored class-major: computed column j (0 <= j < FS_W) sits
+// at entry (j % FS_NC) * 16 + j / FS_NC, so the MFMA column tile "class s"
+// (lane ln <-> column s + FS_NC * ln) is 16 consecutive entries.  Tap kx of
+// class t reads columns (t + kx - 1) + FS_NC * ln, which is class t + kx - 1
+// for 1 <= t + kx <= FS_NC; class -1 is class FS_NC - 1 one lane down and
+// class FS_NC is class 0 one lane up, with column -1 and column FS_W (zero
+// padding, or in the input ring of pass B the neighbouring image columns) at
+// entries FS_W and FS_W + 1.  Every B fragment is therefore loaded once and
+// feeds the three horizontal taps of its neighbours, and each MFMA still sees
+// exactly the operands of the layer-by-layer kernels.
+//
+// A pixel's four 16-byte channel chunks are XOR-swizzled by (entry >> 1) & 3,
+// so that the eight lanes of a ds_read_b128 group, which read the same chunk
+// of eight consecutive entries, hit eight different bank groups (64-byte
+// entries would otherwise put every other entry on the same banks).
+constexpr int FS_NC = FS_W / 16;   // column classes (6)
+constexpr int FS_CPW = FS_NC / 2;  // classes per wave (a wave owns one half of the row)
+constexpr int FS_WIN = FS_CPW + 2; // classes a wave reads: its own and one either side
+static_assert(FS_NC % 2 == 0, "two waves per channel half split the classes");
+__host__ __device__ constexpr int fs_entry(int col) { return col < 0 ? FS_W : col >= FS_W ? FS_W + 1 : (col % FS_NC) * 16 + col / FS_NC; }
+__device__ __forceinline__ int fs_chunk(int e, int c) { return e * 4 + (c ^ ((e >> 1) & 3)); }  // uint4 index in a ring row
 
-// Ring rows store a pixel's four 16-byte channel chunks XOR-swizzled by
-// (pixel >> 1) & 3, so that the eight lanes of a ds_read_b128 group, which
-// read the same chunk of eight consecutive pixels, hit eight different bank
-// groups (64-byte pixels would otherwise put every other pixel on the same
-// banks).
-__device__ __forceinline__ int fs_sw(int pr) { return (pr >> 1) & 3; }
-__device__ __forceinline__ int fs_chunk(int pr, int c) { return pr * 4 + (c ^ fs_sw(pr)); }  // uint4 index in a ring row
-
-// Byte offsets, within a ring row, of the operands a lane touches.  A
-// lane's B fragment for column tile tb and horizontal tap kx is the 16-byte
-// chunk g of pixel 16 tb + ln + kx; the swizzle (pixel >> 1) & 3 depends
-// only on ln + kx (16 tb is a multiple of 8), so the offset is a per-lane
-// part plus 1024 tb bytes.  The epilogue writes pixel 16 tb + ln + 1 the
-// same way.
+// Byte offsets, within a ring row, of the operands a lane touches.  Wave
+// half h owns classes FS_CPW h .. FS_CPW h + FS_CPW - 1; its B fragments are
+// those of the classes FS_CPW h - 1 + i, i < FS_WIN.  The epilogue writes
+// class FS_CPW h + q at st + 1024 q (16 entries of 64 bytes further per class;
+// the swizzle depends on the entry modulo 8 only).
 struct FsLane {
-    int b[3];  // B fragment, tap kx = 0..2, column tile 0
-    int st;    // epilogue store (8 bytes), column tile 0
+    int b[FS_WIN];
+    int st;
 };
-__device__ __forceinline__ FsLane fs_lane(int lane, int mt) {
+__device__ __forceinline__ FsLane fs_lane(int lane, int mt, int half) {
     const int g = lane >> 4, ln = lane & 15;
     FsLane o;
 #pragma unroll
-    for (int kx = 0; kx < 3; kx++) o.b[kx] = fs_chunk(ln + kx, g) * 16;
-    o.st = fs_chunk(ln + 1, 2 * mt + (g >> 1)) * 16 + 8 * (g & 1);
+    for (int i = 0; i < FS_WIN; i++) o.b[i] = fs_chunk(fs_entry(FS_CPW * half - 1 + i + FS_NC * ln), g) * 16;
+    o.st = fs_chunk(FS_CPW * half * 16 + ln, 2 * mt + (g >> 1)) * 16 + 8 * (g & 1);
     return o;
 }
 template <class T>
 __device__ __forceinline__ T lds_at(const void* base, int byte_off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(base) + byte_off);
 }
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+// relu(a + ba), relu(b + bb) rounded to bf16 (nearest even) and packed low/high
+__device__ __forceinline__ uint32_t fs_pack(float a, float ba, float b, float bb) {
+    const f32x2 v = {fmaxf(a + ba, 0.0f), fmaxf(b + bb, 0.0f)};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
 
-// One MFMA layer row: dst row (columns 0..FS_W-1) from three source rows.
+// One MFMA layer row: dst row (the wave's classes) from three source rows.
 // GLOBAL_OUT: store the stored columns to HBM instead of an LDS ring.
 template <bool GLOBAL_OUT>
-__device__ __forceinline__ void fs_layer(const uint16_t* const src[3], uint16_t* dst, const bf16x8 (&wa)[9], const float (&bias)[4], int wave,
-                                         int lane, const FsLane& lo, bool row_in, int gx0, int Wd, int store_lo, int store_hi) {
-    const int g = lane >> 4, ln = lane & 15, mt = wave & 1;
-    const int tb = wave >> 1;  // column tiles tb, tb + 2, tb + 4
-    f32x4 acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-    bf16x8 bcur[3], bnxt[3];
+__device__ __forceinline__ void fs_layer(const uint16_t* const src[3], uint16_t* dst, const bf16x8 (&wa)[9], const float (&bias)[4], int lane,
+                                         int mt, int half, const FsLane& lo, bool row_in, int gx0, int Wd, int store_lo, int store_hi) {
+    const int g = lane >> 4, ln = lane & 15;
+    f32x4 acc[FS_CPW];
 #pragma unroll
-    for (int q = 0; q < 3; q++) bcur[q] = lds_at<bf16x8>(src[0], lo.b[0] + (tb + 2 * q) * 1024);
+    for (int q = 0; q < FS_CPW; q++) acc[q] = f32x4{0, 0, 0, 0};
+    bf16x8 b[3][FS_WIN];
 #pragma unroll
-    for (int t = 0; t < 9; t++) {
-        if (t < 8) {
-            const int ky = (t + 1) / 3, kx = (t + 1) % 3;
+    for (int ky = 0; ky < 3; ky++)
 #pragma unroll
-            for (int q = 0; q < 3; q++) bnxt[q] = lds_at<bf16x8>(src[ky], lo.b[kx] + (tb + 2 * q) * 1024);
-        }
+        for (int i = 0; i < FS_WIN; i++) b[ky][i] = lds_at<bf16x8>(src[ky], lo.b[i]);
 #pragma unroll
-        for (int q = 0; q < 3; q++) acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t], bcur[q], acc[q], 0, 0, 0);
-        if (t < 8) {
+    for (int t = 0; t < 9; t++)
 #pragma unroll
-            for (int q = 0; q < 3; q++) bcur[q] = bnxt[q];
-        }
-    }
+        for (int q = 0; q < FS_CPW; q++) acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t], b[t / 3][q + t % 3], acc[q], 0, 0, 0);
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const int j = (tb + 2 * q) * 16 + ln;
+    for (int q = 0; q < FS_CPW; q++) {
+        const int j = FS_CPW * half + q + FS_NC * ln;
         const int gx = gx0 + j;
         const bool in = row_in && gx >= 0 && gx < Wd;
         const uint32_t keep = in ? 0xffffffffu : 0u;
         uint2 o;
-        o.x = (f2bf(fmaxf(acc[q][0] + bias[0], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[q][1] + bias[1], 0.0f)) << 16)) & keep;
-        o.y = (f2bf(fmaxf(acc[q][2] + bias[2], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[q][3] + bias[3], 0.0f)) << 16)) & keep;
+        o.x = fs_pack(acc[q][0], bias[0], acc[q][1], bias[1]) & keep;
+        o.y = fs_pack(acc[q][2], bias[2], acc[q][3], bias[3]) & keep;
         if (GLOBAL_OUT) {
             if (in && j >= store_lo && j < store_hi) *reinterpret_cast<uint2*>(dst + (size_t)gx * C + mt * 16 + 4 * g) = o;
         } else {
-            *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + (tb + 2 * q) * 1024) = o;
+            *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + q * 1024) = o;
         }
     }
 }
@@ -690,14 +701,22 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
 #pragma unroll
         for (int i = 0; i < 4; i++) bm[l][i] = params[CNN_OFF_MID + (size_t)(l0 + l) * CNN_MID_STRIDE + C * 9 * C + mt * 16 + 4 * g + i];
     }
-    const FsLane lo = fs_lane(lane, mt);
+    const int half = wave >> 1;
+    const FsLane lo = fs_lane(lane, mt, half);
 
-    // input rows: A reads x rows (FS_RW floats), B reads a4 rows (FS_RW pixels of 64 bytes)
-    // row i of the input ring holds global column gx0 - 1 + i
+    // input rows: A reads x rows (FS_RW floats, element i = global column
+    // gx0 - 1 + i), B reads a4 rows (FS_RW pixels of 64 bytes, pixel i =
+    // global column gx0 - 1 + i, stored at ring entry fs_entry(i - 1))
     constexpr int IN_UNITS = PASS == 0 ? FS_RW : FS_RW * 4;  // floats / 16-byte chunks per row
     constexpr int IN_K = (IN_UNITS + FS_NT - 1) / FS_NT;
     float xv[PASS == 0 ? IN_K : 1];
     uint4 av[PASS == 0 ? 1 : IN_K];
+    int cofs[IN_K];  // B: uint4 index of this thread's chunks in a ring row
+#pragma unroll
+    for (int k = 0; k < IN_K; k++) {
+        const int u = tid + k * FS_NT;
+        cofs[k] = fs_chunk(fs_entry((u >> 2) - 1), u & 3);
+    }
     auto fetch = [&](int row) {  // global -> registers
 #pragma unroll
         for (int k = 0; k < IN_K; k++) {
@@ -721,7 +740,7 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
                 if (PASS == 0)
                     xr[slot * FS_RW + u] = xv[k];
                 else
-                    reinterpret_cast<uint4*>(RING(0, slot))[fs_chunk(u >> 2, u & 3)] = av[k];
+                    reinterpret_cast<uint4*>(RING(0, slot))[cofs[k]] = av[k];
             }
         }
     };
@@ -744,13 +763,12 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         fetch(r + 2);
         __syncthreads();
         if (PASS == 0) {
-            // ---- a1 row r into ring 0 (one MFMA per tile, K = taps; as k_conv_in)
+            // ---- a1 row r into ring 0 (one MFMA per class, K = taps; as k_conv_in)
             uint16_t* dst = RING(0, slot(0));
             const bool row_in = r >= 0 && r < H;
-            const int tb = wave >> 1;
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const int j = (tb + 2 * q) * 16 + ln;
+            for (int q = 0; q < FS_CPW; q++) {
+                const int j = FS_CPW * half + q + FS_NC * ln;
                 bf16x8 bv;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
@@ -763,9 +781,9 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
                 const int gx = gx0 + j;
                 const uint32_t keep = (row_in && gx >= 0 && gx < Wd) ? 0xffffffffu : 0u;
                 uint2 o;
-                o.x = (f2bf(fmaxf(acc[0] + bin[0], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[1] + bin[1], 0.0f)) << 16)) & keep;
-                o.y = (f2bf(fmaxf(acc[2] + bin[2], 0.0f)) | ((uint32_t)f2bf(fmaxf(acc[3] + bin[3], 0.0f)) << 16)) & keep;
-                *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + (tb + 2 * q) * 1024) = o;
+                o.x = fs_pack(acc[0], bin[0], acc[1], bin[1]) & keep;
+                o.y = fs_pack(acc[2], bin[2], acc[3], bin[3]) & keep;
+                *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + q * 1024) = o;
             }
             __syncthreads();
         }
@@ -780,29 +798,29 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
             if (PASS == 0 && l == 2) {
                 // a4: stored columns and rows of this block go to HBM
                 if (orow >= y0 && orow < y0 + FS_SH)
-                    fs_layer<true>(src, act_out + ioff * C + (size_t)orow * Wd * C, wa[l], bm[l], wave, lane, lo, row_in, gx0, Wd, FS_HALO,
+                    fs_layer<true>(src, act_out + ioff * C + (size_t)orow * Wd * C, wa[l], bm[l], lane, mt, half, lo, row_in, gx0, Wd, FS_HALO,
                                    FS_HALO + FS_OUT);
             } else {
-                fs_layer<false>(src, RING(l + 1, os), wa[l], bm[l], wave, lane, lo, row_in, gx0, Wd, 0, 0);
+                fs_layer<false>(src, RING(l + 1, os), wa[l], bm[l], lane, mt, half, lo, row_in, gx0, Wd, 0, 0);
                 __syncthreads();
             }
         }
         if (PASS == 1) {
-            // ---- output row r - 3: W_out in row 0 of the A operand, one MFMA per tap (as k_conv_out)
+            // ---- output row r - 3: W_out in row 0 of the A operand, one MFMA per tap (as k_conv_out);
+            // of the wave half's classes, the mt = 0 wave takes the first FS_CPW - 1, the mt = 1 wave the last
             const int orow = r - 3;
             const uint16_t* src[3] = {RING(3, slot(-4)), RING(3, slot(-3)), RING(3, slot(-2))};
             const bool store_row = orow >= y0 && orow < y0 + FS_SH && orow < H;
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const int nt = wave + 4 * q;
-                if (nt < FS_W / 16) {
+            for (int q = 0; q < FS_CPW; q++) {
+                if ((q < FS_CPW - 1) == (mt == 0)) {
                     f32x4 acc = {0, 0, 0, 0};
 #pragma unroll
                     for (int t = 0; t < 9; t++) {
-                        const bf16x8 bv = lds_at<bf16x8>(src[t / 3], lo.b[t % 3] + nt * 1024);
+                        const bf16x8 bv = lds_at<bf16x8>(src[t / 3], lo.b[q + t % 3]);
                         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wout[t], bv, acc, 0, 0, 0);
                     }
-                    const int j = nt * 16 + ln, gx = gx0 + j;
+                    const int j = FS_CPW * half + q + FS_NC * ln, gx = gx0 + j;
                     if (g == 0 && store_row && j >= FS_HALO && j < FS_HALO + FS_OUT && gx < Wd) {
                         const size_t p = ioff + (size_t)orow * Wd + gx;
                         out[p] = x[p] + (acc[0] + b_out);
