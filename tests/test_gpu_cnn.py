@@ -8,7 +8,9 @@
 * the learned denoise's NLM spot check and fallback (forced by thresholds
   and by a corrupted network);
 * training on the GPU lowers the loss, and the trained network gains PSNR
-  over the untrained one against the noise-free frame.
+  over the untrained one against the noise-free frame;
+* the shipped checkpoint against NLM and against no denoise at noise levels
+  1x, 4x and 16x (luminance PSNR against the noise-free frame).
 """
 import sys
 import time
@@ -260,5 +262,33 @@ def test_training_gains_over_untrained(ctx):
         assert hist[-1] < 0.8 * hist[0]
         assert g1 > g0 + 0.2
         assert g1 > 0.0
+    finally:
+        net.close()
+
+
+@pytest.mark.parametrize("iso", [1.0, 4.0, 16.0])
+def test_trained_network_against_nlm_and_no_denoise(ctx, iso):
+    """The shipped checkpoint against NLM (h = 0.6 sigma, 21 x 21 search, 7 x 7
+    patch, the plain path's denoise) and against no denoise: luminance PSNR
+    against the noise-free frame, four seeded 512 x 512 synthetic frames per
+    noise level.  The network must improve on the noisy input and stay
+    within 0.3 dB of NLM at every level."""
+    import os
+    import darkslide_amd as D
+    ck = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "darkslide_amd", "weights", "cnn_l32x8.dsck")
+    net = D.CNN(ctx, seed=1)
+    try:
+        net.load(ck)
+        rows = []
+        for sd in range(4):
+            _, Ln = _frame_L(ctx, 0xC0 + sd, 512, 512, iso)
+            _, Lc = _frame_L(ctx, 0xC0 + sd, 512, 512, 0.0)
+            s = max(np.float32(ctx.estimate_noise(Ln)), np.float32(1e-4))
+            rows.append((_psnr(Ln, Lc), _psnr(net.denoise_plane(Ln), Lc), _psnr(ctx.nlm(Ln, s * np.float32(0.6), 10, 3), Lc)))
+        noisy, cnn, nlm = (float(np.mean(c)) for c in zip(*rows))
+        say(f"  iso {iso:g}: luminance PSNR noisy {noisy:.3f} dB, trained CNN {cnn:.3f} dB ({cnn - noisy:+.3f}), "
+            f"NLM {nlm:.3f} dB ({nlm - noisy:+.3f})")
+        assert cnn > noisy
+        assert cnn >= nlm - 0.3
     finally:
         net.close()
