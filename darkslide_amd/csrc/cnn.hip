@@ -106,14 +106,19 @@ __device__ __forceinline__ void load_px32(const uint16_t* p, float* v) {
     }
 }
 
-// Output layer: y[px] = x[px] + (b_out + sum_{tap, ci} W_out[tap][ci] a7[px + tap][ci]),
-// nine MFMAs (one per tap, K = 32 channels) with W_out in row 0 of the A
-// operand; tile and a7 staging as k_conv_mid.
+// Output layer: y[px] = x[px] + (((E_0[px - 1] + E_1[px]) + E_2[px + 1]) + b_out),
+// where E_kx[q] = sum over (ky, ci) of W_out[ky][kx][ci] a7[q + (ky - 1) rows][ci].
+// Three MFMAs (one per kernel row ky, K = 32 channels, a chain) with
+// W_out[ky][kx] in row kx of the A operand give E_0..E_2 of 16 pixels at once;
+// the fused inference kernel sums in the same order.  Tile and a7 staging as
+// k_conv_mid.
 constexpr int CP_ = 40;
 __global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ act, const float* __restrict__ x, const uint16_t* __restrict__ wout,
                                                   const float* __restrict__ b_out, float* __restrict__ out, int H, int Wd) {
-    constexpr int TW = 64, TH = 4;
+    constexpr int TW = 64, TH = 4, NE = TW + 2;  // E columns x0 - 1 .. x0 + TW
+    constexpr int NET = (NE + 15) / 16;           // E tiles of 16 columns
     __shared__ __attribute__((aligned(16))) uint16_t s_in[(TH + 2) * (TW + 2) * CP_];
+    __shared__ float s_e[TH][3][NET * 16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, ln = lane & 15;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
     const size_t ioff = (size_t)blockIdx.z * H * Wd;
@@ -136,30 +141,35 @@ __global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ a
             if (i < N16) reinterpret_cast<uint4*>(s_in + (i >> 2) * CP_)[i & 3] = r[k];
         }
     }
-    bf16x8 wa[9];
+    bf16x8 wa[3];
 #pragma unroll
-    for (int t = 0; t < 9; t++) {
+    for (int ky = 0; ky < 3; ky++) {
         bf16x8 z = {};
-        if (ln == 0) z = *reinterpret_cast<const bf16x8*>(wout + t * C + 8 * g);
-        wa[t] = z;
+        if (ln < 3) z = *reinterpret_cast<const bf16x8*>(wout + (ky * 3 + ln) * C + 8 * g);
+        wa[ky] = z;
     }
     const float bo = b_out[0];
     __syncthreads();
-    const int oy = y0 + wave;
 #pragma unroll
-    for (int nt = 0; nt < TW / 16; nt++) {
+    for (int nt = 0; nt < NET; nt++) {
+        const int j = nt * 16 + ln;  // E column x0 - 1 + j
         f32x4 acc = {0, 0, 0, 0};
 #pragma unroll
-        for (int t = 0; t < 9; t++) {
-            const int ky = t / 3, kx = t % 3;
-            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(s_in + ((wave + ky) * (TW + 2) + nt * 16 + ln + kx) * CP_ + 8 * g);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t], bv, acc, 0, 0, 0);
+        for (int ky = 0; ky < 3; ky++) {
+            bf16x8 bv = {};
+            if (j < NE) bv = *reinterpret_cast<const bf16x8*>(s_in + ((wave + ky) * (TW + 2) + j) * CP_ + 8 * g);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[ky], bv, acc, 0, 0, 0);
         }
-        const int ox = x0 + nt * 16 + ln;
-        if (g == 0 && oy < H && ox < Wd) {
-            const size_t p = ioff + (size_t)oy * Wd + ox;
-            out[p] = x[p] + (acc[0] + bo);
+        if (g == 0) {
+#pragma unroll
+            for (int kx = 0; kx < 3; kx++) s_e[wave][kx][j] = acc[kx];
         }
+    }
+    __syncthreads();
+    const int ox = x0 + lane, oy = y0 + wave;
+    if (oy < H && ox < Wd) {
+        const size_t p = ioff + (size_t)oy * Wd + ox;
+        out[p] = x[p] + (((s_e[wave][0][lane] + s_e[wave][1][lane + 1]) + s_e[wave][2][lane + 2]) + bo);
     }
 }
 
@@ -618,24 +628,26 @@ __device__ __forceinline__ uint32_t fs_pack(float a, float ba, float b, float bb
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
-// One MFMA layer row: dst row (the wave's classes) from three source rows.
-// GLOBAL_OUT: store the stored columns to HBM instead of an LDS ring.
+// Kernel row ky of one MFMA layer row: taps 3 ky .. 3 ky + 2 of the wave's
+// classes from source row `src`, accumulated into acc (ky = 0, 1, 2 in turn
+// gives the tap order of k_conv_mid).
+__device__ __forceinline__ void fs_mma_row(f32x4 (&acc)[FS_CPW], const bf16x8 (&wa)[9], int ky, const uint16_t* src, const FsLane& lo) {
+    bf16x8 b[FS_WIN];
+#pragma unroll
+    for (int i = 0; i < FS_WIN; i++) b[i] = lds_at<bf16x8>(src, lo.b[i]);
+#pragma unroll
+    for (int kx = 0; kx < 3; kx++)
+#pragma unroll
+        for (int q = 0; q < FS_CPW; q++) acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[3 * ky + kx], b[q + kx], acc[q], 0, 0, 0);
+}
+
+// Epilogue of one MFMA layer row: relu(acc + bias) as bf16 into the dst row
+// (the wave's classes).  GLOBAL_OUT: store the stored columns to HBM instead
+// of an LDS ring.
 template <bool GLOBAL_OUT>
-__device__ __forceinline__ void fs_layer(const uint16_t* const src[3], uint16_t* dst, const bf16x8 (&wa)[9], const float (&bias)[4], int lane,
-                                         int mt, int half, const FsLane& lo, bool row_in, int gx0, int Wd, int store_lo, int store_hi) {
+__device__ __forceinline__ void fs_store(const f32x4 (&acc)[FS_CPW], uint16_t* dst, const float (&bias)[4], int lane, int mt, int half,
+                                         const FsLane& lo, bool row_in, int gx0, int Wd, int store_lo, int store_hi) {
     const int g = lane >> 4, ln = lane & 15;
-    f32x4 acc[FS_CPW];
-#pragma unroll
-    for (int q = 0; q < FS_CPW; q++) acc[q] = f32x4{0, 0, 0, 0};
-    bf16x8 b[3][FS_WIN];
-#pragma unroll
-    for (int ky = 0; ky < 3; ky++)
-#pragma unroll
-        for (int i = 0; i < FS_WIN; i++) b[ky][i] = lds_at<bf16x8>(src[ky], lo.b[i]);
-#pragma unroll
-    for (int t = 0; t < 9; t++)
-#pragma unroll
-        for (int q = 0; q < FS_CPW; q++) acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t], b[t / 3][q + t % 3], acc[q], 0, 0, 0);
 #pragma unroll
     for (int q = 0; q < FS_CPW; q++) {
         const int j = FS_CPW * half + q + FS_NC * ln;
@@ -665,7 +677,8 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     constexpr int NR = PASS == 0 ? 3 : 4;  // LDS rings
     uint16_t* ring = reinterpret_cast<uint16_t*>(fs_lds);
     float* xr = reinterpret_cast<float*>(fs_lds + NR * 3 * FS_ROW_BYTES);  // A: x ring [3][FS_RW]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, ln = lane & 15;
+    const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, ln = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x0 = blockIdx.x * FS_OUT, y0 = blockIdx.y * FS_SH;
     static_assert(FS_SH % 3 == 0, "ring slots follow the row index mod 3");
     const size_t ioff = (size_t)blockIdx.z * H * Wd;
@@ -674,8 +687,8 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     auto RING = [&](int k, int slot) { return ring + (size_t)(k * 3 + slot) * FS_RW * C; };
 
     for (int i = tid; i < (int)(NR * 3 * FS_ROW_BYTES / 16); i += FS_NT) reinterpret_cast<uint4*>(fs_lds)[i] = make_uint4(0, 0, 0, 0);
-    // input / output layer operands (as k_conv_in / k_conv_out)
-    bf16x8 win = {}, wout[9];
+    // input / output layer operands (as k_conv_in / k_conv_out: W_out[ky][kx] in row kx of the A operand)
+    bf16x8 win = {}, wout[3];
     float bin[4] = {0, 0, 0, 0};
     if (PASS == 0) {
         win = *reinterpret_cast<const bf16x8*>(wpk + CNN_WPK_IN + ((wave & 1) * 16 + ln) * 32 + 8 * g);
@@ -683,10 +696,10 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         for (int i = 0; i < 4; i++) bin[i] = params[CNN_OFF_IN + C * 9 + (wave & 1) * 16 + 4 * g + i];
     }
 #pragma unroll
-    for (int t = 0; t < 9; t++) {
+    for (int ky = 0; ky < 3; ky++) {
         bf16x8 z = {};
-        if (PASS == 1 && ln == 0) z = *reinterpret_cast<const bf16x8*>(wpk + CNN_WPK_OUT + t * C + 8 * g);
-        wout[t] = z;
+        if (PASS == 1 && ln < 3) z = *reinterpret_cast<const bf16x8*>(wpk + CNN_WPK_OUT + (ky * 3 + ln) * C + 8 * g);
+        wout[ky] = z;
     }
     const float b_out = params[CNN_OFF_OUT + 9 * C];
     const int l0 = PASS == 0 ? 0 : 3;  // first middle layer of the pass
@@ -755,12 +768,24 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     commit(0);
     fetch(r_first + 1);
 
-    // one step at row r with r mod 3 == SUB
+    // One step at row r with r mod 3 == SUB.  Of the three source rows of a
+    // layer only the last (kernel row ky = 2) is written in the same step, so
+    // the ky = 0, 1 MFMAs of the next layer run before the barrier that
+    // publishes it, beside the epilogue of the layer before; each
+    // accumulator still takes its taps in the order 0 .. 8.
+    constexpr int BASE = PASS == 0 ? -1 : 0;  // first middle layer's output row relative to r
     auto step = [&](auto sub_tag, int r) {
         constexpr int SUB = decltype(sub_tag)::value;
         auto slot = [](int c) { return ((SUB + c) % 3 + 3) % 3; };  // constant-folded
+        auto src_row = [&](int l, int ky) { return RING(l, slot(BASE - l + ky - 1)); };  // layer l, kernel row ky
+        const bool a4_row = PASS == 0 && r + BASE - 2 >= y0 && r + BASE - 2 < y0 + FS_SH;  // A: layer 2 feeds a stored a4 row
         commit(slot(1));
         fetch(r + 2);
+        f32x4 acc[FS_CPW];
+#pragma unroll
+        for (int q = 0; q < FS_CPW; q++) acc[q] = f32x4{0, 0, 0, 0};
+        fs_mma_row(acc, wa[0], 0, src_row(0, 0), lo);
+        fs_mma_row(acc, wa[0], 1, src_row(0, 1), lo);
         __syncthreads();
         if (PASS == 0) {
             // ---- a1 row r into ring 0 (one MFMA per class, K = taps; as k_conv_in)
@@ -777,53 +802,70 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
                     const float v = t < 9 ? xr[slot(ty - 1) * FS_RW + j + tx] : 0.0f;
                     bv[e] = (__bf16)v;
                 }
-                const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(win, bv, f32x4{0, 0, 0, 0}, 0, 0, 0);
+                const f32x4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(win, bv, f32x4{0, 0, 0, 0}, 0, 0, 0);
                 const int gx = gx0 + j;
                 const uint32_t keep = (row_in && gx >= 0 && gx < Wd) ? 0xffffffffu : 0u;
                 uint2 o;
-                o.x = fs_pack(acc[0], bin[0], acc[1], bin[1]) & keep;
-                o.y = fs_pack(acc[2], bin[2], acc[3], bin[3]) & keep;
+                o.x = fs_pack(a[0], bin[0], a[1], bin[1]) & keep;
+                o.y = fs_pack(a[2], bin[2], a[3], bin[3]) & keep;
                 *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + q * 1024) = o;
             }
             __syncthreads();
         }
         // ---- three MFMA layers: ring k -> ring k+1 (A: rows r-1, r-2, r-3; B: rows r, r-1, r-2)
-        constexpr int BASE = PASS == 0 ? -1 : 0;  // first output row relative to r
+        // B: output row r - 3 (as k_conv_out): E_kx of window class i in row kx of a
+        // three-MFMA chain over ky, then y = x + (((E_0(col - 1) + E_1(col)) + E_2(col + 1)) + b_out);
+        // of the wave half's classes, the mt = 0 wave takes the first FS_CPW - 1 (window
+        // classes 0 .. FS_CPW), the mt = 1 wave the last (window classes FS_CPW - 1 .. FS_CPW + 1)
+        f32x4 e[FS_WIN];
+        auto out_row = [&](int ky) {
+            const uint16_t* src = RING(3, slot(-4 + ky));
+#pragma unroll
+            for (int i = 0; i < FS_WIN; i++)
+                if ((mt == 0 && i <= FS_CPW) || (mt == 1 && i >= FS_CPW - 1))
+                    e[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wout[ky], lds_at<bf16x8>(src, lo.b[i]), e[i], 0, 0, 0);
+        };
 #pragma unroll
         for (int l = 0; l < 3; l++) {
             const int orow = r + BASE - l;
-            const int os = slot(BASE - l);
-            const uint16_t* src[3] = {RING(l, slot(BASE - l - 1)), RING(l, os), RING(l, slot(BASE - l + 1))};
             const bool row_in = orow >= 0 && orow < H;
+            const bool on = !(PASS == 0 && l == 2) || a4_row;
+            if (on) fs_mma_row(acc, wa[l], 2, src_row(l, 2), lo);
+            f32x4 nxt[FS_CPW];
+#pragma unroll
+            for (int q = 0; q < FS_CPW; q++) nxt[q] = f32x4{0, 0, 0, 0};
             if (PASS == 0 && l == 2) {
                 // a4: stored columns and rows of this block go to HBM
-                if (orow >= y0 && orow < y0 + FS_SH)
-                    fs_layer<true>(src, act_out + ioff * C + (size_t)orow * Wd * C, wa[l], bm[l], lane, mt, half, lo, row_in, gx0, Wd, FS_HALO,
-                                   FS_HALO + FS_OUT);
+                if (on) fs_store<true>(acc, act_out + ioff * C + (size_t)orow * Wd * C, bm[l], lane, mt, half, lo, row_in, gx0, Wd, FS_HALO, FS_HALO + FS_OUT);
             } else {
-                fs_layer<false>(src, RING(l + 1, os), wa[l], bm[l], lane, mt, half, lo, row_in, gx0, Wd, 0, 0);
+                fs_store<false>(acc, RING(l + 1, slot(BASE - l)), bm[l], lane, mt, half, lo, row_in, gx0, Wd, 0, 0);
+                if (l < 2) {
+                    if (l + 1 < 2 || PASS == 1 || a4_row) {
+                        fs_mma_row(nxt, wa[l + 1], 0, src_row(l + 1, 0), lo);
+                        fs_mma_row(nxt, wa[l + 1], 1, src_row(l + 1, 1), lo);
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < FS_WIN; i++) e[i] = f32x4{0, 0, 0, 0};
+                    out_row(0);
+                    out_row(1);
+                }
                 __syncthreads();
             }
+#pragma unroll
+            for (int q = 0; q < FS_CPW; q++) acc[q] = nxt[q];
         }
         if (PASS == 1) {
-            // ---- output row r - 3: W_out in row 0 of the A operand, one MFMA per tap (as k_conv_out);
-            // of the wave half's classes, the mt = 0 wave takes the first FS_CPW - 1, the mt = 1 wave the last
+            out_row(2);
             const int orow = r - 3;
-            const uint16_t* src[3] = {RING(3, slot(-4)), RING(3, slot(-3)), RING(3, slot(-2))};
             const bool store_row = orow >= y0 && orow < y0 + FS_SH && orow < H;
 #pragma unroll
             for (int q = 0; q < FS_CPW; q++) {
                 if ((q < FS_CPW - 1) == (mt == 0)) {
-                    f32x4 acc = {0, 0, 0, 0};
-#pragma unroll
-                    for (int t = 0; t < 9; t++) {
-                        const bf16x8 bv = lds_at<bf16x8>(src[t / 3], lo.b[q + t % 3]);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wout[t], bv, acc, 0, 0, 0);
-                    }
                     const int j = FS_CPW * half + q + FS_NC * ln, gx = gx0 + j;
                     if (g == 0 && store_row && j >= FS_HALO && j < FS_HALO + FS_OUT && gx < Wd) {
                         const size_t p = ioff + (size_t)orow * Wd + gx;
-                        out[p] = x[p] + (acc[0] + b_out);
+                        out[p] = x[p] + (((e[q][0] + e[q + 1][1]) + e[q + 2][2]) + b_out);
                     }
                 }
             }
