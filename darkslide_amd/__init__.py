@@ -10,6 +10,7 @@ array, a CFA an ``(h, w)`` float32 (normalised) or uint16 (raw DN) array.
 Everything runs on the GPU through the C ABI in include/darkslide_gpu.h.
 There is no CPU fallback: without the built library or a GPU the calls raise.
 """
+import atexit
 import ctypes
 import os
 import threading
@@ -345,6 +346,24 @@ class DeviceBuffer:
             self.ptr = None
 
 
+# Contexts still open when the interpreter exits are closed by an atexit
+# handler, which runs before the HIP runtime's own teardown; left to
+# finalisation, their destructors would call into a runtime already being
+# torn down.
+_live_contexts = weakref.WeakSet()
+
+
+def _close_live_contexts():
+    for c in list(_live_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
+atexit.register(_close_live_contexts)
+
+
 class Context:
     """One GPU context (device + HIP stream) of the C ABI."""
 
@@ -356,6 +375,7 @@ class Context:
             raise DarkslideError(rc, L.ds_last_error(None).decode())
         self.handle = h
         self._nets = weakref.WeakSet()  # networks are destroyed before their context
+        _live_contexts.add(self)
 
     def close(self):
         if self.handle:

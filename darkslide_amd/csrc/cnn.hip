@@ -767,6 +767,7 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     fetch(r_first);
     commit(0);
     fetch(r_first + 1);
+    __syncthreads();  // the first step reads rows r_first - 1 and r_first before its barrier
 
     // One step at row r with r mod 3 == SUB.  Of the three source rows of a
     // layer only the last (kernel row ky = 2) is written in the same step, so

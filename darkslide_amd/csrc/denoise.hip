@@ -4,14 +4,15 @@
 //
 // The guided filters' box blurs use the exact running-sum passes of box.hip
 // (their variance term cancels too much precision for any other summation
-// order to stay within the error budget).  The first horizontal pass computes
-// its six inputs (I, a, b, I a, I b, I I) on the fly; the coefficients and the
-// final a*I + b with the inverse Oklab are pointwise kernels between passes.
+// order to stay within the error budget).  Pointwise kernels form the guide
+// and its products (I, I a, I b, I I), the coefficients, and the final
+// a*I + b with the inverse Oklab between the passes.
 
 #include "box.h"
 #include "common.h"
 #include "libm_ref.h"
 #include "denoise.h"
+#include "guided.h"
 #include "nlm.h"
 
 #pragma clang fp contract(off)
@@ -263,18 +264,12 @@ __global__ void k_gf_coef(const float* __restrict__ mi_, const float* __restrict
                           float* __restrict__ b_b) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float s2 = fmaxr(*sigma_dev, 1e-4f) * 2.0f;
-    const float eps = s2 * s2;
-    const float mi = mi_[i], mpa = mpa_[i], mpb = mpb_[i];
-    const float var = mii_[i] - mi * mi;
-    const float cov_a = mipa_[i] - mi * mpa;
-    const float aa = cov_a / (var + eps);
+    float aa, ba, ab, bb;
+    gf_coef_px(mi_[i], mpa_[i], mpb_[i], mipa_[i], mipb_[i], mii_[i], gf_eps(sigma_dev), aa, ba, ab, bb);
     a_a[i] = aa;
-    b_a[i] = mpa - aa * mi;
-    const float cov_b = mipb_[i] - mi * mpb;
-    const float ab = cov_b / (var + eps);
+    b_a[i] = ba;
     a_b[i] = ab;
-    b_b[i] = mpb - ab * mi;
+    b_b[i] = bb;
 }
 
 // image.rs:404 a*I + b for both planes, then Oklab -> ProPhoto and max(0)
@@ -283,17 +278,7 @@ __global__ void k_gf_final(const float* __restrict__ I_, const float* __restrict
                            const float* __restrict__ ma_b, const float* __restrict__ mb_b, size_t n, OklabM ok, float* __restrict__ rgb) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float I = I_[i];
-    float lab[3] = {I, ma_a[i] * I + mb_a[i], ma_b[i] * I + mb_b[i]};
-    float l[3], cube[3], p[3];
-    mul_mv(ok.m2_inv, lab, l);
-    cube[0] = l[0] * l[0] * l[0];
-    cube[1] = l[1] * l[1] * l[1];
-    cube[2] = l[2] * l[2] * l[2];
-    mul_mv(ok.from_lms, cube, p);
-    rgb[3 * i + 0] = fmaxr(p[0], 0.0f);
-    rgb[3 * i + 1] = fmaxr(p[1], 0.0f);
-    rgb[3 * i + 2] = fmaxr(p[2], 0.0f);
+    gf_final_px(I_[i], ma_a[i], mb_a[i], ma_b[i], mb_b[i], ok, rgb + 3 * i);
 }
 
 }  // namespace

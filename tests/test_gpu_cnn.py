@@ -89,6 +89,20 @@ def test_fused_forward_equals_layer_by_layer(net, shape):
     assert np.array_equal(yf, y), np.abs(yf - y).max()
 
 
+@pytest.mark.parametrize("shape", [(1, 600, 700), (2, 400, 181)])
+def test_fused_forward_repeats_across_row_blocks(net, shape):
+    """Repeated fused inference over several 192-row blocks gives the same
+    bits every time, equal to the layer-by-layer forward: each block's first
+    step reads the input rows its prologue committed only after a barrier."""
+    say("cnn fused forward repeated", shape)
+    net.set_params(_params(23))
+    x = np.random.default_rng(24).normal(0, 2, shape).astype(np.float32)
+    y = net.forward(x)
+    for _ in range(4):
+        yf = net.infer(x)
+        assert np.array_equal(yf, y), np.argwhere(yf != y)[:4]
+
+
 @pytest.mark.parametrize("shape,margin", [((2, 19, 70), 2), ((1, 40, 64), 0)])
 def test_gradient_matches_oracle(net, oracle_mod, shape, margin):
     say("cnn gradient", shape, margin)
