@@ -13,7 +13,12 @@ namespace ds {
 // (decode.rs:26-28).
 struct Bayer {
     uint8_t c[4];
-    __host__ __device__ __forceinline__ int at(long x, long y) const { return c[((y & 1) << 1) | (x & 1)]; }
+    // the four entries packed in a word and shifted out, so that a runtime
+    // site index never makes the compiler keep the array in scratch memory
+    __host__ __device__ __forceinline__ int at(long x, long y) const {
+        const uint32_t u = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
+        return (int)((u >> (8 * (((y & 1) << 1) | (x & 1)))) & 0xffu);
+    }
     // parity (x + y) & 1 of the non-green sites; Bayer greens form a checkerboard
     __host__ __device__ __forceinline__ int rb_parity() const { return c[0] == 1 ? 1 : 0; }
 };

@@ -48,13 +48,18 @@ struct CfaSrc {
     Bayer pat;
     int w, h;
 
+    // white[c], mul[c] by selects: a runtime index into the arrays would put
+    // this struct in scratch memory
+    __device__ __forceinline__ float white_of(int c) const { return c == 0 ? white[0] : (c == 1 ? white[1] : white[2]); }
+    __device__ __forceinline__ float mul_of(int c) const { return c == 0 ? mul[0] : (c == 1 ? mul[1] : mul[2]); }
+
     // normalised CFA value at an in-image pixel: decode.rs:300-303, with the
     // black level black_at(x, y) of a table of period <= 2
     __device__ __forceinline__ float cfa(int x, int y, int c) const {
         if (f32) return f32[(size_t)y * stride + x];
         const float raw = (float)u16[(size_t)y * stride + x];
         const float b = (y & 1) ? ((x & 1) ? blk[3] : blk[2]) : ((x & 1) ? blk[1] : blk[0]);
-        return (raw - b) / fmaxr(white[c] - b, 1.0f);
+        return (raw - b) / fmaxr(white_of(c) - b, 1.0f);
     }
 };
 
@@ -149,7 +154,9 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
 
     auto in_img = [&](int gx, int gy) { return INTERIOR || (gx >= 0 && gx < w && gy >= 0 && gy < h); };
     auto in_box = [&](int gx, int gy, int b) { return INTERIOR || (gx >= b && gx < w - b && gy >= b && gy < h - b); };
-    auto cval = [&](int x, int y) { return fmaxr(cm(x, y), 0.0f); };
+    // RCD reads c = max(cm, 0); MHC (frame-border pixels only) reads cm.  An
+    // interior tile never runs MHC, so it stores c itself.
+    auto cval = [&](int x, int y) { return INTERIOR ? cm(x, y) : fmaxr(cm(x, y), 0.0f); };
 
     // ---- stage 0: load cfa*mul (and clip flags) with clamped coordinates
     {
@@ -162,7 +169,8 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
             [&](int i, float v) {
                 int tx = i % CW - H_CM, ty = i / CW - H_CM;
                 int gx = clampi(ox + tx, 0, w - 1), gy = clampi(oy + ty, 0, h - 1);
-                cm(tx, ty) = v * src.mul[pat.at(gx, gy)];
+                const float m = v * src.mul_of(pat.at(gx, gy));
+                cm(tx, ty) = INTERIOR ? fmaxr(m, 0.0f) : m;
                 if (clip && tx >= -H_FLAG && tx < TX + H_FLAG && ty >= -H_FLAG && ty < TY + H_FLAG)
                     L.flag[(ty + H_FLAG) * Tile<TX, TY, H_FLAG>::W + (tx + H_FLAG)] = v >= kClip ? 1 : 0;
             });
@@ -418,7 +426,7 @@ __global__ __launch_bounds__(NT) void k_mhc(CfaSrc src, float* __restrict__ rgb,
             [&](int i, float v) {
                 int tx = i % CW - H, ty = i / CW - H;
                 int gx = clampi(ox + tx, 0, w - 1), gy = clampi(oy + ty, 0, h - 1);
-                cm(tx, ty) = v * src.mul[pat.at(gx, gy)];
+                cm(tx, ty) = v * src.mul_of(pat.at(gx, gy));
                 if (clip && tx >= -1 && tx <= TX && ty >= -1 && ty <= TY) s_flag[(ty + 1) * Tile<TX, TY, 1>::W + (tx + 1)] = v >= kClip ? 1 : 0;
             });
     }
