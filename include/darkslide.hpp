@@ -87,14 +87,20 @@ class Gpu {
 inline const float* flat(const std::vector<Rgb>& v) { return v.empty() ? nullptr : v.front().data(); }
 inline float* flat(std::vector<Rgb>& v) { return v.empty() ? nullptr : v.front().data(); }
 
-// demosaic.rs:20.  Full quality on a Bayer CFA runs RCD (demosaic.rs:147)
-// and the clip plane (demosaic.rs:31) on the GPU.  The Draft superpixel
-// path and X-Trans are outside this library and throw.
+// demosaic.rs:20.  On a Bayer CFA, Full quality runs RCD (demosaic.rs:147)
+// and the clip plane (demosaic.rs:31), Draft the superpixel
+// (demosaic.rs:44-87, one pixel per 2x2 tile), on the GPU.  X-Trans is
+// outside this library and throws.
 inline std::pair<Image3, Plane> demosaic(const CfaImage& cfa, Rgb mul, Quality quality) {
-    if (quality != Quality::Full || !cfa.pattern.is_bayer())
-        throw Error(DS_ERR_INVALID, "only Quality::Full on a Bayer CFA runs on the GPU");
+    if (!cfa.pattern.is_bayer()) throw Error(DS_ERR_INVALID, "only Bayer CFAs run on the GPU");
     uint8_t pat[4];
     for (int i = 0; i < 4; i++) pat[i] = (uint8_t)cfa.pattern.color_at((size_t)(i & 1), (size_t)(i >> 1));
+    if (quality == Quality::Draft) {
+        Image3 img(cfa.w / 2, cfa.h / 2);
+        Plane clip(cfa.w / 2, cfa.h / 2);
+        Gpu::check(ds_superpixel_f32(Gpu::ctx(), cfa.data.data(), cfa.w, cfa.h, pat, mul.data(), flat(img.data), clip.data.data()));
+        return {std::move(img), std::move(clip)};
+    }
     Image3 img(cfa.w, cfa.h);
     Plane clip(cfa.w, cfa.h);
     Gpu::check(ds_demosaic_f32(Gpu::ctx(), cfa.data.data(), cfa.w, cfa.h, pat, mul.data(), DS_ALGO_RCD, flat(img.data), clip.data.data()));

@@ -1,6 +1,7 @@
 // Drives the C++ mirror (include/darkslide.hpp) the way the reference's
 // Engine::base_for drives the Rust functions (pipeline/mod.rs:229-252):
-// CFA -> demosaic -> denoise -> raw_details.  Writes each stage's output as
+// CFA -> demosaic -> denoise -> raw_details, and the Draft demosaic.  Writes
+// each stage's output as
 // raw float32 files into the directory given as argv[1], for the GPU test to
 // compare against the oracle.
 #include <cstdio>
@@ -41,7 +42,10 @@ int main(int argc, char** argv) {
         for (size_t i = 0; i < w * h; i++) L.data[i] = img.data[i][1];
         float sigma = darkslide::estimate_noise(L);
         darkslide::Plane lf = darkslide::nlm(L, sigma * 0.6f, 10, 3);
-        bool ok = dump(dir + "/cfa.f32", cfa.data.data(), w * h) && dump(dir + "/rgb.f32", darkslide::flat(img.data), 3 * w * h) &&
+        auto [draft, dclip] = darkslide::demosaic(cfa, {2.0f, 1.0f, 1.5f}, darkslide::Quality::Draft);
+        bool ok = dump(dir + "/draft.f32", darkslide::flat(draft.data), 3 * (w / 2) * (h / 2)) &&
+                  dump(dir + "/draft_clip.f32", dclip.data.data(), (w / 2) * (h / 2)) &&
+                  dump(dir + "/cfa.f32", cfa.data.data(), w * h) && dump(dir + "/rgb.f32", darkslide::flat(img.data), 3 * w * h) &&
                   dump(dir + "/clip.f32", clip.data.data(), w * h) && dump(dir + "/denoise.f32", darkslide::flat(dn.data), 3 * w * h) &&
                   dump(dir + "/rl.f32", darkslide::flat(rl.data), 3 * w * h) && dump(dir + "/nlm.f32", lf.data.data(), w * h);
         std::printf("sigma %.9g\n", (double)sigma);

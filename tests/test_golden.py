@@ -42,3 +42,8 @@ def test_gpu_matches_golden(ctx, name):
     assert maxabs(ctx.nlm(g["L"], g["sigma"] * np.float32(1.5), 3, 1), g["nlm_3_1"]) <= 2e-5
     assert maxabs(ctx.nlm(g["L"], g["sigma"] * np.float32(0.6), 10, 3), g["nlm_10_3"]) <= 2e-5
     assert np.array_equal(ctx.raw_details(g["rcd"], 1.0, 20), g["raw_details_1_20"])
+    # the fixtures' NLM and denoise were written by the oracle with one thread:
+    # the reference-order mode reproduces them bit for bit
+    assert np.array_equal(ctx.nlm_reference_order(g["L"], g["sigma"] * np.float32(1.5), 3, 1, 1), g["nlm_3_1"])
+    assert np.array_equal(ctx.nlm_reference_order(g["L"], g["sigma"] * np.float32(0.6), 10, 3, 1), g["nlm_10_3"])
+    assert np.array_equal(ctx.denoise_reference_order(g["rcd"], 0.0, 10, 3, 1), g["denoise_10_3"])

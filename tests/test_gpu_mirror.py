@@ -1,6 +1,6 @@
 """The C++ mirror of the reference API (include/darkslide.hpp) end to end:
-tests/cpp/mirror_smoke runs demosaic -> denoise -> raw_details through it and
-the stages are compared with the oracle."""
+tests/cpp/mirror_smoke runs demosaic -> denoise -> raw_details (and the Draft
+demosaic) through it and the stages are compared with the oracle."""
 import os
 import subprocess
 import sys
@@ -27,6 +27,9 @@ def test_cpp_mirror_pipeline(oracle_mod, tmp_path):
     rgb, clip = ld("rgb.f32", h, w, 3), ld("clip.f32", h, w)
     ref, rclip = O.demosaic(cfa)
     assert maxabs(rgb, ref) <= TOL and np.array_equal(clip, rclip)
+    # Quality::Draft: the superpixel, bit for bit
+    drgb, dclip = O.superpixel(cfa, (0, 1, 1, 2), (2.0, 1.0, 1.5))
+    assert np.array_equal(ld("draft.f32", h // 2, w // 2, 3), drgb) and np.array_equal(ld("draft_clip.f32", h // 2, w // 2), dclip)
     dn = ld("denoise.f32", h, w, 3)
     dref = O.denoise(ref, 0.0, 10, 3)
     env = max(maxabs(O.denoise(ref, 0.0, 10, 3, nthreads=t), dref) for t in (1, 4, 16))
