@@ -784,13 +784,11 @@ int cnn_plane_dev(ds_ctx* c, ds_cnn* net, const float* L, size_t w, size_t h, co
         net->packed = true;
     }
     SLOT(c, x, float*, S_X, n * 4);
-    SLOT(c, o, float*, S_T, n * 4);
     int rc = run(c, "cnn_input", [&] { return cnn_input(L, (int)w, (int)h, sig, x, c->stream); });
     if (rc) return rc;
     SLOT(c, act, uint16_t*, S_ACT, cnn_fused_act_bytes(1, (int)h, (int)w));
-    rc = run(c, "cnn_forward", [&] { return cnn_forward_fused(net->params(), net->wpk, x, 1, (int)h, (int)w, act, o, c->stream); });
-    if (rc) return rc;
-    return run(c, "cnn_output", [&] { return cnn_output(o, x, L, n, sig, lf, c->stream); });
+    // the output pass forms LF = L + s (y - x) itself
+    return run(c, "cnn_forward", [&] { return cnn_forward_fused(net->params(), net->wpk, x, 1, (int)h, (int)w, act, lf, c->stream, L, sig); });
 }
 
 // denoise with the learned luminance stand-in for NLM, the per-region NLM
@@ -1072,7 +1070,6 @@ void frame_needs(const ds_frame& f, const FramePlan& p, size_t need[S_COUNT]) {
             const size_t nreg = nlm_regions(p.w, p.h);
             up(S_REGIONS, (2 * nreg + 64 + nlm_spot_scratch_floats(p.w, p.h)) * 4);
             up(S_X, n * 4);
-            up(S_T, n * 4);
             up(S_ACT, cnn_fused_act_bytes(1, (int)p.h, (int)p.w));
         }
     }

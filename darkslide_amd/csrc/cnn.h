@@ -31,10 +31,12 @@ hipError_t cnn_pack(const float* params, uint16_t* wpk, hipStream_t st);
 hipError_t cnn_forward(const float* params, const uint16_t* wpk, const float* x, int n_img, int H, int W, uint16_t* const* acts, float* out,
                        hipStream_t st);
 // Inference forward in two fused streaming passes (activation a4 in HBM,
-// cnn_fused_act_bytes); equals cnn_forward bit for bit.
+// cnn_fused_act_bytes); equals cnn_forward bit for bit.  With lum set, out
+// receives lum + max(*sigma_dev, 1e-4) (y - x), the denoised luminance,
+// instead of y.
 size_t cnn_fused_act_bytes(int n_img, int H, int W);
 hipError_t cnn_forward_fused(const float* params, const uint16_t* wpk, const float* x, int n_img, int H, int W, uint16_t* act, float* out,
-                             hipStream_t st);
+                             hipStream_t st, const float* lum = nullptr, const float* sigma_dev = nullptr);
 size_t cnn_wgrad_blocks(int n_img, int H, int W);
 size_t cnn_scratch_floats(int n_img, int H, int W);
 hipError_t cnn_backward(const float* params, const uint16_t* wpk, const float* x, const float* tgt, int n_img, int H, int W, int margin,
@@ -43,7 +45,6 @@ hipError_t cnn_backward(const float* params, const uint16_t* wpk, const float* x
 hipError_t cnn_adam(float* params, const float* grad, float* m, float* v, int step, float lr, hipStream_t st);
 hipError_t cnn_input(const float* L, int w, int h, const float* sigma_dev, float* x, hipStream_t st);
 hipError_t cnn_target(const float* x, const float* L, const float* LF, size_t n, const float* sigma_dev, float* t, hipStream_t st);
-hipError_t cnn_output(const float* out, const float* x, const float* L, size_t n, const float* sigma_dev, float* lf, hipStream_t st);
 hipError_t cnn_crops(const float* a, const float* b, int w, int h, int c, int n, uint64_t seed, float* ca, float* cb, hipStream_t st);
 
 }  // namespace ds

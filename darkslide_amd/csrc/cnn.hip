@@ -507,15 +507,32 @@ __global__ void k_pack_mid(const float* __restrict__ wmid, uint16_t* __restrict_
 
 // x = clamp((L - mean5x5(L)) / s, -64, 64) with s = max(sigma, 1e-4); the
 // 5x5 mean uses clamped coordinates.
-__global__ void k_cnn_input(const float* __restrict__ L, int w, int h, const float* __restrict__ sigma_dev, float* __restrict__ x) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)w * h) return;
-    const int px = (int)(i % w), py = (int)(i / w);
-    float s = 0.0f;
-    for (int dy = -2; dy <= 2; dy++)
-        for (int dx = -2; dx <= 2; dx++) s += L[(size_t)clampi(py + dy, 0, h - 1) * w + clampi(px + dx, 0, w - 1)];
+// x = clamp((L - mean5x5(L)) / s, +-64) with clamped edges; the 5x5 window
+// is summed row by row, left to right, from a 64x16 tile plus its 2-pixel
+// halo staged in LDS.
+constexpr int CI_TW = 64, CI_TH = 16;
+__global__ __launch_bounds__(256) void k_cnn_input(const float* __restrict__ L, int w, int h, const float* __restrict__ sigma_dev,
+                                                   float* __restrict__ x) {
+    constexpr int SW = CI_TW + 4, SH = CI_TH + 4;
+    __shared__ float t[SH][SW + 1];
+    const int x0 = blockIdx.x * CI_TW, y0 = blockIdx.y * CI_TH;
+    for (int i = threadIdx.x; i < SW * SH; i += 256) {
+        const int u = i % SW, v = i / SW;
+        t[v][u] = L[(size_t)clampi(y0 + v - 2, 0, h - 1) * w + clampi(x0 + u - 2, 0, w - 1)];
+    }
+    __syncthreads();
     const float sg = fmaxf(*sigma_dev, 1e-4f);
-    x[i] = fminf(fmaxf((L[i] - s * (1.0f / 25.0f)) / sg, -64.0f), 64.0f);
+    const int u = threadIdx.x % CI_TW;
+    for (int v = threadIdx.x / CI_TW; v < CI_TH; v += 256 / CI_TW) {
+        const int px = x0 + u, py = y0 + v;
+        if (px >= w || py >= h) continue;
+        float s = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < 5; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 5; dx++) s += t[v + dy][u + dx];
+        x[(size_t)py * w + px] = fminf(fmaxf((t[v + 2][u + 2] - s * (1.0f / 25.0f)) / sg, -64.0f), 64.0f);
+    }
 }
 
 // target of training: x + (LF - L) / s
@@ -527,12 +544,6 @@ __global__ void k_cnn_target(const float* __restrict__ x, const float* __restric
 }
 
 // lf = L + s * (out - x)
-__global__ void k_cnn_output(const float* __restrict__ out, const float* __restrict__ x, const float* __restrict__ L, size_t n,
-                             const float* __restrict__ sigma_dev, float* __restrict__ lf) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    lf[i] = L[i] + fmaxf(*sigma_dev, 1e-4f) * (out[i] - x[i]);
-}
 
 // cut n_crops crops of c x c from two w x h planes at seeded positions
 __global__ void k_crops(const float* __restrict__ a, const float* __restrict__ b, int w, int h, int c, int n_crops, uint64_t seed,
@@ -669,10 +680,13 @@ __device__ __forceinline__ void fs_store(const f32x4 (&acc)[FS_CPW], uint16_t* d
 // The row loop is unrolled by three (y0 and the first row are multiples of
 // three), so every ring slot ((row) mod 3) is a compile-time constant and
 // the LDS addresses are a lane's fixed offsets plus immediates.
+// B with lum != nullptr: out receives lum + max(*sigma_dev, 1e-4) (y - x), the
+// denoised luminance, in place of y.
 template <int PASS>
 __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict__ x, const uint16_t* __restrict__ act_in,
                                                         const float* __restrict__ params, const uint16_t* __restrict__ wpk,
-                                                        uint16_t* __restrict__ act_out, float* __restrict__ out, int H, int Wd) {
+                                                        uint16_t* __restrict__ act_out, float* __restrict__ out, int H, int Wd,
+                                                        const float* __restrict__ lum, const float* __restrict__ sigma_dev) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fs_lds[];
     constexpr int NR = PASS == 0 ? 3 : 4;  // LDS rings
     uint16_t* ring = reinterpret_cast<uint16_t*>(fs_lds);
@@ -702,6 +716,7 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         wout[ky] = z;
     }
     const float b_out = params[CNN_OFF_OUT + 9 * C];
+    const float s_lum = PASS == 1 && lum ? fmaxf(*sigma_dev, 1e-4f) : 0.0f;
     const int l0 = PASS == 0 ? 0 : 3;  // first middle layer of the pass
     const int mt = wave & 1;
     bf16x8 wa[3][9];
@@ -782,6 +797,21 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         const bool a4_row = PASS == 0 && r + BASE - 2 >= y0 && r + BASE - 2 < y0 + FS_SH;  // A: layer 2 feeds a stored a4 row
         commit(slot(1));
         fetch(r + 2);
+        // B: the output pixels of this step (row r - 3) and their x (and L),
+        // loaded now so their latency passes under the step's layers
+        bool o_on[FS_CPW];
+        size_t o_p[FS_CPW];
+        float o_x[FS_CPW], o_l[FS_CPW];
+#pragma unroll
+        for (int q = 0; q < FS_CPW; q++) {
+            const int orow = r - 3;
+            const int j = FS_CPW * half + q + FS_NC * ln, gx = gx0 + j;
+            o_on[q] = PASS == 1 && (q < FS_CPW - 1) == (mt == 0) && g == 0 && orow >= y0 && orow < y0 + FS_SH && orow < H && j >= FS_HALO &&
+                      j < FS_HALO + FS_OUT && gx < Wd;
+            o_p[q] = o_on[q] ? ioff + (size_t)orow * Wd + gx : 0;
+            o_x[q] = o_on[q] ? x[o_p[q]] : 0.0f;
+            o_l[q] = o_on[q] && lum ? lum[o_p[q]] : 0.0f;
+        }
         f32x4 acc[FS_CPW];
 #pragma unroll
         for (int q = 0; q < FS_CPW; q++) acc[q] = f32x4{0, 0, 0, 0};
@@ -858,16 +888,11 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         }
         if (PASS == 1) {
             out_row(2);
-            const int orow = r - 3;
-            const bool store_row = orow >= y0 && orow < y0 + FS_SH && orow < H;
 #pragma unroll
             for (int q = 0; q < FS_CPW; q++) {
-                if ((q < FS_CPW - 1) == (mt == 0)) {
-                    const int j = FS_CPW * half + q + FS_NC * ln, gx = gx0 + j;
-                    if (g == 0 && store_row && j >= FS_HALO && j < FS_HALO + FS_OUT && gx < Wd) {
-                        const size_t p = ioff + (size_t)orow * Wd + gx;
-                        out[p] = x[p] + (((e[q][0] + e[q + 1][1]) + e[q + 2][2]) + b_out);
-                    }
+                if (o_on[q]) {
+                    const float y = o_x[q] + (((e[q][0] + e[q + 1][1]) + e[q + 2][2]) + b_out);
+                    out[o_p[q]] = lum ? o_l[q] + s_lum * (y - o_x[q]) : y;
                 }
             }
         }
@@ -911,7 +936,7 @@ hipError_t cnn_forward(const float* params, const uint16_t* wpk, const float* x,
 size_t cnn_fused_act_bytes(int n_img, int H, int W) { return (size_t)n_img * H * W * C * sizeof(uint16_t); }
 
 hipError_t cnn_forward_fused(const float* params, const uint16_t* wpk, const float* x, int n_img, int H, int W, uint16_t* act, float* out,
-                             hipStream_t st) {
+                             hipStream_t st, const float* lum, const float* sigma_dev) {
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void*)k_cnn_fused<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)FS_LDS_A);
@@ -920,8 +945,10 @@ hipError_t cnn_forward_fused(const float* params, const uint16_t* wpk, const flo
         attr_set = true;
     }
     dim3 grid((unsigned)((W + FS_OUT - 1) / FS_OUT), (unsigned)((H + FS_SH - 1) / FS_SH), (unsigned)n_img);
-    hipLaunchKernelGGL(k_cnn_fused<0>, grid, dim3(FS_NT), FS_LDS_A, st, x, (const uint16_t*)nullptr, params, wpk, act, (float*)nullptr, H, W);
-    hipLaunchKernelGGL(k_cnn_fused<1>, grid, dim3(FS_NT), FS_LDS_B, st, x, (const uint16_t*)act, params, wpk, (uint16_t*)nullptr, out, H, W);
+    hipLaunchKernelGGL(k_cnn_fused<0>, grid, dim3(FS_NT), FS_LDS_A, st, x, (const uint16_t*)nullptr, params, wpk, act, (float*)nullptr, H, W,
+                       (const float*)nullptr, (const float*)nullptr);
+    hipLaunchKernelGGL(k_cnn_fused<1>, grid, dim3(FS_NT), FS_LDS_B, st, x, (const uint16_t*)act, params, wpk, (uint16_t*)nullptr, out, H, W, lum,
+                       sigma_dev);
     return hipGetLastError();
 }
 
@@ -985,17 +1012,14 @@ hipError_t cnn_adam(float* params, const float* grad, float* m, float* v, int st
 }
 
 hipError_t cnn_input(const float* L, int w, int h, const float* sigma_dev, float* x, hipStream_t st) {
-    hipLaunchKernelGGL(k_cnn_input, dim3(nb((size_t)w * h)), dim3(256), 0, st, L, w, h, sigma_dev, x);
+    if (w <= 0 || h <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((w + CI_TW - 1) / CI_TW), (unsigned)((h + CI_TH - 1) / CI_TH));
+    hipLaunchKernelGGL(k_cnn_input, grid, dim3(256), 0, st, L, w, h, sigma_dev, x);
     return hipGetLastError();
 }
 
 hipError_t cnn_target(const float* x, const float* L, const float* LF, size_t n, const float* sigma_dev, float* t, hipStream_t st) {
     hipLaunchKernelGGL(k_cnn_target, dim3(nb(n)), dim3(256), 0, st, x, L, LF, n, sigma_dev, t);
-    return hipGetLastError();
-}
-
-hipError_t cnn_output(const float* out, const float* x, const float* L, size_t n, const float* sigma_dev, float* lf, hipStream_t st) {
-    hipLaunchKernelGGL(k_cnn_output, dim3(nb(n)), dim3(256), 0, st, out, x, L, n, sigma_dev, lf);
     return hipGetLastError();
 }
 

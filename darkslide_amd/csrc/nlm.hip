@@ -327,6 +327,7 @@ constexpr int SPOT_PER_WAVE = 4;
 constexpr int SPOT_BLOCKS = NLM_SPOT * NLM_SPOT / (SPOT_WAVES * SPOT_PER_WAVE);
 constexpr int SPOT_PARTS = SPOT_BLOCKS * SPOT_WAVES;  // (e^2, var) pairs per region
 constexpr int SPOT_MAX_SP = 24;
+constexpr int SPOT_PF = 12;  // prefetched samples per lane: neighbourhoods up to 27 x 27 (S + P <= 13)
 
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
@@ -344,29 +345,65 @@ __global__ __launch_bounds__(SPOT_WAVES * 64) void k_nlm_spot_lds(const float* _
     const int rx = blockIdx.y, ry = blockIdx.z;
     const int x0 = rx * NLM_REGION, y0 = ry * NLM_REGION;
     const int rw = min(NLM_REGION, w - x0), rh = min(NLM_REGION, h - y0);
-    const float inv_h2 = 1.0f / wp.h2();
+    // exp(-(sum of squared differences) norm^2 / h^2) as exp2 of the sum times
+    // one factor; the check only sizes a deviation, so it takes fused
+    // multiply-adds and the hardware exp2 rather than the reference's
+    // operation order
+    const float kexp = -(norm * norm) * 1.44269504088896340736f / wp.h2();
     const float sg = wp.sigma_dev ? fmaxf(*wp.sigma_dev, 1e-4f) : 1e-4f;
     const int side = 2 * S + 1, noff = side * side;
     float se = 0.0f, sv = 0.0f;
-    for (int k = 0; k < SPOT_PER_WAVE; k++) {
+    // Each wave owns its nb region, so the waves need no block barrier: a
+    // wave's LDS accesses complete in program order.  A neighbourhood of up
+    // to 64 SPOT_PF samples is fetched into registers one lattice pixel
+    // ahead, under the previous pixel's offsets.
+    auto pixel = [&](int k, int& x, int& y) {
         const int sidx = (blockIdx.x * SPOT_WAVES + wave) * SPOT_PER_WAVE + k;
         const int i = sidx % NLM_SPOT, j = sidx / NLM_SPOT;
-        const int x = x0 + ((2 * i + 1) * rw) / (2 * NLM_SPOT), y = y0 + ((2 * j + 1) * rh) / (2 * NLM_SPOT);
+        x = x0 + ((2 * i + 1) * rw) / (2 * NLM_SPOT);
+        y = y0 + ((2 * j + 1) * rh) / (2 * NLM_SPOT);
+    };
+    const bool pf = D * D <= 64 * SPOT_PF;
+    float nxt[SPOT_PF];
+    auto fetch = [&](int k) {
+        int x, y;
+        pixel(k, x, y);
+#pragma unroll
+        for (int q = 0; q < SPOT_PF; q++) {
+            const int t = lane + 64 * q;
+            nxt[q] = t < D * D ? in[(size_t)clampi(y + t / D - SP, 0, h - 1) * w + clampi(x + t % D - SP, 0, w - 1)] : 0.0f;
+        }
+    };
+    if (pf) fetch(0);
+    for (int k = 0; k < SPOT_PER_WAVE; k++) {
+        int x, y;
+        pixel(k, x, y);
         // nb[(v + SP) * D + (u + SP)] = p[clamp(x + u), clamp(y + v)]
-        for (int base = 0; base < D * D; base += 64 * 8) {
-            float r[8];
+        if (pf) {
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int t = base + lane + 64 * q;
-                r[q] = t < D * D ? in[(size_t)clampi(y + t / D - SP, 0, h - 1) * w + clampi(x + t % D - SP, 0, w - 1)] : 0.0f;
+            for (int q = 0; q < SPOT_PF; q++) {
+                const int t = lane + 64 * q;
+                if (t < D * D) nb[t] = nxt[q];
             }
+        } else {
+            for (int base = 0; base < D * D; base += 64 * 8) {
+                float r[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int t = base + lane + 64 * q;
-                if (t < D * D) nb[t] = r[q];
+                for (int q = 0; q < 8; q++) {
+                    const int t = base + lane + 64 * q;
+                    r[q] = t < D * D ? in[(size_t)clampi(y + t / D - SP, 0, h - 1) * w + clampi(x + t % D - SP, 0, w - 1)] : 0.0f;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int t = base + lane + 64 * q;
+                    if (t < D * D) nb[t] = r[q];
+                }
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (pf && k + 1 < SPOT_PER_WAVE) fetch(k + 1);
         const bool interior = x - SP >= 0 && y - SP >= 0 && x + SP < w && y + SP < h;
         // centre patch, as the two-level clamp sees it: window pixel (cx, cy) = clamp(x + i, y + j)
         float cpat[2 * P + 1][2 * P + 1];
@@ -387,31 +424,25 @@ __global__ __launch_bounds__(SPOT_WAVES * 64) void k_nlm_spot_lds(const float* _
             if (interior) {
                 const float* q = nb + (SP - P + dy) * D + (SP - P + dx);
 #pragma unroll
-                for (int jj = 0; jj < 2 * P + 1; jj++) {
-                    float row = 0.0f;
+                for (int jj = 0; jj < 2 * P + 1; jj++)
 #pragma unroll
                     for (int ii = 0; ii < 2 * P + 1; ii++) {
                         const float e = cpat[jj][ii] - q[jj * D + ii];
-                        row += e * e;
+                        dist = __builtin_fmaf(e, e, dist);
                     }
-                    dist += row * norm;
-                }
             } else {
 #pragma unroll
                 for (int jj = 0; jj < 2 * P + 1; jj++) {
                     const int sy = clampi(y + cys[jj] + dy, 0, h - 1) - y + SP;
-                    float row = 0.0f;
 #pragma unroll
                     for (int ii = 0; ii < 2 * P + 1; ii++) {
                         const int sx = clampi(x + cxs[ii] + dx, 0, w - 1) - x + SP;
                         const float e = cpat[jj][ii] - nb[sy * D + sx];
-                        row += e * e;
+                        dist = __builtin_fmaf(e, e, dist);
                     }
-                    dist += row * norm;
                 }
             }
-            dist *= norm;
-            const float wt = expf(-(dist * inv_h2));
+            const float wt = __builtin_amdgcn_exp2f(dist * kexp);
             acc += wt * nb[(dy + SP) * D + dx + SP];
             ws += wt;
             w2 += wt * wt;
@@ -423,7 +454,10 @@ __global__ __launch_bounds__(SPOT_WAVES * 64) void k_nlm_spot_lds(const float* _
         const float e = cand[(size_t)y * w + x] - v;
         se += e * e;
         sv += sg * sg * w2 / fmaxf(ws * ws, 1e-30f);
-        __syncthreads();  // nb is restaged next
+        // every lane's reads of nb are done before it is restaged
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (lane == 0) {
         float* pp = part + ((size_t)(ry * gridDim.y + rx) * SPOT_PARTS + blockIdx.x * SPOT_WAVES + wave) * 2;
