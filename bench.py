@@ -334,9 +334,9 @@ def batched_leg(ctx, args):
             entry["regions"] = last["regions"]
         res["legs"][name] = entry
         log(f"batched {name}: {entry['value']} MP/s ({t:.3f} s per batch, {info['branches']} branches)")
-    # the first frames again from pinned host memory: the graph copies each
-    # frame in and its result out, overlapping other branches' kernels; the
-    # same frames device-resident beside it
+    # the first frames again from pinned host memory: uploads and downloads
+    # run on copy streams while other frames' kernels run; the same frames
+    # device-resident beside it
     nh = min(args.batch_host_frames, len(frames))
     if nh:
         sub, hbufs, mp = [], [], 0

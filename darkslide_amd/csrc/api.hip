@@ -260,6 +260,8 @@ struct ds_ctx {
     std::map<std::pair<size_t, size_t>, Tuned> tuning;
     std::vector<GraphEntry> graphs;
     unsigned long slot_gen = 0;  // bumped when a workspace buffer is replaced
+    // host-frame batches: upload and download streams beside `stream`
+    hipStream_t copy_in = nullptr, copy_out = nullptr;
 };
 
 namespace {
@@ -1378,6 +1380,116 @@ int batch_counts(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_
     return DS_OK;
 }
 
+// Host-frame batches (raw in, RGB out in pinned host memory) run outside the
+// graph, as a pipeline over three streams: uploads (copy_in) and downloads
+// (copy_out) overlap the kernels of other frames on the context stream.
+// Frames go largest first, so that each download (12 B/px at PCIe rate)
+// hides under the next frame's kernels and only the smallest frame's download
+// is left after the last kernel.  HOST_STAGES device staging buffers per
+// direction rotate; events order each buffer's reuse after the work that read
+// it.  Every workspace is reserved before the first frame is enqueued.
+constexpr int HOST_STAGES = 3;
+
+// A stream with a CU mask always gets a hardware queue of its own, where a
+// plain stream may share an in-order queue with the context stream or with
+// the other copy stream once the runtime's queues are all taken; a shared
+// queue puts each copy between the kernels of other frames.  The mask names
+// every CU, so nothing is withheld from kernels on that stream.
+hipError_t create_copy_stream(int device, hipStream_t* s) {
+    int cus = 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) return e;
+    std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0u);
+    for (int i = 0; i < cus; i++) mask[(size_t)i / 32] |= 1u << (i % 32);
+    return hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data());
+}
+
+int host_pipeline_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_opts& o, int* counts, ds_batch_info* info) {
+    for (hipStream_t* s : {&c->copy_in, &c->copy_out})
+        if (!*s) HIPCHK(c, create_copy_stream(c->device, s), "hipExtStreamCreateWithCUMask (copy stream)");
+    size_t raw_max = 0, out_max = 0;
+    size_t need[S_COUNT] = {};
+    std::vector<FramePlan> plans(n);
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return frames[a].w * frames[a].h > frames[b].w * frames[b].h; });
+    for (size_t i = 0; i < n; i++) {
+        plans[i] = plan_frame(frames[i], i, o);
+        frame_needs(frames[i], plans[i], need);
+        raw_max = std::max(raw_max, frames[i].stride * frames[i].h * 2);
+        out_max = std::max(out_max, plans[i].w * plans[i].h * 12);
+    }
+    raw_max = (raw_max + 255) & ~(size_t)255;
+    out_max = (out_max + 255) & ~(size_t)255;
+    need[S_PIPE_RAW] = std::max(need[S_PIPE_RAW], HOST_STAGES * raw_max);
+    need[S_PIPE_OUT] = std::max(need[S_PIPE_OUT], HOST_STAGES * out_max);
+    for (int s = 0; s < S_COUNT; s++) {
+        if (!need[s]) continue;
+        hipError_t e;
+        if (!slot(c, (Slot)s, need[s], &e)) return hip_fail(c, e, "batch workspace allocation");
+    }
+    if (o.denoise == DS_DENOISE_LEARNED && !o.net->packed) {
+        int rc = run(c, "cnn_pack", [&] { return cnn_pack(o.net->params(), o.net->wpk, c->stream); });
+        if (rc) return rc;
+        o.net->packed = true;
+    }
+    uint8_t* st_raw = static_cast<uint8_t*>(c->slots[S_PIPE_RAW].p);
+    uint8_t* st_out = static_cast<uint8_t*>(c->slots[S_PIPE_OUT].p);
+    hipEvent_t ev[3][HOST_STAGES] = {};  // uploaded, computed, downloaded; per staging buffer
+    auto destroy = [&] {
+        for (auto& row : ev)
+            for (hipEvent_t& e : row)
+                if (e) (void)hipEventDestroy(e);
+    };
+    for (auto& row : ev)
+        for (hipEvent_t& e : row)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                destroy();
+                return fail(c, DS_ERR_DEVICE, "hipEventCreate");
+            }
+    int rc = DS_OK;
+    for (size_t j = 0; j < n && !rc; j++) {
+        const size_t i = order[j];
+        const int k = (int)(j % HOST_STAGES);
+        const bool reuse = j >= HOST_STAGES;  // buffer k held an earlier frame
+        const FramePlan& p = plans[i];
+        ds_frame f = frames[i];
+        f.raw = reinterpret_cast<const uint16_t*>(st_raw + k * raw_max);
+        f.rgb_out = reinterpret_cast<float*>(st_out + k * out_max);
+        const size_t in_bytes = frames[i].stride * frames[i].h * 2, out_bytes = p.w * p.h * 12;
+        bool ok = true;
+        // upload into buffer k once the kernels of its previous frame have read it
+        if (reuse) ok = ok && hipStreamWaitEvent(c->copy_in, ev[1][k], 0) == hipSuccess;
+        ok = ok && hipMemcpyAsync(const_cast<uint16_t*>(f.raw), frames[i].raw, in_bytes, hipMemcpyHostToDevice, c->copy_in) == hipSuccess;
+        ok = ok && hipEventRecord(ev[0][k], c->copy_in) == hipSuccess;
+        // kernels once the upload has landed and the previous frame's download has read buffer k
+        ok = ok && hipStreamWaitEvent(c->stream, ev[0][k], 0) == hipSuccess;
+        if (reuse) ok = ok && hipStreamWaitEvent(c->stream, ev[2][k], 0) == hipSuccess;
+        if (!ok) {
+            rc = fail(c, DS_ERR_DEVICE, "frame %zu: upload", i);
+            break;
+        }
+        if ((rc = enqueue_stages(c, f, p, o, p.learned ? counts + i : nullptr))) {
+            c->err = "frame " + std::to_string(i) + ": " + c->err;
+            break;
+        }
+        ok = hipEventRecord(ev[1][k], c->stream) == hipSuccess && hipStreamWaitEvent(c->copy_out, ev[1][k], 0) == hipSuccess &&
+             hipMemcpyAsync(frames[i].rgb_out, f.rgb_out, out_bytes, hipMemcpyDeviceToHost, c->copy_out) == hipSuccess &&
+             hipEventRecord(ev[2][k], c->copy_out) == hipSuccess;
+        if (!ok) rc = fail(c, DS_ERR_DEVICE, "frame %zu: download", i);
+    }
+    // everything enqueued finishes before the events go and the call returns
+    const bool synced = hipStreamSynchronize(c->copy_in) == hipSuccess && hipStreamSynchronize(c->stream) == hipSuccess &&
+                        hipStreamSynchronize(c->copy_out) == hipSuccess;
+    destroy();
+    if (rc) return rc;
+    if (!synced) return fail(c, DS_ERR_DEVICE, "host-frame batch did not complete");
+    if (!info) return DS_OK;
+    info->frames = n;
+    info->branches = 1;
+    return batch_counts(c, frames, n, o, counts, info);
+}
+
 int pipeline_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipeline_opts& o, ds_batch_info* info) {
     if (info) std::memset(info, 0, sizeof(*info));
     if (int rc = validate_batch(c, frames, n, o)) return rc;
@@ -1387,6 +1499,7 @@ int pipeline_batch(ds_ctx* c, const ds_frame* frames, size_t n, const ds_pipelin
             if (!c->tuning.count({frames[i].w, frames[i].h}) && frames[i].w >= 16 && frames[i].h >= 16)
                 if (int rc = autotune_shape(c, frames[i].w, frames[i].h, nullptr)) return rc;
     SLOT(c, counts, int*, S_COUNTS, std::max<size_t>(n, 1024) * sizeof(int));
+    if (o.host_frames) return host_pipeline_batch(c, frames, n, o, counts, info);
     retire_stale_graphs(c);
     std::vector<int> candidates;
     if (o.streams > 0)
@@ -1620,6 +1733,11 @@ void ds_ctx_destroy(ds_ctx* c) {
             delete b;
         }
         c->branches.clear();
+        for (hipStream_t s : {c->copy_in, c->copy_out})
+            if (s) {
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamDestroy(s);
+            }
         for (auto& b : c->slots)
             if (b.p) (void)hipFree(b.p);
         for (void* p : c->user_allocs) (void)hipFree(p);

@@ -396,7 +396,11 @@ int ds_denoise_learned_dev(ds_ctx* ctx, ds_cnn* net, const float* rgb, size_t w,
  * workspace.  With autotune set, tile shapes are chosen per frame shape by
  * timing every variant once (ds_autotune), and with streams == 0 the branch
  * count too.  All pointers are device pointers; the call only enqueues,
- * unless info is set (then it waits and fills it). */
+ * unless info is set (then it waits and fills it).  With host_frames set the
+ * pointers are pinned host buffers instead: the batch then runs without a
+ * graph, as a pipeline of uploads, kernels and downloads on three streams
+ * (largest frame first), and the call returns when every result is in host
+ * memory; streams is ignored and info reports 1 branch. */
 #define DS_DENOISE_NONE 0
 #define DS_DENOISE_NLM 1
 #define DS_DENOISE_LEARNED 2
@@ -430,8 +434,8 @@ typedef struct ds_pipeline_opts {
     int autotune;
     int highlights;       /* 1: reconstruct_highlights after demosaic, as Engine::base does (pipeline/mod.rs:186-187) */
     int quality;          /* DS_QUALITY_FULL (RCD / MHC) or DS_QUALITY_DRAFT (superpixel, half size; no RL under reference_gating) */
-    int host_frames;      /* 1: every frame's raw and rgb_out are pinned host buffers (ds_host_alloc); the graph copies each
-                             frame in and its result out on its branch, overlapping other branches' kernels */
+    int host_frames;      /* 1: every frame's raw and rgb_out are pinned host buffers (ds_host_alloc); each frame is uploaded
+                             and its result downloaded on copy streams while other frames' kernels run */
 } ds_pipeline_opts;
 
 typedef struct ds_batch_info {

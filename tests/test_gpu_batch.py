@@ -123,9 +123,11 @@ def test_batch_graph_replay(ctx, batch, expected_plain):
 
 @pytest.mark.parametrize("streams", [1, 3])
 def test_batch_host_frames(ctx, batch, expected_plain, streams):
-    """host_frames: the same frames in pinned host memory, copied in and out
-    inside the graph, give the device-frame results bit for bit, on replay
-    too; pageable host memory is rejected."""
+    """host_frames: the same frames in pinned host memory, uploaded, run and
+    downloaded as a pipeline over three streams (no graph; `streams` has no
+    effect), give the device-frame results bit for bit, again in reverse
+    order, so that each staging buffer holds frames of other sizes; pageable
+    host memory is rejected."""
     import darkslide_amd as D
     say("batch host frames, streams", streams)
     hb, frames = [], []
@@ -140,8 +142,9 @@ def test_batch_host_frames(ctx, batch, expected_plain, streams):
         for rep in range(2):
             for (_, hout, w, h) in hb:
                 hout.array((h, w, 3), np.float32)[:] = np.nan
-            info = ctx.pipeline_batch(frames, streams=streams, host_frames=True)
-            assert info["frames"] == len(frames) and info["branches"] == streams and info["graph_reused"] == (1 if rep else 0)
+            order = list(range(len(frames)))[::(-1 if rep else 1)]
+            info = ctx.pipeline_batch([frames[k] for k in order], streams=streams, host_frames=True)
+            assert info["frames"] == len(frames) and info["branches"] == 1 and info["graph_reused"] == 0
             for k, ((_, hout, w, h), ref) in enumerate(zip(hb, expected_plain)):
                 assert np.array_equal(hout.array((h, w, 3), np.float32), ref), (rep, k)
         pageable = np.zeros(FRAMES[0][0] * FRAMES[0][1], np.uint16)
@@ -204,6 +207,25 @@ def test_batch_learned_matches_single_frames(ctx, batch):
         assert info["regions"] == regions
         # the untrained network is far from NLM: the spot check sends every region back
         assert info["fallback_regions"] == regions
+        # the same from pinned host memory, through the three-stream pipeline
+        hb, frames = [], []
+        try:
+            for f, (raw, _, w, h, *_rest) in zip(batch.frames, batch.bufs):
+                hin, hout = ctx.host_alloc(w * h * 2), ctx.host_alloc(w * h * 12)
+                hb.append((hin, hout, w, h))
+                hin.array((h, w), np.uint16)[:] = raw.download((h, w), np.uint16)
+                hout.array((h, w, 3), np.float32)[:] = np.nan
+                g = D.Frame.from_buffer_copy(f)
+                g.raw, g.rgb_out = hin.ptr.value, hout.ptr.value
+                frames.append(g)
+            info = ctx.pipeline_batch(frames, denoise=D.DENOISE_LEARNED, net=net, threshold_db=0.3, host_frames=True)
+            assert info["regions"] == regions and info["fallback_regions"] == regions
+            for k, ((_, hout, w, h), r) in enumerate(zip(hb, ref)):
+                assert np.array_equal(hout.array((h, w, 3), np.float32), r), k
+        finally:
+            for (hin, hout, *_r) in hb:
+                hin.free()
+                hout.free()
     finally:
         net.close()
 
