@@ -15,6 +15,8 @@
 // value at the clamped global coordinate, which is what MHC's clamp-to-edge
 // addressing reads.
 
+#include <type_traits>
+
 #include "common.h"
 #include "demosaic.h"
 
@@ -28,6 +30,21 @@ constexpr float kClip = 0.995f;  // demosaic.rs:17
 constexpr float kEps = 1e-5f;    // demosaic.rs:155
 constexpr int kBorder = 4;       // demosaic.rs:158
 
+// LDS pointer whose value the optimiser cannot see through (an empty asm
+// statement claims to change it).  Reads at non-negative constant offsets
+// from it compile to that one base register plus the LDS instruction's
+// unsigned offset field, where a base of the optimiser's own choosing can sit
+// inside a stencil and cost an address add per read on its other side.
+template <class T>
+using lds_cptr = const __attribute__((address_space(3))) T*;
+
+template <class T>
+__device__ __forceinline__ lds_cptr<T> lds_opaque(const T* p) {
+    uint32_t a = (uint32_t)(uintptr_t)(lds_cptr<T>)p;
+    asm("" : "+v"(a));
+    return (lds_cptr<T>)(uintptr_t)a;
+}
+
 template <int TX, int TY, int H>
 struct Tile {
     static constexpr int W = TX + 2 * H;
@@ -35,6 +52,16 @@ struct Tile {
     static constexpr int N = W * R;
     float* p;
     __device__ __forceinline__ float& operator()(int x, int y) const { return p[(y + H) * W + (x + H)]; }
+    // stencil at (x, y): s(dx, dy) reads (x + dx, y + dy), dx, dy >= 0
+    struct Sten {
+        lds_cptr<float> b;
+        __device__ __forceinline__ float operator()(int dx, int dy) const { return b[dy * W + dx]; }
+    };
+    __device__ __forceinline__ Sten sten(int x, int y) const { return Sten{lds_opaque(&(*this)(x, y))}; }
+    // the same for a byte plane of this shape
+    __device__ static __forceinline__ lds_cptr<uint8_t> sten_u8(const uint8_t* q, int x, int y) {
+        return lds_opaque(q + (y + H) * W + (x + H));
+    }
 };
 
 struct CfaSrc {
@@ -95,10 +122,11 @@ __device__ __forceinline__ float mhc_at(const T& cm, int x, int y, int col, int 
 
 __device__ __forceinline__ float sq(float v) { return v * v; }
 
-template <class T>
-__device__ __forceinline__ float disc(const T& d, int x, int y) {
-    float central = d(x, y);
-    float neigh = 0.25f * (d(x - 1, y - 1) + d(x + 1, y - 1) + d(x - 1, y + 1) + d(x + 1, y + 1));
+// disc at (x, y) of a plane, through its stencil s based at (x - 1, y - 1)
+template <class S>
+__device__ __forceinline__ float disc(const S& s) {
+    float central = s(1, 1);
+    float neigh = 0.25f * (s(0, 0) + s(2, 0) + s(0, 2) + s(2, 2));
     return fabsf(0.5f - central) < fabsf(0.5f - neigh) ? neigh : central;
 }
 
@@ -132,7 +160,15 @@ struct RcdLds {
 
 // Tiles whose 10-pixel halo lies inside [4, w-4) x [4, h-4) (INTERIOR) take
 // every stage's RCD branch, so their body drops the per-pixel border checks.
-// Stages that only touch R/B sites walk the checkerboard of those sites.
+//
+// Every stage walks only the sites whose values a later stage reads: lpf,
+// bp / bq and pq are read at R/B sites alone (lpf two pixels away along a
+// row or column, bp / bq and pq diagonally, all R/B sites again), so they
+// are computed there alone; the output stage runs the G sites (R and B
+// estimated) and the R/B sites (values copied) as two loops, so no wave
+// carries both paths.  Sites of a plane that no stage computes are never
+// read.  Stencil reads go through an opaque base at the stencil's top-left
+// corner (Tile::sten).
 template <int TX, int TY, int NT, bool INTERIOR>
 __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, float* __restrict__ rgb, float* __restrict__ clip) {
     const int tid = threadIdx.x;
@@ -156,10 +192,74 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
     auto in_box = [&](int gx, int gy, int b) { return INTERIOR || (gx >= b && gx < w - b && gy >= b && gy < h - b); };
     // RCD reads c = max(cm, 0); MHC (frame-border pixels only) reads cm.  An
     // interior tile never runs MHC, so it stores c itself.
-    auto cval = [&](int x, int y) { return INTERIOR ? cm(x, y) : fmaxr(cm(x, y), 0.0f); };
+    auto cv = [&](float v) { return INTERIOR ? v : fmaxr(v, 0.0f); };
+    auto cval = [&](int x, int y) { return cv(cm(x, y)); };
+    // c around (x, y) through a base at (x - R, y - R): c(dx, dy), |dx|, |dy| <= R
+    auto c_around = [&](int x, int y, auto r_tag) {
+        constexpr int R = decltype(r_tag)::value;
+        const auto s = cm.sten(x - R, y - R);
+        return [=](int dx, int dy) { return cv(s(dx + R, dy + R)); };
+    };
+    // the i-th site of a plane with halo HH walked on the R/B checkerboard
+    // (G = false) or the G checkerboard (G = true): (HW / 2) sites per row
+    auto site = [&](int i, int HH, int half_w, bool G, int& tx, int& ty) {
+        ty = i / half_w - HH;
+        tx = -HH + 2 * (i % half_w) + ((rbp + HH + ty + (G ? 1 : 0)) & 1);
+    };
 
-    // ---- stage 0: load cfa*mul (and clip flags) with clamped coordinates
-    {
+    // ---- stage 0: load cfa*mul (and clip flags)
+    if constexpr (INTERIOR) {
+        // No clamps.  A thread takes a pair of columns, an even and an odd
+        // one (ox and H_CM are even), so the pair's two CFA sites and their
+        // constants follow from the row's parity alone; the normalisation is
+        // CfaSrc::cfa's expression with those constants.
+        constexpr int CW = Tile<TX, TY, H_CM>::W, PW = CW / 2, NP = PW * Tile<TX, TY, H_CM>::R, K = (NP + NT - 1) / NT;
+        static_assert(CW % 2 == 0 && H_CM % 2 == 0, "column pairs start at even columns");
+        const size_t stride = src.stride;
+        const bool f32 = src.f32 != nullptr;
+        float r0[K], r1[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = tid + k * NT;
+            r0[k] = r1[k] = 0.0f;
+            if (j < NP) {
+                const size_t at = (size_t)(oy - H_CM + j / PW) * stride + (size_t)(ox - H_CM + 2 * (j % PW));
+                if (f32) {
+                    r0[k] = src.f32[at];
+                    r1[k] = src.f32[at + 1];
+                } else {
+                    r0[k] = (float)src.u16[at];
+                    r1[k] = (float)src.u16[at + 1];
+                }
+            }
+        }
+        // site s = (y & 1) * 2 + (x & 1): colour, black, denominator, multiplier
+        const int c0 = pat.at(0, 0), c1 = pat.at(1, 0), c2 = pat.at(0, 1), c3 = pat.at(1, 1);
+        const float b0 = src.blk[0], b1 = src.blk[1], b2 = src.blk[2], b3 = src.blk[3];
+        const float d0 = fmaxr(src.white_of(c0) - b0, 1.0f), d1 = fmaxr(src.white_of(c1) - b1, 1.0f);
+        const float d2 = fmaxr(src.white_of(c2) - b2, 1.0f), d3 = fmaxr(src.white_of(c3) - b3, 1.0f);
+        const float m0 = src.mul_of(c0), m1 = src.mul_of(c1), m2 = src.mul_of(c2), m3 = src.mul_of(c3);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = tid + k * NT;
+            if (j >= NP) continue;
+            const int ty = j / PW - H_CM, tx = 2 * (j % PW) - H_CM;
+            const bool odd = (ty & 1) != 0;
+            float v0 = r0[k], v1 = r1[k];
+            if (!f32) {
+                v0 = (v0 - (odd ? b2 : b0)) / (odd ? d2 : d0);
+                v1 = (v1 - (odd ? b3 : b1)) / (odd ? d3 : d1);
+            }
+            cm(tx, ty) = fmaxr(v0 * (odd ? m2 : m0), 0.0f);
+            cm(tx + 1, ty) = fmaxr(v1 * (odd ? m3 : m1), 0.0f);
+            if (clip && ty >= -H_FLAG && ty < TY + H_FLAG) {
+                uint8_t* f = L.flag + (ty + H_FLAG) * Tile<TX, TY, H_FLAG>::W + (tx + H_FLAG);
+                if (tx >= -H_FLAG && tx < TX + H_FLAG) f[0] = v0 >= kClip ? 1 : 0;
+                if (tx + 1 >= -H_FLAG && tx + 1 < TX + H_FLAG) f[1] = v1 >= kClip ? 1 : 0;
+            }
+        }
+    } else {
+        // clamped coordinates
         constexpr int CW = Tile<TX, TY, H_CM>::W;
         stage_tile<Tile<TX, TY, H_CM>::N, NT>(
             [&](int i) {
@@ -177,41 +277,45 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
     }
     __syncthreads();
 
-    // ---- stage 1: bv / bh (demosaic.rs:160-172) over halo 7
+    // ---- stage 1: bv / bh (demosaic.rs:160-172) over halo 7, every site
     for (int i = tid; i < Tile<TX, TY, H_BVH>::N; i += NT) {
         int tx = i % Tile<TX, TY, H_BVH>::W - H_BVH, ty = i / Tile<TX, TY, H_BVH>::W - H_BVH;
         int gx = ox + tx, gy = oy + ty;
         float v = 0.0f, hz = 0.0f;
         if (in_box(gx, gy, 3)) {
-            v = sq((cval(tx, ty - 3) - cval(tx, ty - 1) - cval(tx, ty + 1) + cval(tx, ty + 3)) - 3.0f * (cval(tx, ty - 2) + cval(tx, ty + 2)) +
-                   6.0f * cval(tx, ty));
-            hz = sq((cval(tx - 3, ty) - cval(tx - 1, ty) - cval(tx + 1, ty) + cval(tx + 3, ty)) - 3.0f * (cval(tx - 2, ty) + cval(tx + 2, ty)) +
-                    6.0f * cval(tx, ty));
+            const auto c = c_around(tx, ty, std::integral_constant<int, 3>{});
+            v = sq((c(0, -3) - c(0, -1) - c(0, 1) + c(0, 3)) - 3.0f * (c(0, -2) + c(0, 2)) + 6.0f * c(0, 0));
+            hz = sq((c(-3, 0) - c(-1, 0) - c(1, 0) + c(3, 0)) - 3.0f * (c(-2, 0) + c(2, 0)) + 6.0f * c(0, 0));
         }
         bv(tx, ty) = v;
         bh(tx, ty) = hz;
     }
-    // lpf at R/B sites (demosaic.rs:188-201) over halo 7; G sites hold 0
-    for (int i = tid; i < Tile<TX, TY, H_LPF>::N; i += NT) {
-        int tx = i % Tile<TX, TY, H_LPF>::W - H_LPF, ty = i / Tile<TX, TY, H_LPF>::W - H_LPF;
-        int gx = ox + tx, gy = oy + ty;
-        float v = 0.0f;
-        if (in_box(gx, gy, 2) && pat.at(gx, gy) != 1) {
-            v = cval(tx, ty) + 0.5f * (cval(tx, ty - 1) + cval(tx, ty + 1) + cval(tx - 1, ty) + cval(tx + 1, ty)) +
-                0.25f * (cval(tx - 1, ty - 1) + cval(tx + 1, ty - 1) + cval(tx - 1, ty + 1) + cval(tx + 1, ty + 1));
+    // lpf (demosaic.rs:188-201) over halo 7, R/B sites
+    {
+        constexpr int HW = Tile<TX, TY, H_LPF>::W / 2;
+        for (int i = tid; i < HW * Tile<TX, TY, H_LPF>::R; i += NT) {
+            int tx, ty;
+            site(i, H_LPF, HW, false, tx, ty);
+            float v = 0.0f;
+            if (in_box(ox + tx, oy + ty, 2)) {
+                const auto c = c_around(tx, ty, std::integral_constant<int, 1>{});
+                v = c(0, 0) + 0.5f * (c(0, -1) + c(0, 1) + c(-1, 0) + c(1, 0)) + 0.25f * (c(-1, -1) + c(1, -1) + c(-1, 1) + c(1, 1));
+            }
+            lpf(tx, ty) = v;
         }
-        lpf(tx, ty) = v;
     }
     __syncthreads();
 
-    // ---- stage 2: vh (demosaic.rs:173-184) over halo 6; 0.5 off the border box
+    // ---- stage 2: vh (demosaic.rs:173-184) over halo 6, every site; 0.5 off the border box
     for (int i = tid; i < Tile<TX, TY, H_VH>::N; i += NT) {
         int tx = i % Tile<TX, TY, H_VH>::W - H_VH, ty = i / Tile<TX, TY, H_VH>::W - H_VH;
         int gx = ox + tx, gy = oy + ty;
         float v = 0.5f;
         if (in_box(gx, gy, kBorder)) {
-            float vs = fmaxr(bv(tx, ty - 1) + bv(tx, ty) + bv(tx, ty + 1), kEps);
-            float hs = fmaxr(bh(tx - 1, ty) + bh(tx, ty) + bh(tx + 1, ty), kEps);
+            const auto sv = bv.sten(tx, ty - 1);
+            const auto sh = bh.sten(tx - 1, ty);
+            float vs = fmaxr(sv(0, 0) + sv(0, 1) + sv(0, 2), kEps);
+            float hs = fmaxr(sh(0, 0) + sh(1, 0) + sh(2, 0), kEps);
             v = vs / (vs + hs);
         }
         vh(tx, ty) = v;
@@ -219,33 +323,32 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
     __syncthreads();
 
     // ---- stage 3: green (demosaic.rs:222-245).  G sites copy c; R/B sites
-    // in the border box get the RCD estimate, outside it the MHC value.  The
-    // diagonal statistics bp / bq follow in the same phase.
+    // in the border box get the RCD estimate, outside it the MHC value.
     constexpr int GW = Tile<TX, TY, H_G>::W, GH = Tile<TX, TY, H_G>::R;
     for (int i = tid; i < (GW / 2) * GH; i += NT) {
-        const int ty = i / (GW / 2) - H_G;
-        const int off = (rbp + H_G + ty) & 1;  // first R/B column of this row
-        const int tx_rb = -H_G + 2 * (i % (GW / 2)) + off;
-        const int tx_g = -H_G + 2 * (i % (GW / 2)) + (off ^ 1);
+        int tx, ty, tx_g, ty_g;
+        site(i, H_G, GW / 2, false, tx, ty);
+        site(i, H_G, GW / 2, true, tx_g, ty_g);
         const int gy = oy + ty;
         // green site
         gg(tx_g, ty) = in_img(ox + tx_g, gy) ? cval(tx_g, ty) : 0.0f;
         // R/B site
-        const int tx = tx_rb, gx = ox + tx;
+        const int gx = ox + tx;
         float gv = 0.0f;
         if (in_img(gx, gy)) {
             if (in_box(gx, gy, kBorder)) {
-                float vd = disc(vh, tx, ty);
-                auto c = [&](int dx, int dy) { return cval(tx + dx, ty + dy); };
+                float vd = disc(vh.sten(tx - 1, ty - 1));
+                const auto c = c_around(tx, ty, std::integral_constant<int, 4>{});
                 float n_grad = kEps + fabsf(c(0, -1) - c(0, 1)) + fabsf(c(0, 0) - c(0, -2)) + fabsf(c(0, -1) - c(0, -3)) + fabsf(c(0, -2) - c(0, -4));
                 float s_grad = kEps + fabsf(c(0, -1) - c(0, 1)) + fabsf(c(0, 0) - c(0, 2)) + fabsf(c(0, 1) - c(0, 3)) + fabsf(c(0, 2) - c(0, 4));
                 float w_grad = kEps + fabsf(c(-1, 0) - c(1, 0)) + fabsf(c(0, 0) - c(-2, 0)) + fabsf(c(-1, 0) - c(-3, 0)) + fabsf(c(-2, 0) - c(-4, 0));
                 float e_grad = kEps + fabsf(c(-1, 0) - c(1, 0)) + fabsf(c(0, 0) - c(2, 0)) + fabsf(c(1, 0) - c(3, 0)) + fabsf(c(2, 0) - c(4, 0));
-                float l0 = lpf(tx, ty);
-                float n_est = c(0, -1) * 2.0f * l0 / (kEps + l0 + lpf(tx, ty - 2));
-                float s_est = c(0, 1) * 2.0f * l0 / (kEps + l0 + lpf(tx, ty + 2));
-                float w_est = c(-1, 0) * 2.0f * l0 / (kEps + l0 + lpf(tx - 2, ty));
-                float e_est = c(1, 0) * 2.0f * l0 / (kEps + l0 + lpf(tx + 2, ty));
+                const auto lp = lpf.sten(tx - 2, ty - 2);
+                float l0 = lp(2, 2);
+                float n_est = c(0, -1) * 2.0f * l0 / (kEps + l0 + lp(2, 0));
+                float s_est = c(0, 1) * 2.0f * l0 / (kEps + l0 + lp(2, 4));
+                float w_est = c(-1, 0) * 2.0f * l0 / (kEps + l0 + lp(0, 2));
+                float e_est = c(1, 0) * 2.0f * l0 / (kEps + l0 + lp(4, 2));
                 float v_est = (s_grad * n_est + n_grad * s_est) / (n_grad + s_grad);
                 float h_est = (w_grad * e_est + e_grad * w_est) / (e_grad + w_grad);
                 gv = fmaxr(vd * h_est + (1.0f - vd) * v_est, 0.0f);
@@ -257,18 +360,19 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
     }
     // disc(vh) at the tile's G sites, for stage 6
     for (int i = tid; i < TX * TY / 2; i += NT) {
-        const int ty = i / (TX / 2);
-        const int tx = 2 * (i % (TX / 2)) + ((rbp + 1 + ty) & 1);
-        vdt[ty * TX + tx] = disc(vh, tx, ty);
+        int tx, ty;
+        site(i, 0, TX / 2, true, tx, ty);
+        vdt[ty * TX + tx] = disc(vh.sten(tx - 1, ty - 1));
     }
     __syncthreads();
-    // bp / bq (demosaic.rs:248-260) over lpf / vh, free since the barrier above
-    for (int i = tid; i < Tile<TX, TY, H_G>::N; i += NT) {
-        int tx = i % Tile<TX, TY, H_G>::W - H_G, ty = i / Tile<TX, TY, H_G>::W - H_G;
-        int gx = ox + tx, gy = oy + ty;
+    // bp / bq (demosaic.rs:248-260) over halo 5, R/B sites, over lpf / vh
+    // (free since the barrier above)
+    for (int i = tid; i < (GW / 2) * GH; i += NT) {
+        int tx, ty;
+        site(i, H_G, GW / 2, false, tx, ty);
         float p = 0.0f, q = 0.0f;
-        if (in_box(gx, gy, 3)) {
-            auto c = [&](int dx, int dy) { return cval(tx + dx, ty + dy); };
+        if (in_box(ox + tx, oy + ty, 3)) {
+            const auto c = c_around(tx, ty, std::integral_constant<int, 3>{});
             p = sq((c(-3, -3) - c(-1, -1) - c(1, 1) + c(3, 3)) - 3.0f * (c(-2, -2) + c(2, 2)) + 6.0f * c(0, 0));
             q = sq((c(3, -3) - c(1, -1) - c(-1, 1) + c(-3, 3)) - 3.0f * (c(2, -2) + c(-2, 2)) + 6.0f * c(0, 0));
         }
@@ -277,17 +381,22 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
     }
     __syncthreads();
 
-    // ---- stage 4: pq (demosaic.rs:261-272) over halo 4
-    for (int i = tid; i < Tile<TX, TY, H_PQ>::N; i += NT) {
-        int tx = i % Tile<TX, TY, H_PQ>::W - H_PQ, ty = i / Tile<TX, TY, H_PQ>::W - H_PQ;
-        int gx = ox + tx, gy = oy + ty;
-        float v = 0.5f;
-        if (in_box(gx, gy, kBorder)) {
-            float ps = fmaxr(bp(tx - 1, ty - 1) + bp(tx, ty) + bp(tx + 1, ty + 1), kEps);
-            float qs = fmaxr(bq(tx + 1, ty - 1) + bq(tx, ty) + bq(tx - 1, ty + 1), kEps);
-            v = ps / (ps + qs);
+    // ---- stage 4: pq (demosaic.rs:261-272) over halo 4, R/B sites
+    {
+        constexpr int HW = Tile<TX, TY, H_PQ>::W / 2;
+        for (int i = tid; i < HW * Tile<TX, TY, H_PQ>::R; i += NT) {
+            int tx, ty;
+            site(i, H_PQ, HW, false, tx, ty);
+            float v = 0.5f;
+            if (in_box(ox + tx, oy + ty, kBorder)) {
+                const auto sp = bp.sten(tx - 1, ty - 1);
+                const auto sq_ = bq.sten(tx - 1, ty - 1);
+                float ps = fmaxr(sp(0, 0) + sp(1, 1) + sp(2, 2), kEps);
+                float qs = fmaxr(sq_(2, 0) + sq_(1, 1) + sq_(0, 2), kEps);
+                v = ps / (ps + qs);
+            }
+            pq(tx, ty) = v;
         }
-        pq(tx, ty) = v;
     }
     __syncthreads();
 
@@ -298,8 +407,8 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
     // site, i.e. c itself.
     constexpr int QW = Tile<TX, TY, H_RB>::W, QR = Tile<TX, TY, H_RB>::R;
     for (int i = tid; i < (QW / 2) * QR; i += NT) {
-        const int ty = i / (QW / 2) - H_RB;
-        const int tx = -H_RB + 2 * (i % (QW / 2)) + ((rbp + H_RB + ty) & 1);
+        int tx, ty;
+        site(i, H_RB, QW / 2, false, tx, ty);
         const int gx = ox + tx, gy = oy + ty;
         float rv = 0.0f, bvv = 0.0f;
         if (in_img(gx, gy)) {
@@ -307,9 +416,10 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
             float own = cval(tx, ty);
             float other;
             if (in_box(gx, gy, kBorder)) {
-                float pd = disc(pq, tx, ty);
-                auto cc = [&](int dx, int dy) { return cval(tx + dx, ty + dy); };
-                auto g = [&](int dx, int dy) { return gg(tx + dx, ty + dy); };
+                float pd = disc(pq.sten(tx - 1, ty - 1));
+                const auto cc = c_around(tx, ty, std::integral_constant<int, 3>{});
+                const auto gs = gg.sten(tx - 2, ty - 2);
+                auto g = [&](int dx, int dy) { return gs(dx + 2, dy + 2); };
                 float nw_g = kEps + fabsf(cc(-1, -1) - cc(1, 1)) + fabsf(cc(-1, -1) - cc(-3, -3)) + fabsf(g(0, 0) - g(-2, -2));
                 float ne_g = kEps + fabsf(cc(1, -1) - cc(-1, 1)) + fabsf(cc(1, -1) - cc(3, -3)) + fabsf(g(0, 0) - g(2, -2));
                 float sw_g = kEps + fabsf(cc(-1, 1) - cc(1, -1)) + fabsf(cc(-1, 1) - cc(-3, 3)) + fabsf(g(0, 0) - g(-2, 2));
@@ -337,57 +447,64 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
     }
     __syncthreads();
 
-    // ---- stage 6: outputs; R and B at G sites (demosaic.rs:310-341)
-    for (int i = tid; i < TX * TY; i += NT) {
-        int tx = i % TX, ty = i / TX;
-        int gx = ox + tx, gy = oy + ty;
-        if (gx >= w || gy >= h) continue;
-        int col = pat.at(gx, gy);
-        float o[3];
-        if (col == 1) {
-            float gc = cval(tx, ty);
-            o[1] = gc;
-            if (in_box(gx, gy, kBorder)) {
-                float vd = vdt[ty * TX + tx];
-                auto g = [&](int dx, int dy) { return gg(tx + dx, ty + dy); };
-#pragma unroll
-                for (int k = 0; k < 2; k++) {
-                    const Tile<TX, TY, H_RB>& ch = k == 0 ? r43 : b43;
-                    auto cc = [&](int dx, int dy) { return ch(tx + dx, ty + dy); };
-                    float n_g = kEps + fabsf(g(0, 0) - g(0, -2)) + fabsf(cc(0, -1) - cc(0, 1)) + fabsf(cc(0, -1) - cc(0, -3));
-                    float s_g = kEps + fabsf(g(0, 0) - g(0, 2)) + fabsf(cc(0, 1) - cc(0, -1)) + fabsf(cc(0, 1) - cc(0, 3));
-                    float w_g = kEps + fabsf(g(0, 0) - g(-2, 0)) + fabsf(cc(-1, 0) - cc(1, 0)) + fabsf(cc(-1, 0) - cc(-3, 0));
-                    float e_g = kEps + fabsf(g(0, 0) - g(2, 0)) + fabsf(cc(1, 0) - cc(-1, 0)) + fabsf(cc(1, 0) - cc(3, 0));
-                    float n_e = cc(0, -1) - g(0, -1);
-                    float s_e = cc(0, 1) - g(0, 1);
-                    float w_e = cc(-1, 0) - g(-1, 0);
-                    float e_e = cc(1, 0) - g(1, 0);
-                    float v_est = (n_g * s_e + s_g * n_e) / (n_g + s_g);
-                    float h_est = (e_g * w_e + w_g * e_e) / (e_g + w_g);
-                    o[k == 0 ? 0 : 2] = fmaxr(g(0, 0) + (1.0f - vd) * v_est + vd * h_est, 0.0f);
-                }
-            } else {
-                int row_c = pat.at(gx + 1, gy);
-                o[0] = mhc_at(cm, tx, ty, 1, row_c, 0);
-                o[2] = mhc_at(cm, tx, ty, 1, row_c, 2);
-            }
-        } else {
-            o[1] = gg(tx, ty);
-            o[0] = r43(tx, ty);
-            o[2] = b43(tx, ty);
-        }
+    // ---- stage 6: outputs
+    auto store = [&](int tx, int ty, float r, float g, float b) {
+        const int gx = ox + tx, gy = oy + ty;
         size_t gi = (size_t)gy * (size_t)w + (size_t)gx;
-        rgb[3 * gi + 0] = o[0];
-        rgb[3 * gi + 1] = o[1];
-        rgb[3 * gi + 2] = o[2];
+        rgb[3 * gi + 0] = r;
+        rgb[3 * gi + 1] = g;
+        rgb[3 * gi + 2] = b;
         if (clip) {
+            const auto f = Tile<TX, TY, H_FLAG>::sten_u8(L.flag, tx - 1, ty - 1);
             uint8_t m = 0;
 #pragma unroll
-            for (int dy = -1; dy <= 1; dy++)
+            for (int dy = 0; dy < 3; dy++)
 #pragma unroll
-                for (int dx = -1; dx <= 1; dx++) m |= L.flag[(ty + dy + H_FLAG) * Tile<TX, TY, H_FLAG>::W + (tx + dx + H_FLAG)];
+                for (int dx = 0; dx < 3; dx++) m |= f[dy * Tile<TX, TY, H_FLAG>::W + dx];
             clip[gi] = m ? 1.0f : 0.0f;
         }
+    };
+    // R and B at G sites (demosaic.rs:310-341)
+    for (int i = tid; i < (TX / 2) * TY; i += NT) {
+        int tx, ty;
+        site(i, 0, TX / 2, true, tx, ty);
+        const int gx = ox + tx, gy = oy + ty;
+        if (gx >= w || gy >= h) continue;
+        float o[3];
+        o[1] = cval(tx, ty);
+        if (in_box(gx, gy, kBorder)) {
+            float vd = vdt[ty * TX + tx];
+            const auto gs = gg.sten(tx - 2, ty - 2);
+            auto g = [&](int dx, int dy) { return gs(dx + 2, dy + 2); };
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const auto cs = (k == 0 ? r43 : b43).sten(tx - 3, ty - 3);
+                auto cc = [&](int dx, int dy) { return cs(dx + 3, dy + 3); };
+                float n_g = kEps + fabsf(g(0, 0) - g(0, -2)) + fabsf(cc(0, -1) - cc(0, 1)) + fabsf(cc(0, -1) - cc(0, -3));
+                float s_g = kEps + fabsf(g(0, 0) - g(0, 2)) + fabsf(cc(0, 1) - cc(0, -1)) + fabsf(cc(0, 1) - cc(0, 3));
+                float w_g = kEps + fabsf(g(0, 0) - g(-2, 0)) + fabsf(cc(-1, 0) - cc(1, 0)) + fabsf(cc(-1, 0) - cc(-3, 0));
+                float e_g = kEps + fabsf(g(0, 0) - g(2, 0)) + fabsf(cc(1, 0) - cc(-1, 0)) + fabsf(cc(1, 0) - cc(3, 0));
+                float n_e = cc(0, -1) - g(0, -1);
+                float s_e = cc(0, 1) - g(0, 1);
+                float w_e = cc(-1, 0) - g(-1, 0);
+                float e_e = cc(1, 0) - g(1, 0);
+                float v_est = (n_g * s_e + s_g * n_e) / (n_g + s_g);
+                float h_est = (e_g * w_e + w_g * e_e) / (e_g + w_g);
+                o[k == 0 ? 0 : 2] = fmaxr(g(0, 0) + (1.0f - vd) * v_est + vd * h_est, 0.0f);
+            }
+        } else {
+            int row_c = pat.at(gx + 1, gy);
+            o[0] = mhc_at(cm, tx, ty, 1, row_c, 0);
+            o[2] = mhc_at(cm, tx, ty, 1, row_c, 2);
+        }
+        store(tx, ty, o[0], o[1], o[2]);
+    }
+    // R/B sites: the values of stages 3 and 5
+    for (int i = tid; i < (TX / 2) * TY; i += NT) {
+        int tx, ty;
+        site(i, 0, TX / 2, false, tx, ty);
+        if (ox + tx >= w || oy + ty >= h) continue;
+        store(tx, ty, r43(tx, ty), gg(tx, ty), b43(tx, ty));
     }
 }
 

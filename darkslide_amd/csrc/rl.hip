@@ -27,6 +27,13 @@ namespace {
 
 constexpr int RL_TX = 64;
 
+// Start of every tap sum.  The reference starts from +0.0, and +0.0 + t
+// equals t for every t other than -0.0; here every term is a positive tap
+// times a value >= 0 (est and y stay >= 1e-6, the ratio > 0), so no term is
+// -0.0 and starting from -0.0 gives the same bits.  -0.0 + t == t for all t,
+// so the compiler drops that first add, one per output and pass.
+constexpr float kSumStart = -0.0f;
+
 struct Taps {
     float k[13];
 };
@@ -162,7 +169,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
             float* dst = s_tmp + (v + 2 * R) * MS + (u0 + R);
 #pragma unroll
             for (int j = 0; j < B; j++) {
-                float acc = 0.0f;
+                float acc = kSumStart;
 #pragma unroll
                 for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
                 if (u0 + j < TX + R) dst[j] = acc;
@@ -187,7 +194,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
             for (int i = 0; i < B + 2 * R; i++) x[i] = src[min(i, MH - 1 - (v0 - R + 2 * R)) * MS];
 #pragma unroll
             for (int j = 0; j < B; j++) {
-                float acc = 0.0f;
+                float acc = kSumStart;
 #pragma unroll
                 for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
                 if (v0 + j < TY + R) s_rat[(v0 + j + R) * YS + (u + R)] = yv[j] / fmaxr(acc, 1e-6f);
@@ -205,7 +212,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
             float* dst = s_tmp + (v + R) * QS + u0;
 #pragma unroll
             for (int j = 0; j < B; j++) {
-                float acc = 0.0f;
+                float acc = kSumStart;
 #pragma unroll
                 for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
                 dst[j] = acc;
@@ -226,7 +233,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
             float* o = est_out + (size_t)(oy + v0) * (size_t)w + (ox + u);
 #pragma unroll
             for (int j = 0; j < B; j++) {
-                float acc = 0.0f;
+                float acc = kSumStart;
 #pragma unroll
                 for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
                 *o = clampr(ev[q * B + j] * acc, 0.0f, 64.0f);
@@ -244,7 +251,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
     for (int i = tid; i < MH * MW; i += NT) {
         int u = i % MW - R, v = i / MW - 2 * R;
         int cu = cu_of(u);
-        float acc = 0.0f;
+        float acc = kSumStart;
 #pragma unroll
         for (int j = 0; j < K; j++) acc += kk[j] * E(cu + j - R, v);
         s_tmp[(v + 2 * R) * MS + (u + R)] = acc;
@@ -253,7 +260,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
     for (int i = tid; i < YH * YW; i += NT) {
         int u = i % YW - R, v = i / YW - R;
         int cv = cv_of(v);
-        float acc = 0.0f;
+        float acc = kSumStart;
 #pragma unroll
         for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(cv + j - R + 2 * R) * MS + (u + R)];
         const float yv = yp[(size_t)(oy + cv) * w + (ox + cu_of(u))];  // y at the clamped pixel
@@ -263,7 +270,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
     for (int i = tid; i < QH * QW; i += NT) {
         int u = i % QW, v = i / QW - R;
         int cu = cu_of(u);
-        float acc = 0.0f;
+        float acc = kSumStart;
 #pragma unroll
         for (int j = 0; j < K; j++) acc += kk[j] * s_rat[(v + R) * YS + (cu + j - R + R)];
         s_tmp[(v + R) * QS + u] = acc;
@@ -275,7 +282,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
         int u = i % TX, v = i / TX;
         int gx = ox + u, gy = oy + v;
         if (i >= TX * TY || gx >= w || gy >= h) continue;
-        float acc = 0.0f;
+        float acc = kSumStart;
 #pragma unroll
         for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(v + j - R + R) * QS + u];
         est_out[(size_t)gy * w + gx] = clampr(ev[k] * acc, 0.0f, 64.0f);
