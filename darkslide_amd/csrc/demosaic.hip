@@ -201,10 +201,15 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
         return [=](int dx, int dy) { return cv(s(dx + R, dy + R)); };
     };
     // the i-th site of a plane with halo HH walked on the R/B checkerboard
-    // (G = false) or the G checkerboard (G = true): (HW / 2) sites per row
+    // (G = false) or the G checkerboard (G = true), half_w sites per row.
+    // Consecutive items alternate between two rows at the same column pair:
+    // the rows' sites then sit an odd number of words apart (the row length
+    // is even), so a half-wave's 32 lanes hit 32 distinct LDS banks, where
+    // 32 sites of one row, two words apart, would share 16.
     auto site = [&](int i, int HH, int half_w, bool G, int& tx, int& ty) {
-        ty = i / half_w - HH;
-        tx = -HH + 2 * (i % half_w) + ((rbp + HH + ty + (G ? 1 : 0)) & 1);
+        const int j = i >> 1;
+        ty = 2 * (j / half_w) + (i & 1) - HH;
+        tx = -HH + 2 * (j % half_w) + ((rbp + HH + ty + (G ? 1 : 0)) & 1);
     };
 
     // ---- stage 0: load cfa*mul (and clip flags)
