@@ -155,34 +155,37 @@ __global__ __launch_bounds__(NLM_NT, HSB == 1 ? 6 : 2) void k_nlm_tile(const flo
             for (int task = 0; task < NTASK; task++) {
                 const float* c = cc[task];
                 const int u0 = u00 + NSG * SEG * task;
-                float d[DW];
+                float d[DW];  // differences (not squared)
                 if constexpr (IN) {
                     const float* sp = pt + (cv + SP) * PW + (u0 - P + SP) + doff;
 #pragma unroll
-                    for (int i = 0; i < DW; i++) {
-                        float e = c[i] - sp[i];
-                        d[i] = e * e;
-                    }
+                    for (int i = 0; i < DW; i++) d[i] = c[i] - sp[i];
                 } else {
                     const int sv = clampi(oy + cv + dy, 0, h - 1) - oy;
 #pragma unroll
                     for (int i = 0; i < DW; i++) {
                         const int su = clampi(ox + cu_of(u0 - P + i) + dx, 0, w - 1) - ox;
-                        float e = c[i] - P_at(su, sv);
-                        d[i] = e * e;
+                        d[i] = c[i] - P_at(su, sv);
                     }
                 }
-                // sliding window sums, as the reference's box blur slides them
-                // (a prefix-sum form saves an operation per sum but moves the
-                // denoise composite outside the reference's own spread)
+                // sliding window sums, as the reference's box blur slides
+                // them, with each square folded into a fused multiply-add: the
+                // entering square is added and the leaving one subtracted
+                // unrounded, two operations per step instead of a square, a
+                // difference and an add (a prefix-sum form saves an operation
+                // per sum but moves the denoise composite outside the
+                // reference's own spread)
                 float hsum[SEG];
-                float s = 0.0f;
+                float s = d[0] * d[0];
 #pragma unroll
-                for (int i = 0; i < 2 * P + 1; i++) s += d[i];
+                for (int i = 1; i < 2 * P + 1; i++) s = __builtin_fmaf(d[i], d[i], s);
 #pragma unroll
                 for (int j = 0; j < SEG; j++) {
                     hsum[j] = s;
-                    if (j + 1 < SEG) s += d[j + 2 * P + 1] - d[j];
+                    if (j + 1 < SEG) {
+                        s = __builtin_fmaf(d[j + 2 * P + 1], d[j + 2 * P + 1], s);
+                        s = __builtin_fmaf(-d[j], d[j], s);
+                    }
                 }
                 float4* dst = reinterpret_cast<float4*>(hs + (v1 + P) * HS + u0);
 #pragma unroll
@@ -193,7 +196,8 @@ __global__ __launch_bounds__(NLM_NT, HSB == 1 ? 6 : 2) void k_nlm_tile(const flo
             {
                 const float* hc = hs + row0 * HS + col;  // hs row r holds tile row r - P
                 const float* qp = pt + (row0 + SP) * PW + (col + SP) + doff;
-                float s = 0.0f;
+                // -0.0 + t == t for every t, so the first add folds away
+                float s = -0.0f;
 #pragma unroll
                 for (int j = 0; j < 2 * P + 1; j++) s += hc[j * HS];
                 // every row segment runs RPT rows (the last one past the tile

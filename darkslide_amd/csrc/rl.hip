@@ -118,7 +118,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
     const int ox = tx * TX, oy = ty * TY;
     float kk[K];
 #pragma unroll
-    for (int i = 0; i < K; i++) kk[i] = taps.k[i];
+    for (int i = 0; i < K; i++) kk[i] = taps.k[i <= R ? i : K - 1 - i];
 
     const bool interior = ox >= 2 * R && oy >= 2 * R && ox + TX + 2 * R <= w && oy + TY + 2 * R <= h;
     if (interior) {
