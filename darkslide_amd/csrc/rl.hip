@@ -38,9 +38,37 @@ struct Taps {
     float k[13];
 };
 
-// luminance (image.rs:174-176, color.rs:196-198), y = max(lum, 1e-6); est = y
-__global__ void k_rl_init(const float* __restrict__ rgb, size_t n, float* __restrict__ y, float* __restrict__ est) {
+// y outside [1e-6, 2^64] (infinite, or too large for div_in_range) seen by
+// the first iteration of a call: the later iterations then keep the plain
+// division.  y = max(luminance, 1e-6) is never NaN (fmaxf returns the
+// non-NaN operand).
+constexpr float kDivSafeMax = 0x1p64f;
+
+// a / b, bit for bit the compiler's IEEE division (v_div_scale, v_rcp,
+// two Newton steps on the reciprocal and two remainder corrections,
+// v_div_fmas, v_div_fixup) with the scale and fixup steps left out, for
+// a in [1e-6, 2^64] and b in [1e-6, 64.01].  There v_div_scale returns its
+// operand unchanged with VCC clear: neither operand is 0 or denormal, the
+// exponents differ by less than 96, 1/b and a/b are normal and a >= 2^-103.
+// So v_div_fmas is a plain fused multiply-add, and v_div_fixup returns the
+// quotient (no NaN, infinity or zero operand, no overflow or underflow, both
+// signs positive).  8 instructions instead of 11.
+__device__ __forceinline__ float div_in_range(float a, float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, r, 1.0f);
+    r = __builtin_fmaf(e, r, r);
+    float q = a * r;
+    float t = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(t, r, q);
+    t = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(t, r, q);
+}
+
+// luminance (image.rs:174-176, color.rs:196-198), y = max(lum, 1e-6); est = y.
+// Clears the call's range flag, which the first iteration sets.
+__global__ void k_rl_init(const float* __restrict__ rgb, size_t n, float* __restrict__ y, float* __restrict__ est, int* __restrict__ flag) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *flag = 0;
     if (i >= n) return;
     float l = 0.2880402f * rgb[3 * i] + 0.7118741f * rgb[3 * i + 1] + 0.0000857f * rgb[3 * i + 2];
     float v = fmaxr(l, 1e-6f);
@@ -89,9 +117,13 @@ __device__ __forceinline__ void stage_rows(const float* __restrict__ g, int w, f
         }
 }
 
-template <int R, int TY, int NT>
+// FIRST: the call's first iteration, with the plain division; it checks
+// every y it reads and sets *flag if one is outside [1e-6, 2^64].  Later
+// iterations (FIRST false) read *flag: clear, the interior tiles divide with
+// div_in_range (est is then in [0, 64], so conv is in [0, 64.01]).
+template <int R, int TY, int NT, bool FIRST>
 __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, const float* __restrict__ yp, float* __restrict__ est_out,
-                                                   int w, int h, Taps taps, TileOrder order) {
+                                                   int w, int h, Taps taps, TileOrder order, int* __restrict__ flag) {
     static_assert(TY % 8 == 0, "row blocks of 8");
     constexpr int TX = RL_TX, K = 2 * R + 1, B = 8;
     // est: [-2R, T+2R) both ways; y, ratio: [-R, T+R); tmp: cols [-R, TX+R), rows [-2R, TY+2R)
@@ -116,6 +148,10 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
     int tx, ty;
     tile_edge_first_xcd(blockIdx.x, order, tx, ty);
     const int ox = tx * TX, oy = ty * TY;
+    // The taps are symmetric bit for bit (k[i] and k[2R - i] come from the
+    // same expression), so taps i and 2R - i share one register, and a
+    // product k[i] x[m] that two outputs of a register-blocked task need is
+    // formed once.
     float kk[K];
 #pragma unroll
     for (int i = 0; i < K; i++) kk[i] = taps.k[i <= R ? i : K - 1 - i];
@@ -153,7 +189,13 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
         }
     }
 
-    if (interior) {
+    // est * corr is >= +0 or NaN: est is >= 0 (y >= 1e-6, then clamped), the
+    // taps are positive and every ratio is > 0, so clamp(x, 0, 64) is
+    // x > 64 ? 64 : x, with NaN kept as f32::clamp keeps it.
+    auto clamp64 = [](float x) { return x > 64.0f ? 64.0f : x; };
+    auto interior_passes = [&](auto fast_tag) {
+        constexpr bool FAST = decltype(fast_tag)::value;
+        bool y_ok = true;
         // Register-blocked passes: each task produces B consecutive outputs
         // from B + 2R staged samples.  Every output still accumulates its
         // taps in order 0..2R with separate multiply and add.  Horizontal
@@ -185,8 +227,12 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
             const float* yr = yp + (size_t)(oy + v0) * (size_t)w + (ox + u);
 #pragma unroll
             for (int j = 0; j < B; j++) {
-                yv[j] = v0 + j < TY + R ? *yr : 0.0f;
+                yv[j] = v0 + j < TY + R ? *yr : 1.0f;
                 yr += w;
+            }
+            if (FIRST) {
+#pragma unroll
+                for (int j = 0; j < B; j++) y_ok = y_ok && yv[j] <= kDivSafeMax;
             }
             const float* src = s_tmp + (v0 - R + 2 * R) * MS + (u + R);
             float x[B + 2 * R];
@@ -197,7 +243,8 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
                 float acc = kSumStart;
 #pragma unroll
                 for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
-                if (v0 + j < TY + R) s_rat[(v0 + j + R) * YS + (u + R)] = yv[j] / fmaxr(acc, 1e-6f);
+                const float c = fmaxr(acc, 1e-6f);
+                if (v0 + j < TY + R) s_rat[(v0 + j + R) * YS + (u + R)] = FAST ? div_in_range(yv[j], c) : yv[j] / c;
             }
         }
         __syncthreads();
@@ -236,10 +283,17 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
                 float acc = kSumStart;
 #pragma unroll
                 for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
-                *o = clampr(ev[q * B + j] * acc, 0.0f, 64.0f);
+                *o = clamp64(ev[q * B + j] * acc);
                 o += w;
             }
         }
+        if (FIRST && !y_ok) *flag = 1;
+    };
+    if (interior) {
+        if (FIRST || __builtin_amdgcn_readfirstlane(*flag) != 0)
+            interior_passes(std::false_type{});
+        else
+            interior_passes(std::true_type{});
         return;
     }
 
@@ -264,6 +318,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
 #pragma unroll
         for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(cv + j - R + 2 * R) * MS + (u + R)];
         const float yv = yp[(size_t)(oy + cv) * w + (ox + cu_of(u))];  // y at the clamped pixel
+        if (FIRST && !(yv <= kDivSafeMax)) *flag = 1;
         s_rat[(v + R) * YS + (u + R)] = yv / fmaxr(acc, 1e-6f);
     }
     __syncthreads();
@@ -285,7 +340,7 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
         float acc = kSumStart;
 #pragma unroll
         for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(v + j - R + R) * QS + u];
-        est_out[(size_t)gy * w + gx] = clampr(ev[k] * acc, 0.0f, 64.0f);
+        est_out[(size_t)gy * w + gx] = clamp64(ev[k] * acc);
     }
 }
 
@@ -300,11 +355,15 @@ __global__ void k_rl_update(float* __restrict__ est, const float* __restrict__ c
 }
 
 template <int R, int TY, int NT>
-hipError_t launch_iter_ty(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, hipStream_t st) {
+hipError_t launch_iter_ty(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, bool first, int* flag,
+                          hipStream_t st) {
     // interior tiles: ox >= 2R, oy >= 2R, ox + TX + 2R <= w, oy + TY + 2R <= h
     const TileOrder order = tile_order((int)w, (int)h, RL_TX, TY, 2 * R, 2 * R);
     dim3 grid((unsigned)(order.nx * order.ny));
-    hipLaunchKernelGGL((k_rl_iter<R, TY, NT>), grid, dim3(NT), 0, st, est, y, est_out, (int)w, (int)h, t, order);
+    if (first)
+        hipLaunchKernelGGL((k_rl_iter<R, TY, NT, true>), grid, dim3(NT), 0, st, est, y, est_out, (int)w, (int)h, t, order, flag);
+    else
+        hipLaunchKernelGGL((k_rl_iter<R, TY, NT, false>), grid, dim3(NT), 0, st, est, y, est_out, (int)w, (int)h, t, order, flag);
     return hipGetLastError();
 }
 
@@ -313,11 +372,12 @@ hipError_t launch_iter_ty(const float* est, const float* y, float* est_out, size
 // 8 % less halo work than 64x32), 1 = 64x16, 2 = 64x32 (26 KB, six tiles of
 // 256 threads per CU).
 template <int R>
-hipError_t launch_iter(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, int variant, hipStream_t st) {
+hipError_t launch_iter(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, int variant, bool first, int* flag,
+                       hipStream_t st) {
     switch (variant) {
-        case 1: return launch_iter_ty<R, 16, 256>(est, y, est_out, w, h, t, st);
-        case 2: return launch_iter_ty<R, 32, 256>(est, y, est_out, w, h, t, st);
-        default: return launch_iter_ty<R, 64, 512>(est, y, est_out, w, h, t, st);
+        case 1: return launch_iter_ty<R, 16, 256>(est, y, est_out, w, h, t, first, flag, st);
+        case 2: return launch_iter_ty<R, 32, 256>(est, y, est_out, w, h, t, first, flag, st);
+        default: return launch_iter_ty<R, 64, 512>(est, y, est_out, w, h, t, first, flag, st);
     }
 }
 
@@ -336,7 +396,8 @@ int gaussian_taps(float sigma, float* k) {
     return r;
 }
 
-size_t rl_workspace_floats(size_t w, size_t h) { return 5 * w * h; }
+// five planes and the range flag (64 floats keep the tail 256-byte aligned)
+size_t rl_workspace_floats(size_t w, size_t h) { return 5 * w * h + 64; }
 
 hipError_t launch_raw_details(const float* rgb, size_t w, size_t h, float sigma, size_t iters, float* ws, float* rgb_out, hipStream_t st,
                               int variant) {
@@ -348,7 +409,8 @@ hipError_t launch_raw_details(const float* rgb, size_t w, size_t h, float sigma,
     float* t0 = ws + 3 * n;
     float* t1 = ws + 4 * n;
     const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_rl_init, dim3(nb), dim3(256), 0, st, rgb, n, y, est);
+    int* flag = reinterpret_cast<int*>(ws + 5 * n);
+    hipLaunchKernelGGL(k_rl_init, dim3(nb), dim3(256), 0, st, rgb, n, y, est, flag);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const bool exact = sigma >= 0.3f && sigma < 2.0f;  // image.rs:328-333
@@ -358,12 +420,12 @@ hipError_t launch_raw_details(const float* rgb, size_t w, size_t h, float sigma,
     for (size_t it = 0; it < iters; it++) {
         if (exact) {
             switch (r) {
-                case 1: e = launch_iter<1>(est, y, est2, w, h, taps, variant, st); break;
-                case 2: e = launch_iter<2>(est, y, est2, w, h, taps, variant, st); break;
-                case 3: e = launch_iter<3>(est, y, est2, w, h, taps, variant, st); break;
-                case 4: e = launch_iter<4>(est, y, est2, w, h, taps, variant, st); break;
-                case 5: e = launch_iter<5>(est, y, est2, w, h, taps, variant, st); break;
-                case 6: e = launch_iter<6>(est, y, est2, w, h, taps, variant, st); break;
+                case 1: e = launch_iter<1>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
+                case 2: e = launch_iter<2>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
+                case 3: e = launch_iter<3>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
+                case 4: e = launch_iter<4>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
+                case 5: e = launch_iter<5>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
+                case 6: e = launch_iter<6>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
                 default: return hipErrorInvalidValue;
             }
             if (e != hipSuccess) return e;
