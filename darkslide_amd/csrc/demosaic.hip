@@ -58,10 +58,6 @@ struct Tile {
         __device__ __forceinline__ float operator()(int dx, int dy) const { return b[dy * W + dx]; }
     };
     __device__ __forceinline__ Sten sten(int x, int y) const { return Sten{lds_opaque(&(*this)(x, y))}; }
-    // the same for a byte plane of this shape
-    __device__ static __forceinline__ lds_cptr<uint8_t> sten_u8(const uint8_t* q, int x, int y) {
-        return lds_opaque(q + (y + H) * W + (x + H));
-    }
 };
 
 struct CfaSrc {
@@ -130,6 +126,52 @@ __device__ __forceinline__ float disc(const S& s) {
     return fabsf(0.5f - central) < fabsf(0.5f - neigh) ? neigh : central;
 }
 
+// Clip flags of a TX x TY tile (demosaic.rs:29-42): byte (ty + 1) * FW +
+// (tx + 1) holds cfa >= 0.995 at the clamped pixel (tx, ty), tx in [-1, TX],
+// ty in [-1, TY].  Rows are padded to TX + 4 bytes, so the 6 flags a run of
+// 4 outputs needs from one row are two aligned 32-bit words.
+template <int TX, int TY>
+struct ClipFlags {
+    static constexpr int FW = TX + 4, N = (TY + 2) * FW;
+    static_assert(TX % 4 == 0, "runs of 4 outputs");
+    __device__ static __forceinline__ int at(int tx, int ty) { return (ty + 1) * FW + (tx + 1); }
+};
+
+// The clip plane of a tile: clip = 1 where a flag in the clamped 3 x 3
+// neighbourhood is set, else 0.  A thread takes 4 consecutive outputs of a
+// row: the three flag rows' bytes are ORed as one 48-bit word, the 3-wide
+// horizontal OR is two shifts, and the row goes out as one 16-byte store
+// where the plane allows.
+template <int TX, int TY, int NT>
+__device__ __forceinline__ void store_clip(const uint8_t* __restrict__ flag, float* __restrict__ clip, int ox, int oy, int w, int h) {
+    constexpr int FW = ClipFlags<TX, TY>::FW, NQ = TX / 4;
+    const bool vec = (w & 3) == 0 && (reinterpret_cast<uintptr_t>(clip) & 15) == 0;
+    for (int t = threadIdx.x; t < TY * NQ; t += NT) {
+        const int y = t / NQ, x0 = 4 * (t % NQ);
+        const int gy = oy + y, gx = ox + x0;
+        if (gy >= h || gx >= w) continue;
+        uint64_t v = 0;
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) {
+            // flags of columns x0 - 1 .. x0 + 4 (bytes 0..5; 6, 7 are the next two)
+            const uint32_t* r = reinterpret_cast<const uint32_t*>(flag + (y + dy) * FW + x0);
+            v |= (uint64_t)r[0] | ((uint64_t)r[1] << 32);
+        }
+        v &= 0xffffffffffffull;
+        const uint32_t m = (uint32_t)(v | (v >> 8) | (v >> 16));  // byte i: output x0 + i, 0 or 1
+        const float o0 = (float)(m & 0xffu), o1 = (float)((m >> 8) & 0xffu), o2 = (float)((m >> 16) & 0xffu), o3 = (float)(m >> 24);
+        float* dst = clip + (size_t)gy * (size_t)w + (size_t)gx;
+        if (vec && gx + 4 <= w) {
+            *reinterpret_cast<float4*>(dst) = make_float4(o0, o1, o2, o3);
+        } else {
+            dst[0] = o0;
+            if (gx + 1 < w) dst[1] = o1;
+            if (gx + 2 < w) dst[2] = o2;
+            if (gx + 3 < w) dst[3] = o3;
+        }
+    }
+}
+
 // Halo of each stage, in pixels around the output tile.
 constexpr int H_CM = 10;   // cfa*mul
 constexpr int H_BVH = 7;   // bv, bh   (later bp, bq at 5)
@@ -155,7 +197,7 @@ struct RcdLds {
     float cm[Tile<TX, TY, H_CM>::N];
     float b[NB];
     float c[NC];
-    uint8_t flag[Tile<TX, TY, H_FLAG>::N];
+    alignas(4) uint8_t flag[ClipFlags<TX, TY>::N];
 };
 
 // Tiles whose 10-pixel halo lies inside [4, w-4) x [4, h-4) (INTERIOR) take
@@ -258,7 +300,7 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
             cm(tx, ty) = fmaxr(v0 * (odd ? m2 : m0), 0.0f);
             cm(tx + 1, ty) = fmaxr(v1 * (odd ? m3 : m1), 0.0f);
             if (clip && ty >= -H_FLAG && ty < TY + H_FLAG) {
-                uint8_t* f = L.flag + (ty + H_FLAG) * Tile<TX, TY, H_FLAG>::W + (tx + H_FLAG);
+                uint8_t* f = L.flag + ClipFlags<TX, TY>::at(tx, ty);
                 if (tx >= -H_FLAG && tx < TX + H_FLAG) f[0] = v0 >= kClip ? 1 : 0;
                 if (tx + 1 >= -H_FLAG && tx + 1 < TX + H_FLAG) f[1] = v1 >= kClip ? 1 : 0;
             }
@@ -277,7 +319,7 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
                 const float m = v * src.mul_of(pat.at(gx, gy));
                 cm(tx, ty) = INTERIOR ? fmaxr(m, 0.0f) : m;
                 if (clip && tx >= -H_FLAG && tx < TX + H_FLAG && ty >= -H_FLAG && ty < TY + H_FLAG)
-                    L.flag[(ty + H_FLAG) * Tile<TX, TY, H_FLAG>::W + (tx + H_FLAG)] = v >= kClip ? 1 : 0;
+                    L.flag[ClipFlags<TX, TY>::at(tx, ty)] = v >= kClip ? 1 : 0;
             });
     }
     __syncthreads();
@@ -459,15 +501,6 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
         rgb[3 * gi + 0] = r;
         rgb[3 * gi + 1] = g;
         rgb[3 * gi + 2] = b;
-        if (clip) {
-            const auto f = Tile<TX, TY, H_FLAG>::sten_u8(L.flag, tx - 1, ty - 1);
-            uint8_t m = 0;
-#pragma unroll
-            for (int dy = 0; dy < 3; dy++)
-#pragma unroll
-                for (int dx = 0; dx < 3; dx++) m |= f[dy * Tile<TX, TY, H_FLAG>::W + dx];
-            clip[gi] = m ? 1.0f : 0.0f;
-        }
     };
     // R and B at G sites (demosaic.rs:310-341)
     for (int i = tid; i < (TX / 2) * TY; i += NT) {
@@ -511,6 +544,7 @@ __device__ __forceinline__ void rcd_body(RcdLds<TX, TY>& L, const CfaSrc& src, f
         if (ox + tx >= w || oy + ty >= h) continue;
         store(tx, ty, r43(tx, ty), gg(tx, ty), b43(tx, ty));
     }
+    if (clip) store_clip<TX, TY, NT>(L.flag, clip, ox, oy, w, h);
 }
 
 template <int TX, int TY, int NT>
@@ -532,7 +566,7 @@ template <int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) void k_mhc(CfaSrc src, float* __restrict__ rgb, float* __restrict__ clip) {
     constexpr int H = 2;
     __shared__ float s_cm[Tile<TX, TY, H>::N];
-    __shared__ uint8_t s_flag[Tile<TX, TY, 1>::N];
+    __shared__ __attribute__((aligned(4))) uint8_t s_flag[ClipFlags<TX, TY>::N];
     Tile<TX, TY, H> cm{s_cm};
     const int tid = threadIdx.x;
     const int ox = blockIdx.x * TX, oy = blockIdx.y * TY;
@@ -549,7 +583,7 @@ __global__ __launch_bounds__(NT) void k_mhc(CfaSrc src, float* __restrict__ rgb,
                 int tx = i % CW - H, ty = i / CW - H;
                 int gx = clampi(ox + tx, 0, w - 1), gy = clampi(oy + ty, 0, h - 1);
                 cm(tx, ty) = v * src.mul_of(pat.at(gx, gy));
-                if (clip && tx >= -1 && tx <= TX && ty >= -1 && ty <= TY) s_flag[(ty + 1) * Tile<TX, TY, 1>::W + (tx + 1)] = v >= kClip ? 1 : 0;
+                if (clip && tx >= -1 && tx <= TX && ty >= -1 && ty <= TY) s_flag[ClipFlags<TX, TY>::at(tx, ty)] = v >= kClip ? 1 : 0;
             });
     }
     __syncthreads();
@@ -563,13 +597,8 @@ __global__ __launch_bounds__(NT) void k_mhc(CfaSrc src, float* __restrict__ rgb,
         rgb[3 * gi + 0] = mhc_at(cm, tx, ty, col, row_c, 0);
         rgb[3 * gi + 1] = mhc_at(cm, tx, ty, col, row_c, 1);
         rgb[3 * gi + 2] = mhc_at(cm, tx, ty, col, row_c, 2);
-        if (clip) {
-            uint8_t m = 0;
-            for (int dy = -1; dy <= 1; dy++)
-                for (int dx = -1; dx <= 1; dx++) m |= s_flag[(ty + dy + 1) * Tile<TX, TY, 1>::W + (tx + dx + 1)];
-            clip[gi] = m ? 1.0f : 0.0f;
-        }
     }
+    if (clip) store_clip<TX, TY, NT>(s_flag, clip, ox, oy, w, h);
 }
 
 template <int TX, int TY, int NT>
