@@ -584,9 +584,15 @@ constexpr int FS_RW = FS_W + 2;             // ring row: 1 zero pixel each side
 constexpr int FS_SH = 192;
 constexpr int FS_NT = 256;
 constexpr size_t FS_ROW_BYTES = (size_t)FS_RW * C * sizeof(uint16_t);
-// pass A: rings a1..a3, x ring, W_in/b_in; pass B: rings a4..a7, output partials, W_out
-constexpr size_t FS_LDS_A = 3 * 3 * FS_ROW_BYTES + 3 * FS_RW * sizeof(float);  // rings a1..a3, x ring
-constexpr size_t FS_LDS_B = 4 * 3 * FS_ROW_BYTES;                              // rings a4..a7
+// Pass A's input-layer operands for one a1 row: per computed column the
+// 32-element K vector of k_conv_in (taps 0..8 as bf16, then zeros) in four
+// 16-byte chunks, chunk-major (FS_IM_CHUNK bytes per chunk, the column's
+// ring entry inside it).  Chunks 2 and 3 are zeros.
+constexpr size_t FS_IM_CHUNK = (size_t)FS_W * 16;
+constexpr size_t FS_IM_BYTES = 4 * FS_IM_CHUNK;
+// pass A: rings a1..a3, input-layer operands, x ring; pass B: rings a4..a7
+constexpr size_t FS_LDS_A = 3 * 3 * FS_ROW_BYTES + FS_IM_BYTES + 3 * FS_RW * sizeof(float);
+constexpr size_t FS_LDS_B = 4 * 3 * FS_ROW_BYTES;
 
 // Ring rows are stored class-major: computed column j (0 <= j < FS_W) sits
 // at entry (j % FS_NC) * 16 + j / FS_NC, so the MFMA column tile "class s"
@@ -690,7 +696,8 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     extern __shared__ __attribute__((aligned(16))) uint8_t fs_lds[];
     constexpr int NR = PASS == 0 ? 3 : 4;  // LDS rings
     uint16_t* ring = reinterpret_cast<uint16_t*>(fs_lds);
-    float* xr = reinterpret_cast<float*>(fs_lds + NR * 3 * FS_ROW_BYTES);  // A: x ring [3][FS_RW]
+    uint8_t* im = fs_lds + NR * 3 * FS_ROW_BYTES;                       // A: input-layer operands of the next a1 row
+    float* xr = reinterpret_cast<float*>(im + FS_IM_BYTES);             // A: x ring [3][FS_RW]
     const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, ln = lane & 15;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x0 = blockIdx.x * FS_OUT, y0 = blockIdx.y * FS_SH;
@@ -700,7 +707,8 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     // ring k, slot s (= row mod 3)
     auto RING = [&](int k, int slot) { return ring + (size_t)(k * 3 + slot) * FS_RW * C; };
 
-    for (int i = tid; i < (int)(NR * 3 * FS_ROW_BYTES / 16); i += FS_NT) reinterpret_cast<uint4*>(fs_lds)[i] = make_uint4(0, 0, 0, 0);
+    constexpr int ZERO_BYTES = NR * 3 * FS_ROW_BYTES + (PASS == 0 ? FS_IM_BYTES : 0);  // rings and (A) the operand chunks
+    for (int i = tid; i < ZERO_BYTES / 16; i += FS_NT) reinterpret_cast<uint4*>(fs_lds)[i] = make_uint4(0, 0, 0, 0);
     // input / output layer operands (as k_conv_in / k_conv_out: W_out[ky][kx] in row kx of the A operand)
     bf16x8 win = {}, wout[3];
     float bin[4] = {0, 0, 0, 0};
@@ -772,8 +780,54 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
             }
         }
     };
-    // step r consumes input rows r-1..r+1 and fetches r+2; the first step is
-    // r = y0 - 3, a multiple of 3
+    // A: input-layer operands of an a1 row from the x rows above, at and below
+    // it (ring slots s0, s1, s2), cast to bf16 as k_conv_in does.  Entry e
+    // holds computed column j = (e % 16) FS_NC + e / 16 (the inverse of
+    // fs_entry), so consecutive lanes write consecutive entries.  Waves 0 and
+    // 1 write taps 0..3 of entries 0..95, waves 2 and 3 taps 4..8.
+    auto build_im = [&](int s0, int s1, int s2) {
+        if (PASS == 0 && lane < FS_W / 2) {
+            const int e = (wave & 1) * (FS_W / 2) + lane;
+            const int j = (e & 15) * FS_NC + (e >> 4);
+            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+            if (wave < 2) {
+                const float* a = xr + s0 * FS_RW + j;
+                const float* b = xr + s1 * FS_RW + j;
+                const bf16x4 v = {(__bf16)a[0], (__bf16)a[1], (__bf16)a[2], (__bf16)b[0]};
+                *reinterpret_cast<bf16x4*>(im + e * 16) = v;
+            } else {
+                const float* b = xr + s1 * FS_RW + j;
+                const float* c = xr + s2 * FS_RW + j;
+                const bf16x4 v = {(__bf16)b[1], (__bf16)b[2], (__bf16)c[0], (__bf16)c[1]};
+                *reinterpret_cast<bf16x4*>(im + e * 16 + 8) = v;
+                *reinterpret_cast<__bf16*>(im + FS_IM_CHUNK + e * 16) = (__bf16)c[2];  // the chunk's other elements stay 0
+            }
+        }
+    };
+    // A: byte offset of this lane's input-layer operand (class FS_CPW half + q
+    // at + 256 q): chunk g, the 16 lanes of a group on consecutive entries
+    const int im_off = g * (int)FS_IM_CHUNK + (FS_CPW * half * 16 + ln) * 16;
+    // A: the a4 row of a step is written to HBM at the start of the next
+    // step, after that step's x fetch: a wait for a fetched row then finds
+    // these stores a step old instead of just issued
+    uint2 a4v[FS_CPW];
+    bool a4_pend = false;
+    size_t a4_off = 0;  // element offset of the pending row
+    const int a4_j0 = FS_CPW * half + FS_NC * ln;
+    auto flush_a4 = [&]() {
+        if (PASS == 0 && a4_pend) {
+            uint16_t* p = act_out + a4_off + (size_t)(gx0 + a4_j0) * C + mt * 16 + 4 * g;
+#pragma unroll
+            for (int q = 0; q < FS_CPW; q++) {
+                const int j = a4_j0 + q;
+                if (j >= FS_HALO && j < FS_HALO + FS_OUT && gx0 + j < Wd) *reinterpret_cast<uint2*>(p + q * C) = a4v[q];
+            }
+        }
+    };
+
+    // step r of B consumes input rows r-1..r+1 and fetches r+2; step r of A
+    // reads the x rows r..r+2 (building the operands of a1 row r + 1) and
+    // fetches r+3; the first step is r = y0 - 3, a multiple of 3
     const int r_first = y0 - 3;
     const int r_end = y0 + FS_SH + 4;
     fetch(r_first - 1);
@@ -782,7 +836,13 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     fetch(r_first);
     commit(0);
     fetch(r_first + 1);
-    __syncthreads();  // the first step reads rows r_first - 1 and r_first before its barrier
+    if (PASS == 0) {
+        commit(1);
+        fetch(r_first + 2);
+        __syncthreads();
+        build_im(2, 0, 1);  // a1 row r_first
+    }
+    __syncthreads();  // the first step reads rows r_first - 1 and r_first (A: the operands of a1 row r_first) before its barrier
 
     // One step at row r with r mod 3 == SUB.  Of the three source rows of a
     // layer only the last (kernel row ky = 2) is written in the same step, so
@@ -795,8 +855,14 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         auto slot = [](int c) { return ((SUB + c) % 3 + 3) % 3; };  // constant-folded
         auto src_row = [&](int l, int ky) { return RING(l, slot(BASE - l + ky - 1)); };  // layer l, kernel row ky
         const bool a4_row = PASS == 0 && r + BASE - 2 >= y0 && r + BASE - 2 < y0 + FS_SH;  // A: layer 2 feeds a stored a4 row
-        commit(slot(1));
-        fetch(r + 2);
+        if (PASS == 0) {
+            commit(slot(2));
+            fetch(r + 3);
+            flush_a4();
+        } else {
+            commit(slot(1));
+            fetch(r + 2);
+        }
         // B: the output pixels of this step (row r - 3) and their x (and L),
         // loaded now so their latency passes under the step's layers
         bool o_on[FS_CPW];
@@ -812,28 +878,23 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
             o_x[q] = o_on[q] ? x[o_p[q]] : 0.0f;
             o_l[q] = o_on[q] && lum ? lum[o_p[q]] : 0.0f;
         }
-        f32x4 acc[FS_CPW];
-#pragma unroll
-        for (int q = 0; q < FS_CPW; q++) acc[q] = f32x4{0, 0, 0, 0};
-        fs_mma_row(acc, wa[0], 0, src_row(0, 0), lo);
-        fs_mma_row(acc, wa[0], 1, src_row(0, 1), lo);
-        __syncthreads();
         if (PASS == 0) {
-            // ---- a1 row r into ring 0 (one MFMA per class, K = taps; as k_conv_in)
+            // ---- a1 row r into ring 0 (one MFMA per class, K = taps; as
+            // k_conv_in) from the operands built in the step before.  The slot
+            // held row r - 3, last read before the previous step's first
+            // barrier, and this step's barrier publishes the row.
             uint16_t* dst = RING(0, slot(0));
             const bool row_in = r >= 0 && r < H;
+            bf16x8 bv[FS_CPW];
+            f32x4 a1[FS_CPW];
+#pragma unroll
+            for (int q = 0; q < FS_CPW; q++) bv[q] = lds_at<bf16x8>(im, im_off + q * 256);
+#pragma unroll
+            for (int q = 0; q < FS_CPW; q++) a1[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(win, bv[q], f32x4{0, 0, 0, 0}, 0, 0, 0);
 #pragma unroll
             for (int q = 0; q < FS_CPW; q++) {
+                const f32x4 a = a1[q];
                 const int j = FS_CPW * half + q + FS_NC * ln;
-                bf16x8 bv;
-#pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const int t = 8 * g + e;
-                    const int ty = t < 9 ? t / 3 : 0, tx = t < 9 ? t % 3 : 0;
-                    const float v = t < 9 ? xr[slot(ty - 1) * FS_RW + j + tx] : 0.0f;
-                    bv[e] = (__bf16)v;
-                }
-                const f32x4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(win, bv, f32x4{0, 0, 0, 0}, 0, 0, 0);
                 const int gx = gx0 + j;
                 const uint32_t keep = (row_in && gx >= 0 && gx < Wd) ? 0xffffffffu : 0u;
                 uint2 o;
@@ -841,8 +902,17 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
                 o.y = fs_pack(a[2], bin[2], a[3], bin[3]) & keep;
                 *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + q * 1024) = o;
             }
-            __syncthreads();
         }
+        f32x4 acc[FS_CPW];
+#pragma unroll
+        for (int q = 0; q < FS_CPW; q++) acc[q] = f32x4{0, 0, 0, 0};
+        fs_mma_row(acc, wa[0], 0, src_row(0, 0), lo);
+        fs_mma_row(acc, wa[0], 1, src_row(0, 1), lo);
+        __syncthreads();
+        // A: operands of a1 row r + 1 from x rows r .. r + 2 (row r + 2 was
+        // committed at the start of this step); the next reads of them follow
+        // this step's layer barriers
+        build_im(slot(0), slot(1), slot(2));
         // ---- three MFMA layers: ring k -> ring k+1 (A: rows r-1, r-2, r-3; B: rows r, r-1, r-2)
         // B: output row r - 3 (as k_conv_out): E_kx of window class i in row kx of a
         // three-MFMA chain over ky, then y = x + (((E_0(col - 1) + E_1(col)) + E_2(col + 1)) + b_out);
@@ -866,8 +936,16 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
 #pragma unroll
             for (int q = 0; q < FS_CPW; q++) nxt[q] = f32x4{0, 0, 0, 0};
             if (PASS == 0 && l == 2) {
-                // a4: stored columns and rows of this block go to HBM
-                if (on) fs_store<true>(acc, act_out + ioff * C + (size_t)orow * Wd * C, bm[l], lane, mt, half, lo, row_in, gx0, Wd, FS_HALO, FS_HALO + FS_OUT);
+                // a4: stored columns and rows of this block go to HBM (next step)
+                a4_pend = on && row_in;
+                if (a4_pend) {
+#pragma unroll
+                    for (int q = 0; q < FS_CPW; q++) {
+                        a4v[q].x = fs_pack(acc[q][0], bm[l][0], acc[q][1], bm[l][1]);
+                        a4v[q].y = fs_pack(acc[q][2], bm[l][2], acc[q][3], bm[l][3]);
+                    }
+                    a4_off = (ioff + (size_t)orow * Wd) * C;
+                }
             } else {
                 fs_store<false>(acc, RING(l + 1, slot(BASE - l)), bm[l], lane, mt, half, lo, row_in, gx0, Wd, 0, 0);
                 if (l < 2) {
@@ -902,6 +980,7 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         if (rb + 1 < r_end) step(std::integral_constant<int, 1>{}, rb + 1);
         if (rb + 2 < r_end) step(std::integral_constant<int, 2>{}, rb + 2);
     }
+    flush_a4();
 }
 
 unsigned nb(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
