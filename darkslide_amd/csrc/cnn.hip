@@ -807,21 +807,33 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     // A: byte offset of this lane's input-layer operand (class FS_CPW half + q
     // at + 256 q): chunk g, the 16 lanes of a group on consecutive entries
     const int im_off = g * (int)FS_IM_CHUNK + (FS_CPW * half * 16 + ln) * 16;
-    // A: the a4 row of a step is written to HBM at the start of the next
-    // step, after that step's x fetch: a wait for a fetched row then finds
-    // these stores a step old instead of just issued
-    uint2 a4v[FS_CPW];
-    bool a4_pend = false;
-    size_t a4_off = 0;  // element offset of the pending row
-    const int a4_j0 = FS_CPW * half + FS_NC * ln;
-    auto flush_a4 = [&]() {
-        if (PASS == 0 && a4_pend) {
-            uint16_t* p = act_out + a4_off + (size_t)(gx0 + a4_j0) * C + mt * 16 + 4 * g;
+    // The row a step sends to HBM (A: a4, B: output pixels) is written at
+    // the start of the next step, after that step's input fetch: a wait for
+    // a fetched row then finds these stores a step old instead of just issued.
+    const int o_j0 = FS_CPW * half + FS_NC * ln;  // computed column of class FS_CPW half (+ q for class + q)
+    bool o_col[FS_CPW];                           // this lane stores column o_j0 + q of a stored row
 #pragma unroll
-            for (int q = 0; q < FS_CPW; q++) {
-                const int j = a4_j0 + q;
-                if (j >= FS_HALO && j < FS_HALO + FS_OUT && gx0 + j < Wd) *reinterpret_cast<uint2*>(p + q * C) = a4v[q];
-            }
+    for (int q = 0; q < FS_CPW; q++) {
+        const int j = o_j0 + q;
+        // B: of the wave half's classes the mt = 0 wave owns the first FS_CPW - 1, the
+        // mt = 1 wave the last; one lane group (g = 0) holds a column's output
+        o_col[q] = j >= FS_HALO && j < FS_HALO + FS_OUT && gx0 + j < Wd && (PASS == 0 || ((q < FS_CPW - 1) == (mt == 0) && g == 0));
+    }
+    uint2 a4v[FS_CPW];
+    float ov[FS_CPW];
+    bool ov_pend = false;
+    size_t ov_p = 0;  // A: element offset of the row in act_out; B: pixel index of column o_j0
+    auto flush_out = [&]() {
+        if (!ov_pend) return;
+        if (PASS == 0) {
+            uint16_t* p = act_out + ov_p + (size_t)(gx0 + o_j0) * C + mt * 16 + 4 * g;
+#pragma unroll
+            for (int q = 0; q < FS_CPW; q++)
+                if (o_col[q]) *reinterpret_cast<uint2*>(p + q * C) = a4v[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < FS_CPW; q++)
+                if (o_col[q]) out[ov_p + q] = ov[q];
         }
     };
 
@@ -858,25 +870,23 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         if (PASS == 0) {
             commit(slot(2));
             fetch(r + 3);
-            flush_a4();
         } else {
             commit(slot(1));
             fetch(r + 2);
         }
+        flush_out();
         // B: the output pixels of this step (row r - 3) and their x (and L),
         // loaded now so their latency passes under the step's layers
+        const int orow = r - 3;
+        const bool orow_on = PASS == 1 && orow >= y0 && orow < y0 + FS_SH && orow < H;
+        const size_t orow_p = ioff + (size_t)orow * Wd + gx0 + o_j0;
         bool o_on[FS_CPW];
-        size_t o_p[FS_CPW];
         float o_x[FS_CPW], o_l[FS_CPW];
 #pragma unroll
         for (int q = 0; q < FS_CPW; q++) {
-            const int orow = r - 3;
-            const int j = FS_CPW * half + q + FS_NC * ln, gx = gx0 + j;
-            o_on[q] = PASS == 1 && (q < FS_CPW - 1) == (mt == 0) && g == 0 && orow >= y0 && orow < y0 + FS_SH && orow < H && j >= FS_HALO &&
-                      j < FS_HALO + FS_OUT && gx < Wd;
-            o_p[q] = o_on[q] ? ioff + (size_t)orow * Wd + gx : 0;
-            o_x[q] = o_on[q] ? x[o_p[q]] : 0.0f;
-            o_l[q] = o_on[q] && lum ? lum[o_p[q]] : 0.0f;
+            o_on[q] = orow_on && o_col[q];
+            o_x[q] = o_on[q] ? x[orow_p + q] : 0.0f;
+            o_l[q] = o_on[q] && lum ? lum[orow_p + q] : 0.0f;
         }
         if (PASS == 0) {
             // ---- a1 row r into ring 0 (one MFMA per class, K = taps; as
@@ -937,14 +947,14 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
             for (int q = 0; q < FS_CPW; q++) nxt[q] = f32x4{0, 0, 0, 0};
             if (PASS == 0 && l == 2) {
                 // a4: stored columns and rows of this block go to HBM (next step)
-                a4_pend = on && row_in;
-                if (a4_pend) {
+                ov_pend = on && row_in;
+                if (ov_pend) {
 #pragma unroll
                     for (int q = 0; q < FS_CPW; q++) {
                         a4v[q].x = fs_pack(acc[q][0], bm[l][0], acc[q][1], bm[l][1]);
                         a4v[q].y = fs_pack(acc[q][2], bm[l][2], acc[q][3], bm[l][3]);
                     }
-                    a4_off = (ioff + (size_t)orow * Wd) * C;
+                    ov_p = (ioff + (size_t)orow * Wd) * C;
                 }
             } else {
                 fs_store<false>(acc, RING(l + 1, slot(BASE - l)), bm[l], lane, mt, half, lo, row_in, gx0, Wd, 0, 0);
@@ -970,9 +980,11 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
             for (int q = 0; q < FS_CPW; q++) {
                 if (o_on[q]) {
                     const float y = o_x[q] + (((e[q][0] + e[q + 1][1]) + e[q + 2][2]) + b_out);
-                    out[o_p[q]] = lum ? o_l[q] + s_lum * (y - o_x[q]) : y;
+                    ov[q] = lum ? o_l[q] + s_lum * (y - o_x[q]) : y;
                 }
             }
+            ov_pend = orow_on;
+            ov_p = orow_p;
         }
     };
     for (int rb = r_first; rb < r_end; rb += 3) {
@@ -980,7 +992,7 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         if (rb + 1 < r_end) step(std::integral_constant<int, 1>{}, rb + 1);
         if (rb + 2 < r_end) step(std::integral_constant<int, 2>{}, rb + 2);
     }
-    flush_a4();
+    flush_out();
 }
 
 unsigned nb(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
