@@ -584,12 +584,14 @@ constexpr int FS_RW = FS_W + 2;             // ring row: 1 zero pixel each side
 constexpr int FS_SH = 192;
 constexpr int FS_NT = 256;
 constexpr size_t FS_ROW_BYTES = (size_t)FS_RW * C * sizeof(uint16_t);
-// Pass A's input-layer operands for one a1 row: per computed column the
-// 32-element K vector of k_conv_in (taps 0..8 as bf16, then zeros) in four
-// 16-byte chunks, chunk-major (FS_IM_CHUNK bytes per chunk, the column's
-// ring entry inside it).  Chunks 2 and 3 are zeros.
+// Pass A's input-layer operands for one a1 row (one slot): per computed
+// column the 32-element K vector of k_conv_in (taps 0..8 as bf16, then
+// zeros) in four 16-byte chunks, chunk-major (FS_IM_CHUNK bytes per chunk,
+// the column's ring entry inside it).  Chunks 2 and 3 are zeros.  Three
+// slots, one per a1 row mod 3.
 constexpr size_t FS_IM_CHUNK = (size_t)FS_W * 16;
-constexpr size_t FS_IM_BYTES = 4 * FS_IM_CHUNK;
+constexpr size_t FS_IM_SLOT = 4 * FS_IM_CHUNK;
+constexpr size_t FS_IM_BYTES = 3 * FS_IM_SLOT;
 // pass A: rings a1..a3, input-layer operands, x ring; pass B: rings a4..a7
 constexpr size_t FS_LDS_A = 3 * 3 * FS_ROW_BYTES + FS_IM_BYTES + 3 * FS_RW * sizeof(float);
 constexpr size_t FS_LDS_B = 4 * 3 * FS_ROW_BYTES;
@@ -658,31 +660,36 @@ __device__ __forceinline__ void fs_mma_row(f32x4 (&acc)[FS_CPW], const bf16x8 (&
         for (int q = 0; q < FS_CPW; q++) acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[3 * ky + kx], b[q + kx], acc[q], 0, 0, 0);
 }
 
-// Epilogue of one MFMA layer row: relu(acc + bias) as bf16 into the dst row
-// (the wave's classes).  GLOBAL_OUT: store the stored columns to HBM instead
-// of an LDS ring.
-template <bool GLOBAL_OUT>
-__device__ __forceinline__ void fs_store(const f32x4 (&acc)[FS_CPW], uint16_t* dst, const float (&bias)[4], int lane, int mt, int half,
-                                         const FsLane& lo, bool row_in, int gx0, int Wd, int store_lo, int store_hi) {
-    const int g = lane >> 4, ln = lane & 15;
+// Epilogue of one MFMA layer row: relu(acc + bias) as bf16, packed per class
+// (channels 4g .. 4g + 3 of the wave's M tile), zero outside the image
+// (row_in false, or col_in[q] false for the class's column).
+__device__ __forceinline__ void fs_pack_row(const f32x4 (&acc)[FS_CPW], const float (&bias)[4], bool row_in, const bool (&col_in)[FS_CPW],
+                                            uint2 (&o)[FS_CPW]) {
 #pragma unroll
     for (int q = 0; q < FS_CPW; q++) {
-        const int j = FS_CPW * half + q + FS_NC * ln;
-        const int gx = gx0 + j;
-        const bool in = row_in && gx >= 0 && gx < Wd;
-        const uint32_t keep = in ? 0xffffffffu : 0u;
-        uint2 o;
-        o.x = fs_pack(acc[q][0], bias[0], acc[q][1], bias[1]) & keep;
-        o.y = fs_pack(acc[q][2], bias[2], acc[q][3], bias[3]) & keep;
-        if (GLOBAL_OUT) {
-            if (in && j >= store_lo && j < store_hi) *reinterpret_cast<uint2*>(dst + (size_t)gx * C + mt * 16 + 4 * g) = o;
-        } else {
-            *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + q * 1024) = o;
-        }
+        const uint32_t keep = row_in && col_in[q] ? 0xffffffffu : 0u;
+        o[q].x = fs_pack(acc[q][0], bias[0], acc[q][1], bias[1]) & keep;
+        o[q].y = fs_pack(acc[q][2], bias[2], acc[q][3], bias[3]) & keep;
     }
+}
+// the wave's classes of a packed row into a ring row
+__device__ __forceinline__ void fs_put_row(const uint2 (&o)[FS_CPW], uint16_t* dst, const FsLane& lo) {
+#pragma unroll
+    for (int q = 0; q < FS_CPW; q++) *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + q * 1024) = o[q];
 }
 
 // PASS 0 (A): x -> a4 (act_out, n*H*W*32 bf16).  PASS 1 (B): a4 (act_in) -> y.
+//
+// A step at row r has two phases.  Phase 1 reads only rows published before
+// the step and does all of the step's MFMAs: layer l of the pass computes
+// row r - 1 - 2l (A: a1 row r first, from its operands, then a2 row r - 2,
+// a3 row r - 4, a4 row r - 6; B: a5 row r - 1, a6 row r - 3, a7 row r - 5
+// and output row r - 7), so each layer's newest source row was written in
+// the step before.  After a barrier, phase 2 writes the new rows into the
+// ring slots of the rows phase 1 read last (row - 3), commits the next input
+// row, fetches the one after and sends this step's a4 / output row to HBM;
+// a second barrier publishes them.  A layer whose row no stored row depends
+// on is skipped.
 // The row loop is unrolled by three (y0 and the first row are multiples of
 // three), so every ring slot ((row) mod 3) is a compile-time constant and
 // the LDS addresses are a lane's fixed offsets plus immediates.
@@ -696,8 +703,8 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     extern __shared__ __attribute__((aligned(16))) uint8_t fs_lds[];
     constexpr int NR = PASS == 0 ? 3 : 4;  // LDS rings
     uint16_t* ring = reinterpret_cast<uint16_t*>(fs_lds);
-    uint8_t* im = fs_lds + NR * 3 * FS_ROW_BYTES;                       // A: input-layer operands of the next a1 row
-    float* xr = reinterpret_cast<float*>(im + FS_IM_BYTES);             // A: x ring [3][FS_RW]
+    uint8_t* im = fs_lds + NR * 3 * FS_ROW_BYTES;            // A: input-layer operands, 3 slots (a1 row mod 3)
+    float* xr = reinterpret_cast<float*>(im + FS_IM_BYTES);  // A: x ring [3][FS_RW]
     const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, ln = lane & 15;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x0 = blockIdx.x * FS_OUT, y0 = blockIdx.y * FS_SH;
@@ -739,6 +746,33 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
     }
     const int half = wave >> 1;
     const FsLane lo = fs_lane(lane, mt, half);
+    // computed column of this lane in class FS_CPW half + q is o_j0 + q
+    const int o_j0 = FS_CPW * half + FS_NC * ln;
+    bool col_in[FS_CPW];  // the column lies in the image
+    bool o_col[FS_CPW];   // this lane sends the column of a stored row to HBM
+#pragma unroll
+    for (int q = 0; q < FS_CPW; q++) {
+        const int j = o_j0 + q, gx = gx0 + j;
+        col_in[q] = gx >= 0 && gx < Wd;
+        // B: of the wave half's classes the mt = 0 wave owns the first FS_CPW - 1, the
+        // mt = 1 wave the last; one lane group (g = 0) holds a column's output
+        o_col[q] = j >= FS_HALO && j < FS_HALO + FS_OUT && gx < Wd && (PASS == 0 || ((q < FS_CPW - 1) == (mt == 0) && g == 0));
+    }
+    // B: the output layer.  A wave stores NQ classes at most: class oq[k] (mt = 0:
+    // k; mt = 1: FS_CPW - 1, for k = 0 only) from window classes k0 .. k0 + FS_CPW
+    // (k0 = 0 or FS_CPW - 1, at ring byte offsets ob[]; for mt = 1 the last is a
+    // repeat that no output uses)
+    constexpr int NQ = FS_CPW - 1;
+    int oq[NQ];
+    bool oq_on[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+        oq[k] = mt ? FS_CPW - 1 : k;
+        oq_on[k] = mt ? k == 0 && o_col[FS_CPW - 1] : o_col[k];
+    }
+    int ob[FS_CPW + 1];
+#pragma unroll
+    for (int k = 0; k < FS_CPW + 1; k++) ob[k] = mt ? lo.b[FS_CPW - 1 + k < FS_WIN ? FS_CPW - 1 + k : FS_WIN - 1] : lo.b[k];
 
     // input rows: A reads x rows (FS_RW floats, element i = global column
     // gx0 - 1 + i), B reads a4 rows (FS_RW pixels of 64 bytes, pixel i =
@@ -780,13 +814,15 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
             }
         }
     };
-    // A: input-layer operands of an a1 row from the x rows above, at and below
-    // it (ring slots s0, s1, s2), cast to bf16 as k_conv_in does.  Entry e
-    // holds computed column j = (e % 16) FS_NC + e / 16 (the inverse of
-    // fs_entry), so consecutive lanes write consecutive entries.  Waves 0 and
-    // 1 write taps 0..3 of entries 0..95, waves 2 and 3 taps 4..8.
-    auto build_im = [&](int s0, int s1, int s2) {
+    // A: input-layer operands of an a1 row into operand slot si from the x
+    // rows above, at and below it (x ring slots s0, s1, s2), cast to bf16 as
+    // k_conv_in does.  Entry e holds computed column j = (e % 16) FS_NC + e / 16
+    // (the inverse of fs_entry), so consecutive lanes write consecutive
+    // entries.  Waves 0 and 1 write taps 0..3 of entries 0..95, waves 2 and 3
+    // taps 4..8.
+    auto build_im = [&](int si, int s0, int s1, int s2) {
         if (PASS == 0 && lane < FS_W / 2) {
+            uint8_t* d = im + si * FS_IM_SLOT;
             const int e = (wave & 1) * (FS_W / 2) + lane;
             const int j = (e & 15) * FS_NC + (e >> 4);
             typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -794,54 +830,26 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
                 const float* a = xr + s0 * FS_RW + j;
                 const float* b = xr + s1 * FS_RW + j;
                 const bf16x4 v = {(__bf16)a[0], (__bf16)a[1], (__bf16)a[2], (__bf16)b[0]};
-                *reinterpret_cast<bf16x4*>(im + e * 16) = v;
+                *reinterpret_cast<bf16x4*>(d + e * 16) = v;
             } else {
                 const float* b = xr + s1 * FS_RW + j;
                 const float* c = xr + s2 * FS_RW + j;
                 const bf16x4 v = {(__bf16)b[1], (__bf16)b[2], (__bf16)c[0], (__bf16)c[1]};
-                *reinterpret_cast<bf16x4*>(im + e * 16 + 8) = v;
-                *reinterpret_cast<__bf16*>(im + FS_IM_CHUNK + e * 16) = (__bf16)c[2];  // the chunk's other elements stay 0
+                *reinterpret_cast<bf16x4*>(d + e * 16 + 8) = v;
+                *reinterpret_cast<__bf16*>(d + FS_IM_CHUNK + e * 16) = (__bf16)c[2];  // the chunk's other elements stay 0
             }
         }
     };
-    // A: byte offset of this lane's input-layer operand (class FS_CPW half + q
-    // at + 256 q): chunk g, the 16 lanes of a group on consecutive entries
+    // A: byte offset of this lane's input-layer operand in an operand slot
+    // (class FS_CPW half + q at + 256 q): chunk g, the 16 lanes of a group on
+    // consecutive entries
     const int im_off = g * (int)FS_IM_CHUNK + (FS_CPW * half * 16 + ln) * 16;
-    // The row a step sends to HBM (A: a4, B: output pixels) is written at
-    // the start of the next step, after that step's input fetch: a wait for
-    // a fetched row then finds these stores a step old instead of just issued.
-    const int o_j0 = FS_CPW * half + FS_NC * ln;  // computed column of class FS_CPW half (+ q for class + q)
-    bool o_col[FS_CPW];                           // this lane stores column o_j0 + q of a stored row
-#pragma unroll
-    for (int q = 0; q < FS_CPW; q++) {
-        const int j = o_j0 + q;
-        // B: of the wave half's classes the mt = 0 wave owns the first FS_CPW - 1, the
-        // mt = 1 wave the last; one lane group (g = 0) holds a column's output
-        o_col[q] = j >= FS_HALO && j < FS_HALO + FS_OUT && gx0 + j < Wd && (PASS == 0 || ((q < FS_CPW - 1) == (mt == 0) && g == 0));
-    }
-    uint2 a4v[FS_CPW];
-    float ov[FS_CPW];
-    bool ov_pend = false;
-    size_t ov_p = 0;  // A: element offset of the row in act_out; B: pixel index of column o_j0
-    auto flush_out = [&]() {
-        if (!ov_pend) return;
-        if (PASS == 0) {
-            uint16_t* p = act_out + ov_p + (size_t)(gx0 + o_j0) * C + mt * 16 + 4 * g;
-#pragma unroll
-            for (int q = 0; q < FS_CPW; q++)
-                if (o_col[q]) *reinterpret_cast<uint2*>(p + q * C) = a4v[q];
-        } else {
-#pragma unroll
-            for (int q = 0; q < FS_CPW; q++)
-                if (o_col[q]) out[ov_p + q] = ov[q];
-        }
-    };
 
-    // step r of B consumes input rows r-1..r+1 and fetches r+2; step r of A
-    // reads the x rows r..r+2 (building the operands of a1 row r + 1) and
-    // fetches r+3; the first step is r = y0 - 3, a multiple of 3
+    // The first step is r = y0 - 3, a multiple of 3.  A: step r builds the
+    // operands of a1 row r + 1 from x rows r .. r + 2 and commits x row r + 3;
+    // B: step r reads a4 rows r - 2 .. r and commits a4 row r + 1.
     const int r_first = y0 - 3;
-    const int r_end = y0 + FS_SH + 4;
+    const int r_end = y0 + FS_SH + (PASS == 0 ? 6 : 7);
     fetch(r_first - 1);
     __syncthreads();  // ring zeroing done before any commit
     commit(2);        // row r_first - 1, slot 2
@@ -852,147 +860,125 @@ __global__ __launch_bounds__(FS_NT, 2) void k_cnn_fused(const float* __restrict_
         commit(1);
         fetch(r_first + 2);
         __syncthreads();
-        build_im(2, 0, 1);  // a1 row r_first
+        build_im(0, 2, 0, 1);  // a1 row r_first from x rows r_first - 1 .. r_first + 1
+        __syncthreads();       // x row r_first - 1 read
+        commit(2);             // x row r_first + 2
+        fetch(r_first + 3);
     }
-    __syncthreads();  // the first step reads rows r_first - 1 and r_first (A: the operands of a1 row r_first) before its barrier
+    __syncthreads();
 
-    // One step at row r with r mod 3 == SUB.  Of the three source rows of a
-    // layer only the last (kernel row ky = 2) is written in the same step, so
-    // the ky = 0, 1 MFMAs of the next layer run before the barrier that
-    // publishes it, beside the epilogue of the layer before; each
-    // accumulator still takes its taps in the order 0 .. 8.
-    constexpr int BASE = PASS == 0 ? -1 : 0;  // first middle layer's output row relative to r
+    constexpr size_t IM_SLOT = FS_IM_SLOT;
     auto step = [&](auto sub_tag, int r) {
         constexpr int SUB = decltype(sub_tag)::value;
-        auto slot = [](int c) { return ((SUB + c) % 3 + 3) % 3; };  // constant-folded
-        auto src_row = [&](int l, int ky) { return RING(l, slot(BASE - l + ky - 1)); };  // layer l, kernel row ky
-        const bool a4_row = PASS == 0 && r + BASE - 2 >= y0 && r + BASE - 2 < y0 + FS_SH;  // A: layer 2 feeds a stored a4 row
-        if (PASS == 0) {
-            commit(slot(2));
-            fetch(r + 3);
-        } else {
-            commit(slot(1));
-            fetch(r + 2);
-        }
-        flush_out();
-        // B: the output pixels of this step (row r - 3) and their x (and L),
-        // loaded now so their latency passes under the step's layers
-        const int orow = r - 3;
-        const bool orow_on = PASS == 1 && orow >= y0 && orow < y0 + FS_SH && orow < H;
-        const size_t orow_p = ioff + (size_t)orow * Wd + gx0 + o_j0;
-        bool o_on[FS_CPW];
-        float o_x[FS_CPW], o_l[FS_CPW];
+        auto slot = [](int c) { return ((SUB + c) % 3 + 3) % 3; };  // slot of row r + c; constant-folded
+        // the row each layer computes at this step, and whether a stored row needs it
+        // (S = FS_SH; each 3x3 layer needs one more row either side than the next):
+        // A: a1 r in [y0-3, y0+S+3), a2 r-2 in [y0-2, y0+S+2), a3 r-4 in [y0-1, y0+S+1), a4 r-6 in [y0, y0+S)
+        // B: a5 r-1 in [y0-3, y0+S+3), a6 r-3 in [y0-2, y0+S+2), a7 r-5 in [y0-1, y0+S+1), y r-7 in [y0, y0+S)
+        auto need = [&](int row, int lo_, int hi_) { return row >= y0 + lo_ && row < y0 + FS_SH + hi_; };
+        constexpr int D0 = PASS == 0 ? 2 : 1;  // layer l computes row r - D0 - 2 l
+        const int row_l[3] = {r - D0, r - D0 - 2, r - D0 - 4};
+        bool on_l[3];
 #pragma unroll
-        for (int q = 0; q < FS_CPW; q++) {
-            o_on[q] = orow_on && o_col[q];
-            o_x[q] = o_on[q] ? x[orow_p + q] : 0.0f;
-            o_l[q] = o_on[q] && lum ? lum[orow_p + q] : 0.0f;
+        for (int l = 0; l < 3; l++) {
+            constexpr int NL0 = PASS == 0 ? 2 : 3;  // rows needed around [y0, y0 + S) by layer 0
+            on_l[l] = need(row_l[l], l - NL0, NL0 - l);
         }
+        // B: output row and its x (and L), loaded now so their latency passes under phase 1
+        const int orow = r - 7;
+        const bool orow_on = PASS == 1 && need(orow, 0, 0) && orow < H;
+        const size_t orow_p = ioff + (size_t)orow * Wd + gx0 + o_j0;
+        float o_x[NQ], o_l[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+            const bool on = orow_on && oq_on[k];
+            o_x[k] = on ? x[orow_p + oq[k]] : 0.0f;
+            o_l[k] = on && lum ? lum[orow_p + oq[k]] : 0.0f;
+        }
+
+        // ---------------- phase 1: reads and MFMAs
+        // A: a1 row r (one MFMA per class, K = taps; as k_conv_in); operands of a1 row r + 1
+        uint2 p_in[FS_CPW];
         if (PASS == 0) {
-            // ---- a1 row r into ring 0 (one MFMA per class, K = taps; as
-            // k_conv_in) from the operands built in the step before.  The slot
-            // held row r - 3, last read before the previous step's first
-            // barrier, and this step's barrier publishes the row.
-            uint16_t* dst = RING(0, slot(0));
-            const bool row_in = r >= 0 && r < H;
             bf16x8 bv[FS_CPW];
             f32x4 a1[FS_CPW];
 #pragma unroll
-            for (int q = 0; q < FS_CPW; q++) bv[q] = lds_at<bf16x8>(im, im_off + q * 256);
+            for (int q = 0; q < FS_CPW; q++) bv[q] = lds_at<bf16x8>(im, slot(0) * IM_SLOT + im_off + q * 256);
 #pragma unroll
             for (int q = 0; q < FS_CPW; q++) a1[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(win, bv[q], f32x4{0, 0, 0, 0}, 0, 0, 0);
-#pragma unroll
-            for (int q = 0; q < FS_CPW; q++) {
-                const f32x4 a = a1[q];
-                const int j = FS_CPW * half + q + FS_NC * ln;
-                const int gx = gx0 + j;
-                const uint32_t keep = (row_in && gx >= 0 && gx < Wd) ? 0xffffffffu : 0u;
-                uint2 o;
-                o.x = fs_pack(a[0], bin[0], a[1], bin[1]) & keep;
-                o.y = fs_pack(a[2], bin[2], a[3], bin[3]) & keep;
-                *reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(dst) + lo.st + q * 1024) = o;
-            }
+            fs_pack_row(a1, bin, r >= 0 && r < H, col_in, p_in);
+            build_im(slot(1), slot(0), slot(1), slot(2));
         }
-        f32x4 acc[FS_CPW];
-#pragma unroll
-        for (int q = 0; q < FS_CPW; q++) acc[q] = f32x4{0, 0, 0, 0};
-        fs_mma_row(acc, wa[0], 0, src_row(0, 0), lo);
-        fs_mma_row(acc, wa[0], 1, src_row(0, 1), lo);
-        __syncthreads();
-        // A: operands of a1 row r + 1 from x rows r .. r + 2 (row r + 2 was
-        // committed at the start of this step); the next reads of them follow
-        // this step's layer barriers
-        build_im(slot(0), slot(1), slot(2));
-        // ---- three MFMA layers: ring k -> ring k+1 (A: rows r-1, r-2, r-3; B: rows r, r-1, r-2)
-        // B: output row r - 3 (as k_conv_out): E_kx of window class i in row kx of a
-        // three-MFMA chain over ky, then y = x + (((E_0(col - 1) + E_1(col)) + E_2(col + 1)) + b_out);
-        // of the wave half's classes, the mt = 0 wave takes the first FS_CPW - 1 (window
-        // classes 0 .. FS_CPW), the mt = 1 wave the last (window classes FS_CPW - 1 .. FS_CPW + 1)
-        f32x4 e[FS_WIN];
-        auto out_row = [&](int ky) {
-            const uint16_t* src = RING(3, slot(-4 + ky));
-#pragma unroll
-            for (int i = 0; i < FS_WIN; i++)
-                if ((mt == 0 && i <= FS_CPW) || (mt == 1 && i >= FS_CPW - 1))
-                    e[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wout[ky], lds_at<bf16x8>(src, lo.b[i]), e[i], 0, 0, 0);
-        };
+        // middle layers: layer l reads ring l rows row_l[l] - 1 .. row_l[l] + 1
+        uint2 p_mid[3][FS_CPW];
 #pragma unroll
         for (int l = 0; l < 3; l++) {
-            const int orow = r + BASE - l;
-            const bool row_in = orow >= 0 && orow < H;
-            const bool on = !(PASS == 0 && l == 2) || a4_row;
-            if (on) fs_mma_row(acc, wa[l], 2, src_row(l, 2), lo);
-            f32x4 nxt[FS_CPW];
+            const int c0 = row_l[l] - 1 - r;  // source rows r + c0 .. r + c0 + 2
+            f32x4 acc[FS_CPW];
 #pragma unroll
-            for (int q = 0; q < FS_CPW; q++) nxt[q] = f32x4{0, 0, 0, 0};
-            if (PASS == 0 && l == 2) {
-                // a4: stored columns and rows of this block go to HBM (next step)
-                ov_pend = on && row_in;
-                if (ov_pend) {
+            for (int q = 0; q < FS_CPW; q++) acc[q] = f32x4{0, 0, 0, 0};
 #pragma unroll
-                    for (int q = 0; q < FS_CPW; q++) {
-                        a4v[q].x = fs_pack(acc[q][0], bm[l][0], acc[q][1], bm[l][1]);
-                        a4v[q].y = fs_pack(acc[q][2], bm[l][2], acc[q][3], bm[l][3]);
-                    }
-                    ov_p = (ioff + (size_t)orow * Wd) * C;
-                }
-            } else {
-                fs_store<false>(acc, RING(l + 1, slot(BASE - l)), bm[l], lane, mt, half, lo, row_in, gx0, Wd, 0, 0);
-                if (l < 2) {
-                    if (l + 1 < 2 || PASS == 1 || a4_row) {
-                        fs_mma_row(nxt, wa[l + 1], 0, src_row(l + 1, 0), lo);
-                        fs_mma_row(nxt, wa[l + 1], 1, src_row(l + 1, 1), lo);
-                    }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < FS_WIN; i++) e[i] = f32x4{0, 0, 0, 0};
-                    out_row(0);
-                    out_row(1);
-                }
-                __syncthreads();
-            }
-#pragma unroll
-            for (int q = 0; q < FS_CPW; q++) acc[q] = nxt[q];
+            for (int ky = 0; ky < 3; ky++) fs_mma_row(acc, wa[l], ky, RING(l, slot(c0 + ky)), lo);
+            fs_pack_row(acc, bm[l], row_l[l] >= 0 && row_l[l] < H, col_in, p_mid[l]);
         }
+        // B: output row r - 7 (as k_conv_out): E_kx of window class i in row kx of a
+        // three-MFMA chain over ky (a7 rows r - 8 .. r - 6), then
+        // y = x + (((E_0(col - 1) + E_1(col)) + E_2(col + 1)) + b_out), for the wave's
+        // output classes from its window classes k0 .. k0 + FS_CPW
+        float ov[NQ];
         if (PASS == 1) {
-            out_row(2);
+            f32x4 e[FS_CPW + 1];
 #pragma unroll
-            for (int q = 0; q < FS_CPW; q++) {
-                if (o_on[q]) {
-                    const float y = o_x[q] + (((e[q][0] + e[q + 1][1]) + e[q + 2][2]) + b_out);
-                    ov[q] = lum ? o_l[q] + s_lum * (y - o_x[q]) : y;
-                }
+            for (int k = 0; k < FS_CPW + 1; k++) e[k] = f32x4{0, 0, 0, 0};
+#pragma unroll
+            for (int ky = 0; ky < 3; ky++) {
+                const uint16_t* src = RING(3, slot(-8 + ky));
+#pragma unroll
+                for (int k = 0; k < FS_CPW + 1; k++)
+                    e[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wout[ky], lds_at<bf16x8>(src, ob[k]), e[k], 0, 0, 0);
             }
-            ov_pend = orow_on;
-            ov_p = orow_p;
+#pragma unroll
+            for (int k = 0; k < NQ; k++) {
+                // window classes of output class oq[k]: k .. k + 2 (mt = 0), 0 .. 2 (mt = 1)
+                const float e0 = mt ? e[0][0] : e[k][0], e1 = mt ? e[1][1] : e[k + 1][1], e2 = mt ? e[2][2] : e[k + 2][2];
+                const float y = o_x[k] + (((e0 + e1) + e2) + b_out);
+                ov[k] = lum ? o_l[k] + s_lum * (y - o_x[k]) : y;
+            }
         }
+        __syncthreads();
+
+        // ---------------- phase 2: writes (each into the slot of the row three above, read in phase 1)
+        if (PASS == 0) fs_put_row(p_in, RING(0, slot(0)), lo);
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+            if (PASS == 0 && l == 2) continue;  // A: a4 goes to HBM
+            fs_put_row(p_mid[l], RING(l + 1, slot(row_l[l] - r)), lo);
+        }
+        if (PASS == 0) {
+            commit(slot(0));  // x row r + 3
+            fetch(r + 4);
+            if (on_l[2] && row_l[2] < H) {
+                uint16_t* p = act_out + (ioff + (size_t)row_l[2] * Wd + gx0 + o_j0) * C + mt * 16 + 4 * g;
+#pragma unroll
+                for (int q = 0; q < FS_CPW; q++)
+                    if (o_col[q]) *reinterpret_cast<uint2*>(p + q * C) = p_mid[2][q];
+            }
+        } else {
+            commit(slot(1));  // a4 row r + 1
+            fetch(r + 2);
+            if (orow_on) {
+#pragma unroll
+                for (int k = 0; k < NQ; k++)
+                    if (oq_on[k]) out[orow_p + oq[k]] = ov[k];
+            }
+        }
+        __syncthreads();
     };
     for (int rb = r_first; rb < r_end; rb += 3) {
         step(std::integral_constant<int, 0>{}, rb);
         if (rb + 1 < r_end) step(std::integral_constant<int, 1>{}, rb + 1);
         if (rb + 2 < r_end) step(std::integral_constant<int, 2>{}, rb + 2);
     }
-    flush_out();
 }
 
 unsigned nb(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
