@@ -48,7 +48,17 @@ constexpr int NLM_NARROW_SP = 13; // halo limit of the narrow instance (search 1
 // second barrier per offset, fits three tiles per CU instead of two.
 constexpr int NLM_PW_WIDE = NLM_TW + 2 * NLM_MAX_SP + 1;
 constexpr int NLM_PW_NARROW = NLM_TW + 2 * NLM_NARROW_SP + 1;
-__host__ __device__ constexpr int nlm_th(int P) { return NLM_HR - 2 * P; }
+// Tile rows.  Phase 1 forms NLM_HR rows of horizontal sums per tile (TH + 2P
+// of them used), phase 2 slides down NLM_NG * ceil(TH / NLM_NG) rows (TH of
+// them stored).  Of TH = NLM_HR - 2P and that rounded down to a multiple of
+// NLM_NG, the one that forms fewer rows per stored row.
+__host__ __device__ constexpr int nlm_rows_formed(int th) { return NLM_HR + NLM_NG * ((th + NLM_NG - 1) / NLM_NG); }
+__host__ __device__ constexpr int nlm_th(int P) {
+    return nlm_rows_formed((NLM_HR - 2 * P) / NLM_NG * NLM_NG) * (NLM_HR - 2 * P) < nlm_rows_formed(NLM_HR - 2 * P) * ((NLM_HR - 2 * P) / NLM_NG * NLM_NG)
+               ? (NLM_HR - 2 * P) / NLM_NG * NLM_NG
+               : NLM_HR - 2 * P;
+}
+static_assert(nlm_th(3) == 56 && nlm_th(2) == 60 && nlm_th(1) == 62, "tile rows per patch radius");
 // phase-2 rows per thread: the tile rows split over NLM_NG row groups
 __host__ __device__ constexpr int nlm_rpt(int P) { return (nlm_th(P) + NLM_NG - 1) / NLM_NG; }
 // sum rows incl. padding read by the last row group (which runs past the
