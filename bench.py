@@ -32,7 +32,7 @@ NLM_FLOP_PER_PX = 441 * 12      # per (pixel, offset): diff 2, two window update
 DEMOSAIC_BYTES_PER_PX = 18      # u16 in + f32 RGB out + f32 clip out
 # HBM bytes per pixel that k_nlm_tile moves per launch, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE runs calibrated on a 1 GiB copy (profiles/hbm_traffic_counters.md)
-NLM_TRAFFIC_BYTES_PER_PX = 14.06
+NLM_TRAFFIC_BYTES_PER_PX = 14.20
 MHC_BYTES_PER_PX = 14
 # reconstruct_highlights, algorithmic HBM bytes per pixel when something is
 # clipped: flag scan 4, near-clip box 16, weights 32, three-box gaussian of
