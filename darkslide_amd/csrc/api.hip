@@ -614,12 +614,23 @@ int denoise_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float amount, l
     // enhance.rs:82-88
     const float k = amount / 100.0f;
     NlmH hp{0.0f, sig, 0.6f + 1.6f * k};
-    rc = ref_threads > 0 ? nlm_ref_dev(c, L, w, h, hp, search, patch, ref_threads, LF) : nlm_dev(c, L, w, h, hp, search, patch, LF);
-    if (rc) return rc;
     const float m = 0.5f + 0.5f * k;
+    // The tile NLM forms the guided pair's guide planes (the first four of
+    // ws) as it stores each result; the reference-order NLM, negative search
+    // radii and the direct kernel leave them to the guided pair.
+    bool guide_ready = false;
+    if (ref_threads > 0 || search < 0) {
+        rc = ref_threads > 0 ? nlm_ref_dev(c, L, w, h, hp, search, patch, ref_threads, LF) : nlm_dev(c, L, w, h, hp, search, patch, LF);
+    } else {
+        if (patch > 64) return fail(c, DS_ERR_INVALID, "patch radius %zu too large", patch);
+        if (search > 1024) return fail(c, DS_ERR_INVALID, "search radius %ld too large", search);
+        const NlmGuide g{L, A, B, m, ws, ws + n, ws + 2 * n, ws + 3 * n};
+        rc = run(c, "nlm", [&] { return launch_nlm_guide(L, LF, w, h, hp, search, patch, g, &guide_ready, c->stream); });
+    }
+    if (rc) return rc;
     float rf = std::fmax(((float)(w > h ? w : h) / 600.0f) * (1.0f + 2.0f * k), 2.0f);
     size_t r = (size_t)rf;
-    return run(c, "guided_filter", [&] { return launch_guided_pair(L, LF, A, B, m, r, sig, c->ok, ws, out, w, h, c->stream); });
+    return run(c, "guided_filter", [&] { return launch_guided_pair(L, LF, A, B, m, r, sig, c->ok, ws, out, w, h, c->stream, guide_ready); });
 }
 
 int raw_details_dev(ds_ctx* c, const float* rgb, size_t w, size_t h, float sigma, size_t iters, float* out) {

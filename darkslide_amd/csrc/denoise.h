@@ -30,8 +30,10 @@ hipError_t launch_estimate_noise(const float* plane, size_t w, size_t h, uint32_
 // The two guided filters of denoise (enhance.rs:86-91) and the inverse Oklab:
 // I = L + (LF - L) m; a2 = gf(I, A), b2 = gf(I, B) with eps = (2 max(sigma,1e-4))^2;
 // rgb = max(Oklab^-1(I, a2, b2), 0).  ws: 12 planes of w*h floats.  LF is
-// overwritten (a spare plane once the guide is formed).
+// overwritten (a spare plane once the guide is formed).  guide_ready: the
+// first four planes of ws already hold I, I A, I B and I I (formed by
+// launch_nlm_guide), and LF is only a spare plane.
 hipError_t launch_guided_pair(const float* L, float* LF, const float* A, const float* B, float m, size_t r, const float* sigma_dev,
-                              const OklabM& ok, float* ws, float* rgb, size_t w, size_t h, hipStream_t st);
+                              const OklabM& ok, float* ws, float* rgb, size_t w, size_t h, hipStream_t st, bool guide_ready = false);
 
 }  // namespace ds

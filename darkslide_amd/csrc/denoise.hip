@@ -341,7 +341,7 @@ hipError_t launch_estimate_noise(const float* plane, size_t w, size_t h, uint32_
 // ws: 12 planes of w*h floats.  LF is dead once the guide is formed and
 // serves as a spare plane.
 hipError_t launch_guided_pair(const float* L, float* LF, const float* A, const float* B, float m, size_t r, const float* sigma_dev,
-                              const OklabM& ok, float* ws, float* rgb, size_t w, size_t h, hipStream_t st) {
+                              const OklabM& ok, float* ws, float* rgb, size_t w, size_t h, hipStream_t st, bool guide_ready) {
     const size_t n = w * h;
     if (n == 0) return hipSuccess;
     float* t[12];
@@ -359,7 +359,7 @@ hipError_t launch_guided_pair(const float* L, float* LF, const float* A, const f
     const unsigned nb = (unsigned)((n + 255) / 256);
     // the guide I = l2 and its products (image.rs:386-391) -> t0..t3; I
     // stays in t0 for the final step
-    hipLaunchKernelGGL(k_gf_inputs, dim3(nb), dim3(256), 0, st, L, LF, A, B, m, n, t[0], t[1], t[2], t[3]);
+    if (!guide_ready) hipLaunchKernelGGL(k_gf_inputs, dim3(nb), dim3(256), 0, st, L, LF, A, B, m, n, t[0], t[1], t[2], t[3]);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // means of I, a, b, I a, I b, I I (horizontal pass into t4, t5, t8..t11 first)

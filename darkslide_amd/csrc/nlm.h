@@ -40,6 +40,26 @@ size_t nlm_spot_scratch_floats(size_t w, size_t h);
 // nlm(p, h, search, patch); search >= 0.
 hipError_t launch_nlm(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, hipStream_t stream);
 
+// The denoise composite's guide planes formed from the NLM result lf of L
+// (enhance.rs:86-89, image.rs:386-391): I = L + (lf - L) m, I A, I B and I I,
+// each rounded as a plain FP32 expression, as k_gf_inputs forms them.
+struct NlmGuide {
+    const float* L;
+    const float* A;
+    const float* B;
+    float m;
+    float* I;
+    float* IA;
+    float* IB;
+    float* II;
+};
+// NLM of L whose output stage stores the guide planes g instead of lf: the
+// tile kernel forms them from each result as it stores it (*fused = true,
+// out untouched).  Where the tile kernel does not take (search, patch), lf
+// goes to out and *fused is false.
+hipError_t launch_nlm_guide(const float* L, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, const NlmGuide& g, bool* fused,
+                            hipStream_t stream);
+
 // NLM over the pixels of the flagged regions only (regions[r] != 0); out
 // keeps its values elsewhere.  regions == nullptr: every pixel.
 hipError_t launch_nlm_regions(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, const uint8_t* regions,

@@ -72,6 +72,18 @@ __host__ __device__ inline int nlm_pt_rows(int SP, int P) {
 }
 __host__ __device__ inline int nlm_pt_floats(int SP, int P, int PW) { return (nlm_pt_rows(SP, P) * PW + 3) & ~3; }
 
+// The guide planes at pixel `at` from its NLM result v, each product rounded
+// on its own (no contraction), as k_gf_inputs rounds them.
+__device__ __forceinline__ void store_guide(const NlmGuide& g, size_t at, float v) {
+#pragma clang fp contract(off)
+    const float o = g.L[at];
+    const float I = o + (v - o) * g.m;
+    g.I[at] = I;
+    g.IA[at] = I * g.A[at];
+    g.IB[at] = I * g.B[at];
+    g.II[at] = I * I;
+}
+
 // Tile kernel.  Phase 1: thread t owns hs row (t % 64) and the column
 // segments t / 64 + NSG k, k < NTASK (TW / (NSG SEG) tasks per thread, NSG =
 // NT / 64).  The centre samples of its segments do not depend on the offset
@@ -79,7 +91,8 @@ __host__ __device__ inline int nlm_pt_floats(int SP, int P, int PW) { return (nl
 // column t % 64 and one of NLM_NG groups of tile rows.
 template <int P, int PW, int HSB>
 __global__ __launch_bounds__(NLM_NT, HSB == 1 ? 6 : 2) void k_nlm_tile(const float* __restrict__ in, float* __restrict__ out, int w, int h, int S,
-                                                              NlmWeight wp, const uint8_t* __restrict__ regions, TileOrder order) {
+                                                              NlmWeight wp, const uint8_t* __restrict__ regions, TileOrder order,
+                                                              NlmGuide guide) {
     extern __shared__ float lds[];
     constexpr int TW = NLM_TW, TH = nlm_th(P), NT = NLM_NT, SEG = NLM_SEG, HS = NLM_HS;
     constexpr int DW = SEG + 2 * P;           // differences per phase-1 task
@@ -240,8 +253,13 @@ __global__ __launch_bounds__(NLM_NT, HSB == 1 ? 6 : 2) void k_nlm_tile(const flo
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
         const int gx = ox + col, gy = oy + row0 + k;
-        if (row0 + k < TH && gx < w && gy < h && (!regions || regions[(gy / NLM_REGION) * nrx + gx / NLM_REGION]))
-            out[(size_t)gy * w + gx] = (acc[k] / cs) / fmaxf(wsum[k], 1e-12f);
+        if (row0 + k < TH && gx < w && gy < h && (!regions || regions[(gy / NLM_REGION) * nrx + gx / NLM_REGION])) {
+            const float v = (acc[k] / cs) / fmaxf(wsum[k], 1e-12f);
+            if (guide.I)
+                store_guide(guide, (size_t)gy * w + gx, v);
+            else
+                out[(size_t)gy * w + gx] = v;
+        }
     }
 }
 
@@ -498,7 +516,8 @@ __global__ __launch_bounds__(64) void k_nlm_spot_final(const float* __restrict__
 }
 
 template <int P, int PW, int HSB>
-hipError_t launch_tile_pw(const float* in, float* out, int w, int h, int S, NlmWeight wp, const uint8_t* regions, hipStream_t stream) {
+hipError_t launch_tile_pw(const float* in, float* out, int w, int h, int S, NlmWeight wp, const uint8_t* regions, const NlmGuide& guide,
+                          hipStream_t stream) {
     const size_t lds = (size_t)(nlm_pt_floats(S + P, P, PW) + HSB * NLM_HSR * NLM_HS) * sizeof(float);
     // interior tiles: ox >= S + P, oy >= S + P, ox + TW + S + P <= w, oy + TH + S + P <= h
     const TileOrder order = tile_order(w, h, NLM_TW, nlm_th(P), S + P, S + P);
@@ -511,14 +530,15 @@ hipError_t launch_tile_pw(const float* in, float* out, int w, int h, int S, NlmW
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_nlm_tile<P, PW, HSB>), grid, dim3(NLM_NT), lds, stream, in, out, w, h, S, wp, regions, order);
+    hipLaunchKernelGGL((k_nlm_tile<P, PW, HSB>), grid, dim3(NLM_NT), lds, stream, in, out, w, h, S, wp, regions, order, guide);
     return hipGetLastError();
 }
 
 template <int P>
-hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeight wp, const uint8_t* regions, hipStream_t stream) {
-    if (S + P <= NLM_NARROW_SP) return launch_tile_pw<P, NLM_PW_NARROW, 1>(in, out, w, h, S, wp, regions, stream);
-    return launch_tile_pw<P, NLM_PW_WIDE, 2>(in, out, w, h, S, wp, regions, stream);
+hipError_t launch_tile(const float* in, float* out, int w, int h, int S, NlmWeight wp, const uint8_t* regions, const NlmGuide& guide,
+                       hipStream_t stream) {
+    if (S + P <= NLM_NARROW_SP) return launch_tile_pw<P, NLM_PW_NARROW, 1>(in, out, w, h, S, wp, regions, guide, stream);
+    return launch_tile_pw<P, NLM_PW_WIDE, 2>(in, out, w, h, S, wp, regions, guide, stream);
 }
 
 }  // namespace
@@ -541,27 +561,44 @@ NlmWeight make_weight(NlmH hp, int P, double& norm) {
 }
 }  // namespace
 
-hipError_t launch_nlm_regions(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, const uint8_t* regions,
-                              hipStream_t stream) {
+namespace {
+// (search, patch) the tile kernel takes; the direct kernel takes the rest
+bool nlm_tiled(long search, size_t patch) { return search + (long)patch <= NLM_MAX_SP && patch <= 5; }
+
+// NLM through the tile kernel (with its optional guide output stage) or the
+// direct kernel (no guide).
+hipError_t run_nlm(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, const uint8_t* regions,
+                   const NlmGuide& guide, hipStream_t stream) {
     if (w == 0 || h == 0) return hipSuccess;
     const int S = (int)search;
     const int P = (int)patch;
     double norm;
     const NlmWeight wp = make_weight(hp, P, norm);
-    if (S + P <= NLM_MAX_SP) {
+    if (nlm_tiled(search, patch)) {
         switch (P) {
-            case 0: return launch_tile<0>(in, out, (int)w, (int)h, S, wp, regions, stream);
-            case 1: return launch_tile<1>(in, out, (int)w, (int)h, S, wp, regions, stream);
-            case 2: return launch_tile<2>(in, out, (int)w, (int)h, S, wp, regions, stream);
-            case 3: return launch_tile<3>(in, out, (int)w, (int)h, S, wp, regions, stream);
-            case 4: return launch_tile<4>(in, out, (int)w, (int)h, S, wp, regions, stream);
-            case 5: return launch_tile<5>(in, out, (int)w, (int)h, S, wp, regions, stream);
-            default: break;
+            case 0: return launch_tile<0>(in, out, (int)w, (int)h, S, wp, regions, guide, stream);
+            case 1: return launch_tile<1>(in, out, (int)w, (int)h, S, wp, regions, guide, stream);
+            case 2: return launch_tile<2>(in, out, (int)w, (int)h, S, wp, regions, guide, stream);
+            case 3: return launch_tile<3>(in, out, (int)w, (int)h, S, wp, regions, guide, stream);
+            case 4: return launch_tile<4>(in, out, (int)w, (int)h, S, wp, regions, guide, stream);
+            default: return launch_tile<5>(in, out, (int)w, (int)h, S, wp, regions, guide, stream);
         }
     }
     dim3 grid((unsigned)((w + 255) / 256), (unsigned)h);
     hipLaunchKernelGGL(k_nlm_direct, grid, dim3(256), 0, stream, in, out, (int)w, (int)h, S, P, wp, (float)norm, regions);
     return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_nlm_regions(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, const uint8_t* regions,
+                              hipStream_t stream) {
+    return run_nlm(in, out, w, h, hp, search, patch, regions, NlmGuide{}, stream);
+}
+
+hipError_t launch_nlm_guide(const float* L, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, const NlmGuide& g, bool* fused,
+                            hipStream_t stream) {
+    *fused = nlm_tiled(search, patch);
+    return run_nlm(L, out, w, h, hp, search, patch, nullptr, *fused ? g : NlmGuide{}, stream);
 }
 
 hipError_t launch_nlm(const float* in, float* out, size_t w, size_t h, NlmH hp, long search, size_t patch, hipStream_t stream) {
