@@ -77,14 +77,17 @@ __global__ void k_rl_init(const float* __restrict__ rgb, size_t n, float* __rest
 }
 
 // out = p * clamp(est / y, 0.6, 1.6) (enhance.rs:113-115)
+__device__ __forceinline__ void apply_px(const float* __restrict__ rgb, float est, float y, size_t i, float* __restrict__ out) {
+    float r = clampr(est / y, 0.6f, 1.6f);
+    out[3 * i + 0] = rgb[3 * i + 0] * r;
+    out[3 * i + 1] = rgb[3 * i + 1] * r;
+    out[3 * i + 2] = rgb[3 * i + 2] * r;
+}
 __global__ void k_rl_apply(const float* __restrict__ rgb, const float* __restrict__ est, const float* __restrict__ y, size_t n,
                            float* __restrict__ out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float r = clampr(est[i] / y[i], 0.6f, 1.6f);
-    out[3 * i + 0] = rgb[3 * i + 0] * r;
-    out[3 * i + 1] = rgb[3 * i + 1] * r;
-    out[3 * i + 2] = rgb[3 * i + 2] * r;
+    apply_px(rgb, est[i], y[i], i, out);
 }
 
 // Interior staging: the ROWS x COLS block of a plane at g (row stride w)
@@ -117,13 +120,17 @@ __device__ __forceinline__ void stage_rows(const float* __restrict__ g, int w, f
         }
 }
 
-// FIRST: the call's first iteration, with the plain division; it checks
-// every y it reads and sets *flag if one is outside [1e-6, 2^64].  Later
-// iterations (FIRST false) read *flag: clear, the interior tiles divide with
+// MODE bit 0 (FIRST): the call's first iteration, with the plain division;
+// it checks every y it reads and sets *flag if one is outside [1e-6, 2^64].
+// Later iterations read *flag: clear, the interior tiles divide with
 // div_in_range (est is then in [0, 64], so conv is in [0, 64.01]).
-template <int R, int TY, int NT, bool FIRST>
+// MODE bit 1 (LAST): the call's last iteration, which applies its est to rgb
+// as k_rl_apply does (into rgb_out) instead of storing est.
+template <int R, int TY, int NT, int MODE>
 __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, const float* __restrict__ yp, float* __restrict__ est_out,
-                                                   int w, int h, Taps taps, TileOrder order, int* __restrict__ flag) {
+                                                   int w, int h, Taps taps, TileOrder order, int* __restrict__ flag,
+                                                   const float* __restrict__ rgb, float* __restrict__ rgb_out) {
+    constexpr bool FIRST = (MODE & 1) != 0, LAST = (MODE & 2) != 0;
     static_assert(TY % 8 == 0, "row blocks of 8");
     constexpr int TX = RL_TX, K = 2 * R + 1, B = 8;
     // est: [-2R, T+2R) both ways; y, ratio: [-R, T+R); tmp: cols [-R, TX+R), rows [-2R, TY+2R)
@@ -277,14 +284,22 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
             float x[B + 2 * R];
 #pragma unroll
             for (int i = 0; i < B + 2 * R; i++) x[i] = src[i * QS];
-            float* o = est_out + (size_t)(oy + v0) * (size_t)w + (ox + u);
+            const size_t p0 = (size_t)(oy + v0) * (size_t)w + (ox + u);
+            float yl[LAST ? B : 1];
+            if (LAST) {
+#pragma unroll
+                for (int j = 0; j < B; j++) yl[j] = yp[p0 + (size_t)j * w];
+            }
 #pragma unroll
             for (int j = 0; j < B; j++) {
                 float acc = kSumStart;
 #pragma unroll
                 for (int i = 0; i < K; i++) acc += kk[i] * x[j + i];
-                *o = clamp64(ev[q * B + j] * acc);
-                o += w;
+                const float e = clamp64(ev[q * B + j] * acc);
+                if (LAST)
+                    apply_px(rgb, e, yl[j], p0 + (size_t)j * w, rgb_out);
+                else
+                    est_out[p0 + (size_t)j * w] = e;
             }
         }
         if (FIRST && !y_ok) *flag = 1;
@@ -340,7 +355,11 @@ __global__ __launch_bounds__(NT) void k_rl_iter(const float* __restrict__ est, c
         float acc = kSumStart;
 #pragma unroll
         for (int j = 0; j < K; j++) acc += kk[j] * s_tmp[(v + j - R + R) * QS + u];
-        est_out[(size_t)gy * w + gx] = clamp64(ev[k] * acc);
+        const size_t p = (size_t)gy * w + gx;
+        if (LAST)
+            apply_px(rgb, clamp64(ev[k] * acc), yp[p], p, rgb_out);
+        else
+            est_out[p] = clamp64(ev[k] * acc);
     }
 }
 
@@ -354,16 +373,22 @@ __global__ void k_rl_update(float* __restrict__ est, const float* __restrict__ c
     if (i < n) est[i] = clampr(est[i] * corr[i], 0.0f, 64.0f);
 }
 
+// mode: bit 0 the call's first iteration, bit 1 its last (rgb -> rgb_out)
 template <int R, int TY, int NT>
-hipError_t launch_iter_ty(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, bool first, int* flag,
-                          hipStream_t st) {
+hipError_t launch_iter_ty(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, int mode, int* flag,
+                          const float* rgb, float* rgb_out, hipStream_t st) {
     // interior tiles: ox >= 2R, oy >= 2R, ox + TX + 2R <= w, oy + TY + 2R <= h
     const TileOrder order = tile_order((int)w, (int)h, RL_TX, TY, 2 * R, 2 * R);
     dim3 grid((unsigned)(order.nx * order.ny));
-    if (first)
-        hipLaunchKernelGGL((k_rl_iter<R, TY, NT, true>), grid, dim3(NT), 0, st, est, y, est_out, (int)w, (int)h, t, order, flag);
-    else
-        hipLaunchKernelGGL((k_rl_iter<R, TY, NT, false>), grid, dim3(NT), 0, st, est, y, est_out, (int)w, (int)h, t, order, flag);
+#define DS_RL_ITER(M) \
+    hipLaunchKernelGGL((k_rl_iter<R, TY, NT, M>), grid, dim3(NT), 0, st, est, y, est_out, (int)w, (int)h, t, order, flag, rgb, rgb_out)
+    switch (mode & 3) {
+        case 0: DS_RL_ITER(0); break;
+        case 1: DS_RL_ITER(1); break;
+        case 2: DS_RL_ITER(2); break;
+        default: DS_RL_ITER(3); break;
+    }
+#undef DS_RL_ITER
     return hipGetLastError();
 }
 
@@ -372,12 +397,12 @@ hipError_t launch_iter_ty(const float* est, const float* y, float* est_out, size
 // 8 % less halo work than 64x32), 1 = 64x16, 2 = 64x32 (26 KB, six tiles of
 // 256 threads per CU).
 template <int R>
-hipError_t launch_iter(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, int variant, bool first, int* flag,
-                       hipStream_t st) {
+hipError_t launch_iter(const float* est, const float* y, float* est_out, size_t w, size_t h, const Taps& t, int variant, int mode, int* flag,
+                       const float* rgb, float* rgb_out, hipStream_t st) {
     switch (variant) {
-        case 1: return launch_iter_ty<R, 16, 256>(est, y, est_out, w, h, t, first, flag, st);
-        case 2: return launch_iter_ty<R, 32, 256>(est, y, est_out, w, h, t, first, flag, st);
-        default: return launch_iter_ty<R, 64, 512>(est, y, est_out, w, h, t, first, flag, st);
+        case 1: return launch_iter_ty<R, 16, 256>(est, y, est_out, w, h, t, mode, flag, rgb, rgb_out, st);
+        case 2: return launch_iter_ty<R, 32, 256>(est, y, est_out, w, h, t, mode, flag, rgb, rgb_out, st);
+        default: return launch_iter_ty<R, 64, 512>(est, y, est_out, w, h, t, mode, flag, rgb, rgb_out, st);
     }
 }
 
@@ -419,16 +444,19 @@ hipError_t launch_raw_details(const float* rgb, size_t w, size_t h, float sigma,
     if (exact) r = gaussian_taps(sigma, taps.k);
     for (size_t it = 0; it < iters; it++) {
         if (exact) {
+            // the last iteration applies its est to rgb itself (k_rl_apply's step)
+            const int mode = (it == 0 ? 1 : 0) | (it + 1 == iters ? 2 : 0);
             switch (r) {
-                case 1: e = launch_iter<1>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
-                case 2: e = launch_iter<2>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
-                case 3: e = launch_iter<3>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
-                case 4: e = launch_iter<4>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
-                case 5: e = launch_iter<5>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
-                case 6: e = launch_iter<6>(est, y, est2, w, h, taps, variant, it == 0, flag, st); break;
+                case 1: e = launch_iter<1>(est, y, est2, w, h, taps, variant, mode, flag, rgb, rgb_out, st); break;
+                case 2: e = launch_iter<2>(est, y, est2, w, h, taps, variant, mode, flag, rgb, rgb_out, st); break;
+                case 3: e = launch_iter<3>(est, y, est2, w, h, taps, variant, mode, flag, rgb, rgb_out, st); break;
+                case 4: e = launch_iter<4>(est, y, est2, w, h, taps, variant, mode, flag, rgb, rgb_out, st); break;
+                case 5: e = launch_iter<5>(est, y, est2, w, h, taps, variant, mode, flag, rgb, rgb_out, st); break;
+                case 6: e = launch_iter<6>(est, y, est2, w, h, taps, variant, mode, flag, rgb, rgb_out, st); break;
                 default: return hipErrorInvalidValue;
             }
             if (e != hipSuccess) return e;
+            if (it + 1 == iters) return hipSuccess;
             float* t = est;
             est = est2;
             est2 = t;
