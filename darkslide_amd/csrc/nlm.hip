@@ -42,6 +42,7 @@ constexpr int NLM_NG = NLM_NT / NLM_TW;  // phase-2 row groups (threads per colu
 constexpr int NLM_HS = NLM_TW + 4;  // padded row stride of the horizontal-sum buffer
 constexpr int NLM_MAX_SP = 24;    // halo limit of the tile kernel
 constexpr int NLM_NARROW_SP = 13; // halo limit of the narrow instance (search 10, patch 3)
+constexpr int NLM_CHUNK = 7;      // offsets per shifted-sample window (interior tiles, narrow instance)
 // Input tile row stride: fixed per instance (so every LDS offset inside the
 // offset loop is an immediate) and odd (lanes walking down one column hit
 // distinct banks).  The narrow instance, with one horizontal-sum buffer and a
@@ -163,12 +164,93 @@ __global__ __launch_bounds__(NLM_NT, HSB == 1 ? 6 : 2) void k_nlm_tile(const flo
         wsum[k] = 0.0f;
     }
 
+    // phase 1 of one task from its differences d (not squared): the sliding
+    // window sums, as the reference's box blur slides them, with each square
+    // folded into a fused multiply-add: the entering square is added and the
+    // leaving one subtracted unrounded, two operations per step instead of a
+    // square, a difference and an add (a prefix-sum form saves an operation
+    // per sum but moves the denoise composite outside the reference's own
+    // spread)
+    auto hsums = [&](const float (&d)[DW], float* hs, int u0) {
+        float hsum[SEG];
+        float s = d[0] * d[0];
+#pragma unroll
+        for (int i = 1; i < 2 * P + 1; i++) s = __builtin_fmaf(d[i], d[i], s);
+#pragma unroll
+        for (int j = 0; j < SEG; j++) {
+            hsum[j] = s;
+            if (j + 1 < SEG) {
+                s = __builtin_fmaf(d[j + 2 * P + 1], d[j + 2 * P + 1], s);
+                s = __builtin_fmaf(-d[j], d[j], s);
+            }
+        }
+        float4* dst = reinterpret_cast<float4*>(hs + (v1 + P) * HS + u0);
+#pragma unroll
+        for (int q = 0; q < SEG / 4; q++) dst[q] = make_float4(hsum[4 * q], hsum[4 * q + 1], hsum[4 * q + 2], hsum[4 * q + 3]);
+    };
+    // phase 2 of the offset at doff (dy * PW + dx): vertical window sums,
+    // weights, accumulation
+    auto phase2 = [&](const float* hs, int doff) {
+        const float* hc = hs + row0 * HS + col;  // hs row r holds tile row r - P
+        const float* qp = pt + (row0 + SP) * PW + (col + SP) + doff;
+        // -0.0 + t == t for every t, so the first add folds away
+        float s = -0.0f;
+#pragma unroll
+        for (int j = 0; j < 2 * P + 1; j++) s += hc[j * HS];
+        // every row segment runs RPT rows (the last one past the tile reads
+        // padding rows of hs and is not stored), so the loop has no per-lane
+        // branches
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const float wt = __builtin_amdgcn_exp2f(-s);
+            acc[k] += wt * qp[k * PW];
+            wsum[k] += wt;
+            if (k + 1 < RPT) s += hc[(k + 2 * P + 1) * HS] - hc[k * HS];
+        }
+    };
+
     // The offset loop is instantiated once for interior tiles (no clamps)
     // and once for frame-edge tiles, so neither pays for the other's path.
     auto offsets = [&](auto in_tag) {
         constexpr bool IN = decltype(in_tag)::value;
         const int side = 2 * S + 1;
         const int noff = side * side;
+        if constexpr (IN && HSB == 1) {
+            // Interior tiles, rows of offsets that split into chunks of
+            // NLM_CHUNK: the shifted samples of a task for dx .. dx + CHUNK - 1
+            // are one window of DW + CHUNK - 1 samples, read once per chunk
+            // instead of DW per offset.  Same offsets, same order, same
+            // arithmetic as the loop below.
+            if (side % NLM_CHUNK == 0) {
+                int drow = -S * PW - S;  // dy * PW - S
+                for (int dy = -S; dy <= S; dy++, drow += PW) {
+                    for (int c0 = 0; c0 < side; c0 += NLM_CHUNK) {
+                        const int doff0 = drow + c0;
+                        float win[NTASK][DW + NLM_CHUNK - 1];
+#pragma unroll
+                        for (int task = 0; task < NTASK; task++) {
+                            const float* sp = pt + (cv + SP) * PW + (u00 + NSG * SEG * task - P + SP) + doff0;
+#pragma unroll
+                            for (int i = 0; i < DW + NLM_CHUNK - 1; i++) win[task][i] = sp[i];
+                        }
+#pragma unroll
+                        for (int j = 0; j < NLM_CHUNK; j++) {
+#pragma unroll
+                            for (int task = 0; task < NTASK; task++) {
+                                float d[DW];
+#pragma unroll
+                                for (int i = 0; i < DW; i++) d[i] = cc[task][i] - win[task][i + j];
+                                hsums(d, hs0, u00 + NSG * SEG * task);
+                            }
+                            __syncthreads();
+                            phase2(hs0, doff0 + j);
+                            __syncthreads();
+                        }
+                    }
+                }
+                return;
+            }
+        }
         int dx = -S, dy = -S;  // offsets in the reference's order: dy outer, dx inner
         int doff = -S * PW - S;  // dy * PW + dx, advanced with the offset
         for (int o = 0; o < noff; o++) {
@@ -191,49 +273,11 @@ __global__ __launch_bounds__(NLM_NT, HSB == 1 ? 6 : 2) void k_nlm_tile(const flo
                         d[i] = c[i] - P_at(su, sv);
                     }
                 }
-                // sliding window sums, as the reference's box blur slides
-                // them, with each square folded into a fused multiply-add: the
-                // entering square is added and the leaving one subtracted
-                // unrounded, two operations per step instead of a square, a
-                // difference and an add (a prefix-sum form saves an operation
-                // per sum but moves the denoise composite outside the
-                // reference's own spread)
-                float hsum[SEG];
-                float s = d[0] * d[0];
-#pragma unroll
-                for (int i = 1; i < 2 * P + 1; i++) s = __builtin_fmaf(d[i], d[i], s);
-#pragma unroll
-                for (int j = 0; j < SEG; j++) {
-                    hsum[j] = s;
-                    if (j + 1 < SEG) {
-                        s = __builtin_fmaf(d[j + 2 * P + 1], d[j + 2 * P + 1], s);
-                        s = __builtin_fmaf(-d[j], d[j], s);
-                    }
-                }
-                float4* dst = reinterpret_cast<float4*>(hs + (v1 + P) * HS + u0);
-#pragma unroll
-                for (int q = 0; q < SEG / 4; q++) dst[q] = make_float4(hsum[4 * q], hsum[4 * q + 1], hsum[4 * q + 2], hsum[4 * q + 3]);
+                hsums(d, hs, u0);
             }
             __syncthreads();
             // ---- phase 2: vertical window sums, weights, accumulation
-            {
-                const float* hc = hs + row0 * HS + col;  // hs row r holds tile row r - P
-                const float* qp = pt + (row0 + SP) * PW + (col + SP) + doff;
-                // -0.0 + t == t for every t, so the first add folds away
-                float s = -0.0f;
-#pragma unroll
-                for (int j = 0; j < 2 * P + 1; j++) s += hc[j * HS];
-                // every row segment runs RPT rows (the last one past the tile
-                // reads padding rows of hs and is not stored), so the loop has
-                // no per-lane branches
-#pragma unroll
-                for (int k = 0; k < RPT; k++) {
-                    const float wt = __builtin_amdgcn_exp2f(-s);
-                    acc[k] += wt * qp[k * PW];
-                    wsum[k] += wt;
-                    if (k + 1 < RPT) s += hc[(k + 2 * P + 1) * HS] - hc[k * HS];
-                }
-            }
+            phase2(hs, doff);
             // with two buffers the other one is written next, and the barrier
             // after that phase 1 orders it against this phase 2; with one,
             // phase 1 of the next offset waits here
