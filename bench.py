@@ -30,9 +30,11 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 NLM_FLOP_PER_PX = 441 * 12      # per (pixel, offset): diff 2, two window updates 4, scale 1, /h^2 1, exp 1, fma 2, add 1
 DEMOSAIC_BYTES_PER_PX = 18      # u16 in + f32 RGB out + f32 clip out
-# HBM bytes per pixel that k_nlm_tile moves per launch, from rocprofv3
-# FETCH_SIZE / WRITE_SIZE runs calibrated on a 1 GiB copy (profiles/hbm_traffic_counters.md)
-NLM_TRAFFIC_BYTES_PER_PX = 14.20
+# HBM bytes per pixel that k_nlm_tile moves per launch in denoise (its output
+# stage forms the guided pair's four guide planes from L, a and b), from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE runs calibrated on a 1 GiB copy
+# (profiles/hbm_traffic_counters.md)
+NLM_TRAFFIC_BYTES_PER_PX = 41.09
 MHC_BYTES_PER_PX = 14
 # reconstruct_highlights, algorithmic HBM bytes per pixel when something is
 # clipped: flag scan 4, near-clip box 16, weights 32, three-box gaussian of
@@ -693,7 +695,7 @@ def main():
         "frac": round(nlm_tflops / PEAK_FP32_TFLOPS, 4),
         "traffic": round(NLM_TRAFFIC_BYTES_PER_PX * n),
         "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (calibrated on a 1 GiB copy); "
-                        "algorithmic minimum 8 B/px",
+                        "algorithmic minimum 28 B/px (L, a, b in; the four guide planes out)",
         "algorithmic": f"{NLM_FLOP_PER_PX} flop/px x {n} px per launch",
         "dominant_stage": dominant,
     }
