@@ -492,7 +492,12 @@ __global__ __launch_bounds__(SPOT_WAVES * 64) void k_nlm_spot_lds(const float* _
 #pragma unroll
         for (int jj = 0; jj < 2 * P + 1; jj++)
 #pragma unroll
-            for (int ii = 0; ii < 2 * P + 1; ii++) cpat[jj][ii] = nb[(cys[jj] + SP) * D + cxs[ii] + SP];
+            for (int ii = 0; ii < 2 * P + 1; ii++) {
+                // the same in every lane (one lattice pixel per wave): held in
+                // scalar registers, so the vector ones go to the offsets' samples
+                const float v = nb[(cys[jj] + SP) * D + cxs[ii] + SP];
+                cpat[jj][ii] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+            }
         float acc = 0.0f, ws = 0.0f, w2 = 0.0f;
         for (int o = lane; o < noff; o += 64) {
             const int dy = o / side - S, dx = o % side - S;
