@@ -403,6 +403,7 @@ constexpr int SPOT_PER_WAVE = 4;
 constexpr int SPOT_BLOCKS = NLM_SPOT * NLM_SPOT / (SPOT_WAVES * SPOT_PER_WAVE);
 constexpr int SPOT_PARTS = SPOT_BLOCKS * SPOT_WAVES;  // (e^2, var) pairs per region
 constexpr int SPOT_MAX_SP = 24;
+constexpr int SPOT_ROLL = 7;  // dy offsets per lane in the rolling-window spot check (search 10: 3 x 21 lanes)
 constexpr int SPOT_PF = 12;  // prefetched samples per lane: neighbourhoods up to 27 x 27 (S + P <= 13)
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -499,6 +500,40 @@ __global__ __launch_bounds__(SPOT_WAVES * 64) void k_nlm_spot_lds(const float* _
                 cpat[jj][ii] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
             }
         float acc = 0.0f, ws = 0.0f, w2 = 0.0f;
+        if (interior && side == 3 * SPOT_ROLL) {
+            // Search 10: lane g * side + i takes dx = i - S and the SPOT_ROLL
+            // offsets dy = SPOT_ROLL g - S + t.  Going down one dy, the patch
+            // under the offset gains one row and loses one, so each step reads
+            // only the new row (2P + 1 samples) into a rolling register window
+            // instead of the whole patch.
+            if (lane < 3 * side) {
+                constexpr int PD = 2 * P + 1;
+                const int dx = lane % side - S, dy0 = SPOT_ROLL * (lane / side) - S;
+                const float* q0 = nb + (SP - P + dy0) * D + (SP - P + dx);
+                float win[PD + SPOT_ROLL - 1][PD];
+#pragma unroll
+                for (int r = 0; r < PD - 1; r++)
+#pragma unroll
+                    for (int ii = 0; ii < PD; ii++) win[r][ii] = q0[r * D + ii];
+#pragma unroll
+                for (int t = 0; t < SPOT_ROLL; t++) {
+#pragma unroll
+                    for (int ii = 0; ii < PD; ii++) win[t + PD - 1][ii] = q0[(t + PD - 1) * D + ii];
+                    float dist = 0.0f;
+#pragma unroll
+                    for (int jj = 0; jj < PD; jj++)
+#pragma unroll
+                        for (int ii = 0; ii < PD; ii++) {
+                            const float e = cpat[jj][ii] - win[t + jj][ii];
+                            dist = __builtin_fmaf(e, e, dist);
+                        }
+                    const float wt = __builtin_amdgcn_exp2f(dist * kexp);
+                    acc += wt * nb[(dy0 + t + SP) * D + dx + SP];
+                    ws += wt;
+                    w2 += wt * wt;
+                }
+            }
+        } else
         for (int o = lane; o < noff; o += 64) {
             const int dy = o / side - S, dx = o % side - S;
             float dist = 0.0f;
