@@ -358,6 +358,36 @@ def test_raw_details(ctx, oracle_mod, sigma, iters):
     assert err <= TOL
 
 
+def test_raw_details_first_and_last_launch(ctx, oracle_mod):
+    """Bit for bit against the oracle: one iteration (a launch that is both
+    the first and the last) and two, one iteration in place through the
+    device entry point, and y beyond the fast division's range (> 2^64) in
+    an interior and an edge tile of a 300x260 frame, after which the later
+    iterations must keep the IEEE division."""
+    import darkslide_amd as D
+    O = oracle_mod
+    w, h = 300, 260
+    rgb = rgb_frame(O, 501, w, h)
+    big = rgb.copy()
+    big[100, 100] = (3e19, 3e19, 3e19)
+    big[5, 290, 1] = 1e30
+    big[150, 150] = 0.0
+    for img, iters in ((rgb, 1), (rgb, 2), (big, 1), (big, 3)):
+        got = ctx.raw_details(img, 1.0, iters)
+        want = O.raw_details(img, 1.0, iters)
+        say(f"raw_details first launch: iters={iters} huge={img is big} bit-equal {np.mean(got == want):.6f}")
+        assert np.array_equal(got, want)
+    L = D.lib()
+    buf = ctx.alloc(w * h * 12)
+    try:
+        buf.upload(rgb)
+        ctx._check(L.ds_raw_details_dev(ctx.handle, buf.ptr, w, h, 1.0, 1, buf.ptr))
+        got = buf.download((h, w, 3), np.float32)
+    finally:
+        buf.free()
+    assert np.array_equal(got, O.raw_details(rgb, 1.0, 1))
+
+
 def test_pipeline_config1_with_fp64_error(ctx, oracle_mod):
     """Demosaic -> denoise (NLM 7x7 patch, 21x21 search, h = 0.6 sigma) ->
     RL (sigma 1, 20 iterations) at 1024x1024: each stage against the FP32
